@@ -1,0 +1,380 @@
+// Bandwidth-bound helpers: fills, precision conversions, the symmetric
+// (lower-stored) product used by the FP64 iterative-refinement residual
+// r = b - A*x, and small reductions for the convergence tests.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace elemamd {
+
+// ------------------------------------------------------------------ fills
+__global__ void fill_int_kernel(int* p, int value, int count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+hipError_t launch_fill_int(int* p, int value, int count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((unsigned)ceil_div(count, 256)), dim3(256), 0, s, p, value, count);
+    return hipGetLastError();
+}
+
+// splitmix64 counter stream + Box-Muller: entry t (column-major order over a
+// rows x cols matrix) is normal number offset+t of stream `seed`.  Pair p
+// uses stream outputs 2p and 2p+1.
+__device__ inline uint64_t sm64(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ inline double u01(uint64_t x) { return ((double)(x >> 11) + 1.0) * 0x1.0p-53; }
+
+__global__ void fill_normal_kernel(double* out, i64 rows, i64 cols, i64 ld, uint64_t seed, i64 offset) {
+    const i64 total = rows * cols;
+    for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
+        const i64 idx = offset + t;
+        const uint64_t p = (uint64_t)(idx >> 1);
+        const double u1 = u01(sm64(seed, 2 * p));
+        const double u2 = u01(sm64(seed, 2 * p + 1));
+        const double r = sqrt(-2.0 * log(u1));
+        const double a = 6.283185307179586476925286766559 * u2;
+        const double v = (idx & 1) ? r * sin(a) : r * cos(a);
+        const i64 i = t % rows, j = t / rows;
+        out[i + j * ld] = v;
+    }
+}
+
+hipError_t launch_fill_normal_f64(double* out, i64 rows, i64 cols, i64 ld, uint64_t seed, i64 offset,
+                                  hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    const i64 total = rows * cols;
+    const unsigned blocks = (unsigned)std::min<i64>(ceil_div(total, 256), 65536);
+    hipLaunchKernelGGL(fill_normal_kernel, dim3(blocks), dim3(256), 0, s, out, rows, cols, ld, seed, offset);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- elementwise 2-D
+__global__ void scale_f64_kernel(int m, int n, double alpha, double* A, i64 lda) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m) A[i + j * lda] *= alpha;
+}
+hipError_t launch_scale_f64(int m, int n, double alpha, double* A, i64 lda, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_f64_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s, m, n, alpha,
+                       A, lda);
+    return hipGetLastError();
+}
+
+__global__ void copy_lower_f64_to_f32_kernel(int n, const double* A, i64 lda, float* S, i64 lds) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < n && i >= j) S[i + j * lds] = (float)A[i + j * lda];
+}
+hipError_t launch_copy_lower_f64_to_f32(int n, const double* A, i64 lda, float* S, i64 lds, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_lower_f64_to_f32_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)n), dim3(256), 0, s,
+                       n, A, lda, S, lds);
+    return hipGetLastError();
+}
+
+__global__ void copy_f64_to_f32_scaled_kernel(int m, const double* X, i64 ldx, const double* cs, float* Y, i64 ldy) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m) {
+        const double sc = cs ? cs[j] : 1.0;
+        Y[i + j * ldy] = (float)(sc > 0.0 ? X[i + j * ldx] / sc : 0.0);
+    }
+}
+hipError_t launch_copy_f64_to_f32_scaled(int m, int n, const double* X, i64 ldx, const double* colscale, float* Y,
+                                         i64 ldy, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_f64_to_f32_scaled_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s,
+                       m, X, ldx, colscale, Y, ldy);
+    return hipGetLastError();
+}
+
+__global__ void axpy_f32_to_f64_scaled_kernel(int m, const float* D, i64 ldd, const double* cs, double* X, i64 ldx) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m) {
+        const double sc = cs ? cs[j] : 1.0;
+        X[i + j * ldx] += sc * (double)D[i + j * ldd];
+    }
+}
+hipError_t launch_axpy_f32_to_f64_scaled(int m, int n, const float* D, i64 ldd, const double* colscale, double* X,
+                                         i64 ldx, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(axpy_f32_to_f64_scaled_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s,
+                       m, D, ldd, colscale, X, ldx);
+    return hipGetLastError();
+}
+
+__global__ void copy_f64_kernel(int m, const double* X, i64 ldx, double* Y, i64 ldy) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m) Y[i + j * ldy] = X[i + j * ldx];
+}
+hipError_t launch_copy_f64(int m, int n, const double* X, i64 ldx, double* Y, i64 ldy, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_f64_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s, m, X, ldx, Y,
+                       ldy);
+    return hipGetLastError();
+}
+
+// ------------------------------------------- symmetric product (lower read)
+// out[p, c] = beta*B[p, c] + alpha * sum_q Asym[p, q] * X[q, c]
+// A workgroup owns 64 rows and up to 32 columns and walks every 64x64 tile
+// of its block row of the symmetric matrix; tiles right of the diagonal are
+// read as the transpose of the stored lower tile.  ABS = true uses |A| and
+// X = ones (row sums for the infinity norm).
+constexpr int kSymT = 64;
+constexpr int kSymCols = 32;
+
+template <bool ABS>
+__global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double alpha, const double* __restrict__ A,
+                                                         i64 lda, const double* __restrict__ X, i64 ldx, double beta,
+                                                         const double* __restrict__ B, i64 ldb, double* __restrict__ R,
+                                                         i64 ldr) {
+    __shared__ double sM[kSymT][kSymT + 1];  // sM[q][p] = Asym(p0+p, q0+q)
+    __shared__ double sX[kSymT][kSymCols + 1];
+    const int t = threadIdx.x;
+    const int I = blockIdx.x;
+    const i64 p0 = (i64)I * kSymT;
+    const int c_base = blockIdx.y * kSymCols;
+    const int kcl = ABS ? 1 : min(kSymCols, k - c_base);
+    const int pl = t & 63, cg = t >> 6;
+    double acc[kSymCols / 4];
+#pragma unroll
+    for (int u = 0; u < kSymCols / 4; ++u) acc[u] = 0.0;
+    const int nT = (int)ceil_div(n, kSymT);
+    for (int J = 0; J < nT; ++J) {
+        const i64 q0 = (i64)J * kSymT;
+        __syncthreads();
+        if (J < I) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int qq = (t >> 6) + 4 * s;
+                const i64 p = p0 + (t & 63), q = q0 + qq;
+                double v = (p < n) ? A[p + q * lda] : 0.0;
+                sM[qq][t & 63] = ABS ? fabs(v) : v;
+            }
+        } else if (J > I) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int pp = (t >> 6) + 4 * s;
+                const i64 q = q0 + (t & 63), p = p0 + pp;
+                double v = (q < n && p < n) ? A[q + p * lda] : 0.0;
+                sM[t & 63][pp] = ABS ? fabs(v) : v;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int qq = (t >> 6) + 4 * s;
+                const int pp = t & 63;
+                if (pp >= qq) {
+                    const i64 p = p0 + pp, q = q0 + qq;
+                    double v = (p < n) ? A[p + q * lda] : 0.0;
+                    if (ABS) v = fabs(v);
+                    sM[qq][pp] = v;
+                    sM[pp][qq] = v;
+                }
+            }
+        }
+        for (int idx = t; idx < kSymT * kSymCols; idx += 256) {
+            const int qq = idx % kSymT, c = idx / kSymT;
+            const i64 q = q0 + qq;
+            sX[qq][c] = ABS ? ((c == 0 && q < n) ? 1.0 : 0.0) : ((q < n && c < kcl) ? X[q + (i64)(c_base + c) * ldx] : 0.0);
+        }
+        __syncthreads();
+        for (int qq = 0; qq < kSymT; ++qq) {
+            const double m = sM[qq][pl];
+#pragma unroll
+            for (int u = 0; u < kSymCols / 4; ++u) acc[u] += m * sX[qq][cg + 4 * u];
+        }
+    }
+    const i64 p = p0 + pl;
+    if (p < n) {
+#pragma unroll
+        for (int u = 0; u < kSymCols / 4; ++u) {
+            const int c = cg + 4 * u;
+            if (c < kcl) {
+                const i64 cc = c_base + c;
+                const double bv = (beta == 0.0) ? 0.0 : beta * B[p + cc * ldb];
+                R[p + cc * ldr] = bv + alpha * acc[u];
+            }
+        }
+    }
+}
+
+hipError_t launch_symm_lower_residual(int n, int k, const double* A, i64 lda, const double* X, i64 ldx,
+                                      const double* B, i64 ldb, double* R, i64 ldr, hipStream_t s) {
+    if (n <= 0 || k <= 0) return hipSuccess;
+    dim3 grid((unsigned)ceil_div(n, kSymT), (unsigned)ceil_div(k, kSymCols));
+    hipLaunchKernelGGL(symm_lower_kernel<false>, grid, dim3(256), 0, s, n, k, -1.0, A, lda, X, ldx, 1.0, B, ldb, R,
+                       ldr);
+    return hipGetLastError();
+}
+
+hipError_t launch_symm_lower(int n, int k, const double* A, i64 lda, const double* X, i64 ldx, double* Y, i64 ldy,
+                             hipStream_t s) {
+    if (n <= 0 || k <= 0) return hipSuccess;
+    dim3 grid((unsigned)ceil_div(n, kSymT), (unsigned)ceil_div(k, kSymCols));
+    hipLaunchKernelGGL(symm_lower_kernel<false>, grid, dim3(256), 0, s, n, k, 1.0, A, lda, X, ldx, 0.0, Y, ldy, Y,
+                       ldy);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_lower_abs_rowsum(int n, const double* A, i64 lda, double* rowsum, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    dim3 grid((unsigned)ceil_div(n, kSymT), 1);
+    hipLaunchKernelGGL(symm_lower_kernel<true>, grid, dim3(256), 0, s, n, 1, 1.0, A, lda, (const double*)nullptr,
+                       (i64)1, 0.0, (const double*)nullptr, (i64)1, rowsum, (i64)n);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ reductions
+__device__ inline double block_max_256(double v) {
+    __shared__ double red[256];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    return red[0];
+}
+
+__global__ __launch_bounds__(256) void col_absmax_kernel(int m, const double* X, i64 ldx, double* colmax) {
+    const int c = blockIdx.x;
+    double v = 0.0;
+    for (i64 i = threadIdx.x; i < m; i += 256) {
+        const double a = fabs(X[i + c * ldx]);
+        v = (a > v || a != a) ? a : v;  // propagate NaN
+    }
+    v = block_max_256(v);
+    if (threadIdx.x == 0) colmax[c] = v;
+}
+hipError_t launch_col_absmax(int m, int k, const double* X, i64 ldx, double* colmax, hipStream_t s) {
+    if (k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(col_absmax_kernel, dim3((unsigned)k), dim3(256), 0, s, m, X, ldx, colmax);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void vec_max_kernel(int n, const double* v, double* out) {
+    double x = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) x = fmax(x, v[i]);
+    x = block_max_256(x);
+    if (threadIdx.x == 0) out[0] = x;
+}
+hipError_t launch_vec_max(int n, const double* v, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(vec_max_kernel, dim3(1), dim3(256), 0, s, n, v, out);
+    return hipGetLastError();
+}
+
+
+// ------------------------------------------------- batched column kernels
+// Right-hand sides are processed as k independent columns; vector i of a
+// set lives at V + i*vstride (an n x k block with leading dim ldv).
+
+// out[i + c*nv] = sum_r V_i(r, c) * W(r, c)   (one workgroup per (c, i))
+__global__ __launch_bounds__(256) void batched_dots_kernel(int n, int nv, const double* V, i64 ldv, i64 vstride,
+                                                           const double* W, i64 ldw, double* out) {
+    __shared__ double red[256];
+    const int c = blockIdx.x, i = blockIdx.y;
+    const double* v = V + (i64)i * vstride + (i64)c * ldv;
+    const double* w = W + (i64)c * ldw;
+    double s = 0.0;
+    for (i64 r = threadIdx.x; r < n; r += 256) s += v[r] * w[r];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i + (i64)c * nv] = red[0];
+}
+hipError_t launch_batched_dots(int n, int k, int nv, const double* V, i64 ldv, i64 vstride, const double* W, i64 ldw,
+                               double* out, hipStream_t s) {
+    if (k <= 0 || nv <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batched_dots_kernel, dim3((unsigned)k, (unsigned)nv), dim3(256), 0, s, n, nv, V, ldv, vstride,
+                       W, ldw, out);
+    return hipGetLastError();
+}
+
+// W(r, c) = beta*W(r, c) + alpha * sum_i h[i + c*nv] * V_i(r, c)
+__global__ __launch_bounds__(256) void batched_gemv_kernel(int n, int nv, double alpha, const double* V, i64 ldv,
+                                                           i64 vstride, const double* h, double beta, double* W,
+                                                           i64 ldw) {
+    const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= n) return;
+    double s = 0.0;
+    for (int i = 0; i < nv; ++i) s += h[i + (i64)c * nv] * V[(i64)i * vstride + r + (i64)c * ldv];
+    double* w = W + r + (i64)c * ldw;
+    *w = (beta == 0.0 ? 0.0 : beta * (*w)) + alpha * s;
+}
+hipError_t launch_batched_gemv(int n, int k, int nv, double alpha, const double* V, i64 ldv, i64 vstride,
+                               const double* h, double beta, double* W, i64 ldw, hipStream_t s) {
+    if (n <= 0 || k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batched_gemv_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)k), dim3(256), 0, s, n, nv,
+                       alpha, V, ldv, vstride, h, beta, W, ldw);
+    return hipGetLastError();
+}
+
+// X(:, c) = f(c) * Y(:, c) with f(c) = s[c] (invert = 0) or 1/s[c] (invert = 1; 0 if s[c] == 0)
+__global__ void col_scale_kernel(int n, const double* Y, i64 ldy, const double* sc, int invert, double* X, i64 ldx) {
+    const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int c = blockIdx.y;
+    if (r >= n) return;
+    double f = sc[c];
+    if (invert) f = (f != 0.0) ? 1.0 / f : 0.0;
+    X[r + (i64)c * ldx] = f * Y[r + (i64)c * ldy];
+}
+hipError_t launch_col_scale(int n, int k, const double* Y, i64 ldy, const double* sc, int invert, double* X, i64 ldx,
+                            hipStream_t s) {
+    if (n <= 0 || k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(col_scale_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)k), dim3(256), 0, s, n, Y, ldy, sc,
+                       invert, X, ldx);
+    return hipGetLastError();
+}
+
+// S(i, i) += sigma * |A(i, i)| on the float copy (diagonal shift of the low-precision factor)
+__global__ void shift_diag_kernel(int n, const double* A, i64 lda, float* S, i64 lds, double sigma) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) S[i + i * lds] = (float)(A[i + i * lda] + sigma * fabs(A[i + i * lda]));
+}
+hipError_t launch_shift_diag(int n, const double* A, i64 lda, float* S, i64 lds, double sigma, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(shift_diag_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, s, n, A, lda, S, lds, sigma);
+    return hipGetLastError();
+}
+
+// Y(:, c) = colscale[c] * double(D(:, c))
+__global__ void copy_f32_to_f64_scaled_kernel(int m, const float* D, i64 ldd, const double* cs, double* Y, i64 ldy) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m) Y[i + j * ldy] = (cs ? cs[j] : 1.0) * (double)D[i + j * ldd];
+}
+hipError_t launch_copy_f32_to_f64_scaled(int m, int n, const float* D, i64 ldd, const double* colscale, double* Y,
+                                         i64 ldy, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_f32_to_f64_scaled_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s,
+                       m, D, ldd, colscale, Y, ldy);
+    return hipGetLastError();
+}
+
+// Y += alpha * X
+__global__ void axpy_f64_kernel(int m, double alpha, const double* X, i64 ldx, double* Y, i64 ldy) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m) Y[i + j * ldy] += alpha * X[i + j * ldx];
+}
+hipError_t launch_axpy_f64(int m, int n, double alpha, const double* X, i64 ldx, double* Y, i64 ldy, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(axpy_f64_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s, m, alpha, X,
+                       ldx, Y, ldy);
+    return hipGetLastError();
+}
+
+}  // namespace elemamd

@@ -1,0 +1,178 @@
+// extern "C" entry points declared in include/elemental_amd.h.
+#include "../../include/elemental_amd.h"
+
+#include <algorithm>
+
+#include "kernels.hpp"
+#include "lapack.hpp"
+
+using namespace elemamd;
+
+namespace {
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+inline char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 32) : c; }
+inline bool is_trans(char c) { c = up(c); return c == 'T' || c == 'C'; }
+inline bool valid_orient(char c) { c = up(c); return c == 'N' || c == 'T' || c == 'C'; }
+
+}  // namespace
+
+extern "C" {
+
+const char* elemamd_version(void) { return "elemental_amd 0.1 (gfx950)"; }
+
+int elemamd_info_read(const int* dinfo, void* stream) {
+    if (dinfo == nullptr) return ELEMAMD_ERR_ARG;
+    int v = 0;
+    if (hipMemcpyAsync(&v, dinfo, sizeof(int), hipMemcpyDeviceToHost, S(stream)) != hipSuccess) return ELEMAMD_ERR_HIP;
+    if (hipStreamSynchronize(S(stream)) != hipSuccess) return ELEMAMD_ERR_HIP;
+    return v;
+}
+
+int elemamd_potrf_unb_f64(int n, double* A, int64_t lda, int* dinfo, void* stream) {
+    if (n < 0 || n > 128 || lda < std::max(1, n) || (n > 0 && A == nullptr)) return ELEMAMD_ERR_ARG;
+    return launch_potrf_unb<double>(n, A, lda, dinfo, 0, S(stream)) == hipSuccess ? ELEMAMD_OK : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_trsm_f64(char side, char uplo, char trans, char diag, int m, int n, double alpha, const double* A,
+                     int64_t lda, double* B, int64_t ldb, void* stream) {
+    side = up(side);
+    uplo = up(uplo);
+    diag = up(diag);
+    if (uplo != 'L' || diag != 'N' || !valid_orient(trans)) return ELEMAMD_ERR_ARG;
+    if (m < 0 || n < 0 || ldb < std::max(1, m)) return ELEMAMD_ERR_ARG;
+    const int na = (side == 'L') ? m : n;
+    if (lda < std::max(1, na)) return ELEMAMD_ERR_ARG;
+    if (m == 0 || n == 0) return ELEMAMD_OK;
+    if (alpha != 1.0 && launch_scale_f64(m, n, alpha, B, ldb, S(stream)) != hipSuccess) return ELEMAMD_ERR_HIP;
+    if (side == 'R') {
+        if (!is_trans(trans) || n > 128) return ELEMAMD_ERR_ARG;
+        return launch_trsm_rlt<double>(m, n, A, lda, B, ldb, nullptr, S(stream)) == hipSuccess ? ELEMAMD_OK
+                                                                                               : ELEMAMD_ERR_HIP;
+    }
+    if (side != 'L') return ELEMAMD_ERR_ARG;
+    return launch_trsm_left_lower<double>(is_trans(trans), m, n, A, lda, B, ldb, nullptr, S(stream)) == hipSuccess
+               ? ELEMAMD_OK
+               : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const double* A, int64_t lda, double beta,
+                     double* C, int64_t ldc, void* stream) {
+    if (up(uplo) != 'L' || up(trans) != 'N') return ELEMAMD_ERR_ARG;
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
+    if (n == 0) return ELEMAMD_OK;
+    return launch_syrk_ln_f64(n, k, alpha, A, lda, beta, C, ldc, nullptr, S(stream)) == hipSuccess ? ELEMAMD_OK
+                                                                                                  : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_herk_f32(int mode, int n, int k, float alpha, const float* A, int64_t lda, float beta, float* C,
+                     int64_t ldc, void* stream) {
+    if (mode != ELEMAMD_TRAILING_F32 && mode != ELEMAMD_TRAILING_BF16X3) return ELEMAMD_ERR_ARG;
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
+    if (n == 0) return ELEMAMD_OK;
+    return launch_syrk_ln_f32(mode == ELEMAMD_TRAILING_BF16X3, n, k, alpha, A, lda, beta, C, ldc, nullptr,
+                              S(stream)) == hipSuccess
+               ? ELEMAMD_OK
+               : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_trrk_f64(char uplo, char orientA, char orientB, int n, int k, double alpha, const double* A, int64_t lda,
+                     const double* B, int64_t ldb, double beta, double* C, int64_t ldc, void* stream) {
+    if (up(uplo) != 'L' || !is_trans(orientA) || up(orientB) != 'N') return ELEMAMD_ERR_ARG;
+    if (n < 0 || k < 0 || lda < std::max(1, k) || ldb < std::max(1, k) || ldc < std::max(1, n))
+        return ELEMAMD_ERR_ARG;
+    if (n == 0) return ELEMAMD_OK;
+    return launch_gemm_tn_f64(n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, true, S(stream)) == hipSuccess
+               ? ELEMAMD_OK
+               : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_trsv_f64(char uplo, char trans, char diag, int n, const double* A, int64_t lda, double* x, int64_t incx,
+                     void* stream) {
+    if (up(uplo) != 'L' || up(diag) != 'N' || !valid_orient(trans) || incx != 1) return ELEMAMD_ERR_ARG;
+    if (n < 0 || lda < std::max(1, n)) return ELEMAMD_ERR_ARG;
+    if (n == 0) return ELEMAMD_OK;
+    return launch_trsm_left_lower<double>(is_trans(trans), n, 1, A, lda, x, std::max(1, n), nullptr, S(stream)) ==
+                   hipSuccess
+               ? ELEMAMD_OK
+               : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_cholesky_f64(char uplo, int n, double* A, int64_t lda, int nb, int* dinfo, void* stream) {
+    if (up(uplo) != 'L') return ELEMAMD_ERR_ARG;
+    return cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, S(stream));
+}
+
+int elemamd_solve_after_f64(char uplo, char orientation, int n, int k, const double* A, int64_t lda, double* B,
+                            int64_t ldb, void* stream) {
+    if (up(uplo) != 'L' || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
+    return solve_after_lower<double>(n, k, A, lda, B, ldb, nullptr, S(stream));
+}
+
+int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double* A, int64_t lda, double* B, int64_t ldb,
+                          int nb, int* dinfo, void* stream) {
+    if (up(uplo) != 'L' || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
+    if (k < 0 || ldb < std::max(1, n)) return ELEMAMD_ERR_ARG;
+    int rc = cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, S(stream));
+    if (rc) return rc;
+    return solve_after_lower<double>(n, k, A, lda, B, ldb, dinfo, S(stream));
+}
+
+static MixedOpts to_opts(const elemamd_ir_opts* o) {
+    MixedOpts m;
+    if (o) {
+        m.nb = o->nb > 0 ? o->nb : 128;
+        m.trailing = o->trailing;
+        m.max_iter = o->max_iter > 0 ? o->max_iter : 30;
+        m.gmres_restart = o->gmres_restart >= 0 ? (o->gmres_restart == 0 ? 20 : o->gmres_restart) : 0;
+        m.force_gmres = o->force_gmres;
+        m.allow_shift = o->allow_shift;
+        m.tol = o->tol > 0.0 ? o->tol : 1e-14;
+    }
+    return m;
+}
+
+size_t elemamd_hpd_solve_mixed_workspace(int n, int k, const elemamd_ir_opts* opts) {
+    if (n < 0 || k < 0) return 0;
+    return mixed_workspace_bytes(n, k, to_opts(opts));
+}
+
+int elemamd_hpd_solve_mixed_f64(int n, int k, const double* A, int64_t lda, double* B, int64_t ldb,
+                                const elemamd_ir_opts* opts, void* work, size_t work_bytes, elemamd_ir_report* report,
+                                void* stream) {
+    MixedOpts o = to_opts(opts);
+    if (o.trailing != kTrailingF32 && o.trailing != kTrailingBf16x3) return ELEMAMD_ERR_ARG;
+    MixedReport r{};
+    int rc = hpd_solve_mixed(n, k, A, lda, B, ldb, o, work, work_bytes, &r, S(stream));
+    if (report) {
+        report->info = r.info;
+        report->lowprec_info = r.lowprec_info;
+        report->iterations = r.iterations;
+        report->gmres_iterations = r.gmres_iterations;
+        report->used_gmres = r.used_gmres;
+        report->shifted = r.shifted;
+        report->converged = r.converged;
+        report->backward_error = r.backward_error;
+        report->anorm_inf = r.anorm_inf;
+    }
+    return rc;
+}
+
+int elemamd_fill_normal_f64(double* out, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, int64_t offset,
+                            void* stream) {
+    if (rows < 0 || cols < 0 || ld < std::max<int64_t>(1, rows)) return ELEMAMD_ERR_ARG;
+    return launch_fill_normal_f64(out, rows, cols, ld, seed, offset, S(stream)) == hipSuccess ? ELEMAMD_OK
+                                                                                             : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_sym_residual_f64(int n, int k, const double* A, int64_t lda, const double* X, int64_t ldx,
+                             const double* B, int64_t ldb, double* R, int64_t ldr, void* stream) {
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldx < std::max(1, n) || ldb < std::max(1, n) ||
+        ldr < std::max(1, n))
+        return ELEMAMD_ERR_ARG;
+    return launch_symm_lower_residual(n, k, A, lda, X, ldx, B, ldb, R, ldr, S(stream)) == hipSuccess
+               ? ELEMAMD_OK
+               : ELEMAMD_ERR_HIP;
+}
+
+}  // extern "C"

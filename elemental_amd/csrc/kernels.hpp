@@ -1,0 +1,66 @@
+// Launchers of the device kernels (each in its own .hip translation unit).
+#pragma once
+
+#include "common.hpp"
+
+namespace elemamd {
+
+// potrf.hip -- LVar3Unb on an n x n (n <= 128) diagonal block.
+template <typename T>
+hipError_t launch_potrf_unb(int n, T* A, i64 lda, int* info, int col_offset, hipStream_t s);
+
+// trsm_panel.hip -- B := B * inv(L)^T, L nb x nb lower (nb <= 128), B m x nb.
+template <typename T>
+hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, hipStream_t s);
+
+// syrk.hip -- lower C := beta*C + alpha*A*A^T.
+hipError_t launch_syrk_ln_f64(int n, int k, double alpha, const double* A, i64 lda, double beta, double* C, i64 ldc,
+                              const int* info, hipStream_t s);
+// float storage; bf16x3 = false: FP32 MFMA, true: split-BF16x3 MFMA with FP32 accumulation.
+hipError_t launch_syrk_ln_f32(bool bf16x3, int n, int k, float alpha, const float* A, i64 lda, float beta, float* C,
+                              i64 ldc, const int* info, hipStream_t s);
+
+// gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
+hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
+                              double beta, double* C, i64 ldc, bool lower_only, hipStream_t s);
+
+// trsm_solve.hip -- B := inv(L) B (trans=false) or inv(L^T) B (trans=true).
+template <typename T>
+hipError_t launch_trsm_left_lower(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const int* info,
+                                  hipStream_t s);
+
+// blas1.hip -- elementwise helpers, conversions, residual and norms.
+hipError_t launch_fill_int(int* p, int value, int count, hipStream_t s);
+hipError_t launch_fill_normal_f64(double* out, i64 rows, i64 cols, i64 ld, uint64_t seed, i64 offset, hipStream_t s);
+hipError_t launch_scale_f64(int m, int n, double alpha, double* A, i64 lda, hipStream_t s);
+hipError_t launch_copy_lower_f64_to_f32(int n, const double* A, i64 lda, float* S, i64 lds, hipStream_t s);
+hipError_t launch_copy_f64_to_f32_scaled(int m, int n, const double* X, i64 ldx, const double* colscale, float* Y,
+                                         i64 ldy, hipStream_t s);
+hipError_t launch_axpy_f32_to_f64_scaled(int m, int n, const float* D, i64 ldd, const double* colscale, double* X,
+                                         i64 ldx, hipStream_t s);
+hipError_t launch_copy_f64(int m, int n, const double* X, i64 ldx, double* Y, i64 ldy, hipStream_t s);
+// R := B - A*X with A symmetric (lower triangle read), n x n; X, B, R n x k.
+hipError_t launch_symm_lower_residual(int n, int k, const double* A, i64 lda, const double* X, i64 ldx,
+                                      const double* B, i64 ldb, double* R, i64 ldr, hipStream_t s);
+// Y := A*X with A symmetric (lower triangle read).
+hipError_t launch_symm_lower(int n, int k, const double* A, i64 lda, const double* X, i64 ldx, double* Y, i64 ldy,
+                             hipStream_t s);
+// colmax[c] = max_i |X(i,c)| for c < k.
+hipError_t launch_col_absmax(int m, int k, const double* X, i64 ldx, double* colmax, hipStream_t s);
+// rowsum[i] = sum_j |A(i,j)| of the symmetric matrix stored in the lower triangle.
+hipError_t launch_sym_lower_abs_rowsum(int n, const double* A, i64 lda, double* rowsum, hipStream_t s);
+// out[0] = max_i v[i]
+hipError_t launch_vec_max(int n, const double* v, double* out, hipStream_t s);
+
+hipError_t launch_batched_dots(int n, int k, int nv, const double* V, i64 ldv, i64 vstride, const double* W, i64 ldw,
+                               double* out, hipStream_t s);
+hipError_t launch_batched_gemv(int n, int k, int nv, double alpha, const double* V, i64 ldv, i64 vstride,
+                               const double* h, double beta, double* W, i64 ldw, hipStream_t s);
+hipError_t launch_col_scale(int n, int k, const double* Y, i64 ldy, const double* sc, int invert, double* X, i64 ldx,
+                            hipStream_t s);
+hipError_t launch_copy_f32_to_f64_scaled(int m, int n, const float* D, i64 ldd, const double* colscale, double* Y,
+                                         i64 ldy, hipStream_t s);
+hipError_t launch_axpy_f64(int m, int n, double alpha, const double* X, i64 ldx, double* Y, i64 ldy, hipStream_t s);
+hipError_t launch_shift_diag(int n, const double* A, i64 lda, float* S, i64 lds, double sigma, hipStream_t s);
+
+}  // namespace elemamd

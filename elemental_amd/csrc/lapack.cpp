@@ -1,0 +1,365 @@
+// Host-side drivers: the blocked factorization loop, SolveAfter, and the
+// mixed-precision iterative-refinement / GMRES-IR solver.  Everything here
+// only enqueues kernels on the caller's stream; nothing is global.
+#include "lapack.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace elemamd {
+
+// ---------------------------------------------------------------- LVar3
+// cholesky::LVar3 (include/elemental/lapack-like/Cholesky/LVar3.hpp:83-123):
+// while the bottom-right block is nonempty, take a diagonal block of
+// min(nb, remaining) (partition_impl.hpp:403-424), factor it with LVar3Unb,
+// solve the panel below it, and apply the symmetric rank-nb update to the
+// trailing matrix.  Every launch checks *dinfo so that work after a failing
+// block is skipped.
+template <typename T>
+int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s) {
+    if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr) return kErrArg;
+    ELEMAMD_HIP_TRY(launch_fill_int(dinfo, 0, 1, s));
+    for (int k = 0; k < n; k += nb) {
+        const int b = std::min(nb, n - k);
+        const int m2 = n - k - b;
+        T* A11 = A + k + (i64)k * lda;
+        ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, A11, lda, dinfo, k, s));
+        if (m2 > 0) {
+            T* A21 = A + (k + b) + (i64)k * lda;
+            T* A22 = A + (k + b) + (i64)(k + b) * lda;
+            ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(m2, b, A11, lda, A21, lda, dinfo, s));
+            hipError_t e;
+            if constexpr (sizeof(T) == 8)
+                e = launch_syrk_ln_f64(m2, b, -1.0, (const double*)A21, lda, 1.0, (double*)A22, lda, dinfo, s);
+            else
+                e = launch_syrk_ln_f32(trailing == kTrailingBf16x3, m2, b, -1.0f, (const float*)A21, lda, 1.0f,
+                                       (float*)A22, lda, dinfo, s);
+            ELEMAMD_HIP_TRY(e);
+        }
+    }
+    return kOk;
+}
+
+template int cholesky_lower<double>(int, double*, i64, int, int, int*, hipStream_t);
+template int cholesky_lower<float>(int, float*, i64, int, int, int*, hipStream_t);
+
+// ----------------------------------------------------------- SolveAfter
+// cholesky::SolveAfter(LOWER, NORMAL) (SolveAfter.hpp:20-75): forward then
+// backward triangular solve with the lower factor.  One column is the Trsv
+// pair in the reference; the blocked kernels handle k = 1 the same way.
+template <typename T>
+int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, hipStream_t s) {
+    if (n < 0 || k < 0 || ldl < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
+    if (n == 0 || k == 0) return kOk;
+    ELEMAMD_HIP_TRY(launch_trsm_left_lower<T>(false, n, k, L, ldl, B, ldb, dinfo, s));
+    ELEMAMD_HIP_TRY(launch_trsm_left_lower<T>(true, n, k, L, ldl, B, ldb, dinfo, s));
+    return kOk;
+}
+
+template int solve_after_lower<double>(int, int, const double*, i64, double*, i64, const int*, hipStream_t);
+template int solve_after_lower<float>(int, int, const float*, i64, float*, i64, const int*, hipStream_t);
+
+// ------------------------------------------------------ mixed precision
+namespace {
+
+struct Carve {
+    char* base;
+    size_t off = 0, cap;
+    Carve(void* p, size_t c) : base((char*)p), cap(c) {}
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = (T*)(base ? base + off : nullptr);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+struct MixedWork {
+    float* S;      // n x n low-precision factor (ld n)
+    double* X;     // n x k iterate
+    double* R;     // n x k residual
+    float* Rf;     // n x k float correction
+    double* D;     // n x k correction (GMRES)
+    double* W;     // n x k GMRES work vector
+    double* V;     // (m+1) blocks of n x k: Krylov basis
+    double* nrm;   // 8k + n + 8 doubles: norms, row sums, scratch
+    double* h;     // (m+1) x k Gram-Schmidt coefficients
+    int* dinfo;
+    size_t bytes;
+};
+
+MixedWork carve_mixed(void* work, size_t cap, int n, int k, int m) {
+    Carve c(work, cap);
+    MixedWork w;
+    const size_t nn = (size_t)n * n, nk = (size_t)n * k;
+    w.S = c.take<float>(nn);
+    w.X = c.take<double>(nk);
+    w.R = c.take<double>(nk);
+    w.Rf = c.take<float>(nk);
+    w.D = c.take<double>(m > 0 ? nk : 0);
+    w.W = c.take<double>(m > 0 ? nk : 0);
+    w.V = c.take<double>(m > 0 ? nk * (size_t)(m + 1) : 0);
+    w.nrm = c.take<double>((size_t)8 * k + n + 8);
+    w.h = c.take<double>((size_t)(m + 2) * k);
+    w.dinfo = c.take<int>(4);
+    w.bytes = c.off + 256;
+    return w;
+}
+
+struct Ctx {
+    int n, k;
+    const double* A;
+    i64 lda;
+    const double* Bin;
+    i64 ldb;
+    MixedWork w;
+    hipStream_t s;
+    double anorm;
+};
+
+// z := M^{-1} y with M = S S^T in float, applied to each column scaled by
+// colscale (device, k entries) to stay inside the float range.
+int precond_apply(Ctx& c, const double* y, double* z, const double* colscale) {
+    const int n = c.n, k = c.k;
+    ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, y, n, colscale, c.w.Rf, n, c.s));
+    int rc = solve_after_lower<float>(n, k, c.w.S, n, c.w.Rf, n, nullptr, c.s);
+    if (rc) return rc;
+    ELEMAMD_HIP_TRY(launch_copy_f32_to_f64_scaled(n, k, c.w.Rf, n, colscale, z, n, c.s));
+    return kOk;
+}
+
+// Column norms: out[c] = ||X(:,c)||_inf, copied to host.
+int host_colmax(Ctx& c, const double* X, double* dev_out, std::vector<double>& host) {
+    ELEMAMD_HIP_TRY(launch_col_absmax(c.n, c.k, X, c.n, dev_out, c.s));
+    host.resize(c.k);
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(host.data(), dev_out, sizeof(double) * c.k, hipMemcpyDeviceToHost, c.s));
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(c.s));
+    return kOk;
+}
+
+// One restarted GMRES cycle for A d = r (per column, left preconditioned by
+// M = S S^T), started from d = 0.  rscale[c] = ||r(:,c)||_inf on device.
+int gmres_cycle(Ctx& c, int m, double inner_tol, int* iters_out) {
+    const int n = c.n, k = c.k;
+    const size_t nk = (size_t)n * k;
+    double* dscale = c.w.nrm + 4 * k;  // device: per-column scale scratch
+    double* dnorm = c.w.nrm + 5 * k;   // device: per-column 2-norms / dots scratch
+    double* h = c.w.h;
+    // z0 = M^{-1} r  -> V_0 after normalization
+    std::vector<double> hs;
+    int rc = host_colmax(c, c.w.R, dscale, hs);
+    if (rc) return rc;
+    rc = precond_apply(c, c.w.R, c.w.V, dscale);
+    if (rc) return rc;
+    ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, c.w.V, n, (i64)nk, c.w.V, n, dnorm, c.s));
+    std::vector<double> beta(k);
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(beta.data(), dnorm, sizeof(double) * k, hipMemcpyDeviceToHost, c.s));
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(c.s));
+    for (int j = 0; j < k; ++j) beta[j] = std::sqrt(beta[j]);
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(dnorm, beta.data(), sizeof(double) * k, hipMemcpyHostToDevice, c.s));
+    ELEMAMD_HIP_TRY(launch_col_scale(n, k, c.w.V, n, dnorm, 1, c.w.V, n, c.s));
+
+    // Hessenberg (column-major (m+1) x m per RHS), Givens rotations, rhs g.
+    std::vector<double> H((size_t)k * (m + 1) * m, 0.0), cs((size_t)k * m), sn((size_t)k * m),
+        g((size_t)k * (m + 1), 0.0);
+    std::vector<char> done(k, 0);
+    for (int col = 0; col < k; ++col) {
+        g[(size_t)col * (m + 1)] = beta[col];
+        if (!(beta[col] > 0.0)) done[col] = 1;
+    }
+    std::vector<double> hcol((size_t)(m + 1) * k), h2((size_t)(m + 1) * k), wn(k);
+    int jdone = 0;
+    for (int j = 0; j < m; ++j) {
+        double* Vj = c.w.V + (size_t)j * nk;
+        double* Vj1 = c.w.V + (size_t)(j + 1) * nk;
+        // W = M^{-1} A V_j
+        ELEMAMD_HIP_TRY(launch_symm_lower(n, k, c.A, c.lda, Vj, n, c.w.D, n, c.s));
+        rc = host_colmax(c, c.w.D, dscale, hs);
+        if (rc) return rc;
+        rc = precond_apply(c, c.w.D, c.w.W, dscale);
+        if (rc) return rc;
+        // CGS2 against V_0..V_j
+        const int nv = j + 1;
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, nv, c.w.V, n, (i64)nk, c.w.W, n, h, c.s));
+        ELEMAMD_HIP_TRY(launch_batched_gemv(n, k, nv, -1.0, c.w.V, n, (i64)nk, h, 1.0, c.w.W, n, c.s));
+        ELEMAMD_HIP_TRY(hipMemcpyAsync(hcol.data(), h, sizeof(double) * nv * k, hipMemcpyDeviceToHost, c.s));
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, nv, c.w.V, n, (i64)nk, c.w.W, n, h, c.s));
+        ELEMAMD_HIP_TRY(launch_batched_gemv(n, k, nv, -1.0, c.w.V, n, (i64)nk, h, 1.0, c.w.W, n, c.s));
+        ELEMAMD_HIP_TRY(hipMemcpyAsync(h2.data(), h, sizeof(double) * nv * k, hipMemcpyDeviceToHost, c.s));
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, c.w.W, n, (i64)nk, c.w.W, n, dnorm, c.s));
+        ELEMAMD_HIP_TRY(hipMemcpyAsync(wn.data(), dnorm, sizeof(double) * k, hipMemcpyDeviceToHost, c.s));
+        ELEMAMD_HIP_TRY(hipStreamSynchronize(c.s));
+        for (int col = 0; col < k; ++col) wn[col] = std::sqrt(std::max(wn[col], 0.0));
+        ELEMAMD_HIP_TRY(hipMemcpyAsync(dnorm, wn.data(), sizeof(double) * k, hipMemcpyHostToDevice, c.s));
+        ELEMAMD_HIP_TRY(launch_col_scale(n, k, c.w.W, n, dnorm, 1, Vj1, n, c.s));
+        bool all = true;
+        for (int col = 0; col < k; ++col) {
+            double* Hc = &H[(size_t)col * (m + 1) * m];
+            double* hc = &Hc[(size_t)j * (m + 1)];
+            for (int i = 0; i < nv; ++i) hc[i] = hcol[i + (size_t)col * nv] + h2[i + (size_t)col * nv];
+            hc[j + 1] = wn[col];
+            for (int i = 0; i < j; ++i) {  // previous rotations
+                const double a = hc[i], b = hc[i + 1];
+                const double cc = cs[(size_t)col * m + i], ss = sn[(size_t)col * m + i];
+                hc[i] = cc * a + ss * b;
+                hc[i + 1] = -ss * a + cc * b;
+            }
+            const double a = hc[j], b = hc[j + 1];
+            const double r = std::hypot(a, b);
+            const double cc = (r > 0.0) ? a / r : 1.0, ss = (r > 0.0) ? b / r : 0.0;
+            cs[(size_t)col * m + j] = cc;
+            sn[(size_t)col * m + j] = ss;
+            hc[j] = r;
+            hc[j + 1] = 0.0;
+            double* gc = &g[(size_t)col * (m + 1)];
+            gc[j + 1] = -ss * gc[j];
+            gc[j] = cc * gc[j];
+            if (!done[col] && std::fabs(gc[j + 1]) <= inner_tol * beta[col]) done[col] = 1;
+            if (!done[col]) all = false;
+        }
+        jdone = j + 1;
+        if (all) break;
+    }
+    *iters_out += jdone;
+    // y = H^{-1} g (upper triangular, per column), then d = V y
+    std::vector<double> y((size_t)jdone * k, 0.0);
+    for (int col = 0; col < k; ++col) {
+        const double* Hc = &H[(size_t)col * (m + 1) * m];
+        const double* gc = &g[(size_t)col * (m + 1)];
+        double* yc = &y[(size_t)col * jdone];
+        for (int i = jdone - 1; i >= 0; --i) {
+            double t = gc[i];
+            for (int l = i + 1; l < jdone; ++l) t -= Hc[i + (size_t)l * (m + 1)] * yc[l];
+            const double d = Hc[i + (size_t)i * (m + 1)];
+            yc[i] = (d != 0.0) ? t / d : 0.0;
+        }
+    }
+    if (jdone > 0) {
+        ELEMAMD_HIP_TRY(hipMemcpyAsync(h, y.data(), sizeof(double) * jdone * k, hipMemcpyHostToDevice, c.s));
+        ELEMAMD_HIP_TRY(launch_batched_gemv(n, k, jdone, 1.0, c.w.V, n, (i64)nk, h, 0.0, c.w.D, n, c.s));
+    } else {
+        ELEMAMD_HIP_TRY(launch_scale_f64(n, k, 0.0, c.w.D, n, c.s));
+    }
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(c.s));
+    return kOk;
+}
+
+}  // namespace
+
+size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o) {
+    const int m = o.gmres_restart;
+    return carve_mixed(nullptr, 0, n, k, m).bytes;
+}
+
+int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o, void* work,
+                    size_t work_bytes, MixedReport* rep, hipStream_t s) {
+    MixedReport r{};
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
+    if (o.nb < 1 || o.nb > 128) return kErrArg;
+    if (n == 0 || k == 0) {
+        r.converged = 1;
+        if (rep) *rep = r;
+        return kOk;
+    }
+    const int m = o.gmres_restart;
+    MixedWork w = carve_mixed(work, work_bytes, n, k, m);
+    if (work == nullptr || work_bytes < w.bytes) return kErrWorkspace;
+    Ctx c{n, k, A, lda, B, ldb, w, s, 0.0};
+
+    // ||A||_inf from the symmetric row sums, ||b||_inf per column
+    double* rowsum = w.nrm + 8 * k;
+    double* dtmp = w.nrm + 8 * k + n;
+    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, s));
+    ELEMAMD_HIP_TRY(launch_vec_max(n, rowsum, dtmp, s));
+    std::vector<double> bn, xn, rn;
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(&c.anorm, dtmp, sizeof(double), hipMemcpyDeviceToHost, s));
+    ELEMAMD_HIP_TRY(launch_col_absmax(n, k, B, ldb, w.nrm, s));
+    bn.resize(k);
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(bn.data(), w.nrm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+
+    // Low-precision factor: S = float(A), FP64 diagonal potrf, float panels.
+    double sigma = 0.0;
+    int info = 0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32(n, A, lda, w.S, n, s));
+        if (sigma > 0.0) ELEMAMD_HIP_TRY(launch_shift_diag(n, A, lda, w.S, n, sigma, s));
+        int rc = cholesky_lower<float>(n, w.S, n, o.nb, o.trailing, w.dinfo, s);
+        if (rc) return rc;
+        ELEMAMD_HIP_TRY(hipMemcpyAsync(&info, w.dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+        ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
+        r.lowprec_info = info;
+        if (info == 0) break;
+        if (!o.allow_shift) break;
+        sigma = (sigma == 0.0) ? 1e-5 : sigma * 10.0;
+        r.shifted = 1;
+    }
+    r.anorm_inf = c.anorm;
+
+    if (info != 0) {
+        // The low-precision factor broke down even after shifting: report it
+        // (the caller can fall back to the FP64 factorization).
+        if (rep) *rep = r;
+        return kErrNotConverged;
+    }
+
+    // X = 0, R = B
+    ELEMAMD_HIP_TRY(launch_copy_f64(n, k, B, ldb, w.R, n, s));
+    ELEMAMD_HIP_TRY(launch_scale_f64(n, k, 0.0, w.X, n, s));
+    bool gmres = o.force_gmres && m > 0;
+    double prev = INFINITY;
+    int stall = 0;
+    double eta = INFINITY;
+    for (int it = 0; it <= o.max_iter; ++it) {
+        if (it > 0) {
+            ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, A, lda, w.X, n, B, ldb, w.R, n, s));
+            r.iterations = it;
+        }
+        // norms
+        int rc = host_colmax(c, w.R, w.nrm + k, rn);
+        if (rc) return rc;
+        rc = host_colmax(c, w.X, w.nrm + 2 * k, xn);
+        if (rc) return rc;
+        eta = 0.0;
+        for (int col = 0; col < k; ++col) {
+            const double den = c.anorm * xn[col] + bn[col];
+            const double e = (den > 0.0) ? rn[col] / den : (rn[col] > 0.0 ? INFINITY : 0.0);
+            eta = std::max(eta, std::isnan(e) ? INFINITY : e);
+        }
+        if (it > 0 && eta <= o.tol) break;
+        if (it == o.max_iter) break;
+        if (it > 1) {
+            if (eta > 0.5 * prev) ++stall;
+            else stall = 0;
+            if (!gmres && m > 0 && (stall >= 2 || !(eta < prev * 4.0))) gmres = true;
+        }
+        prev = std::min(prev, eta);
+        if (!gmres) {
+            // d = M^{-1} r in float, x += d
+            ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, w.R, n, w.nrm + k, w.Rf, n, s));
+            rc = solve_after_lower<float>(n, k, w.S, n, w.Rf, n, nullptr, s);
+            if (rc) return rc;
+            ELEMAMD_HIP_TRY(launch_axpy_f32_to_f64_scaled(n, k, w.Rf, n, w.nrm + k, w.X, n, s));
+        } else {
+            r.used_gmres = 1;
+            int gi = 0;
+            rc = gmres_cycle(c, m, 1e-8, &gi);
+            if (rc) return rc;
+            r.gmres_iterations += gi;
+            // X += D
+            ELEMAMD_HIP_TRY(launch_axpy_f64(n, k, 1.0, w.D, n, w.X, n, s));
+        }
+    }
+    r.backward_error = eta;
+    r.converged = (eta <= o.tol) ? 1 : 0;
+    ELEMAMD_HIP_TRY(launch_copy_f64(n, k, w.X, n, B, ldb, s));
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
+    if (rep) *rep = r;
+    return r.converged ? kOk : kErrNotConverged;
+}
+
+}  // namespace elemamd

@@ -1,0 +1,45 @@
+// Host drivers of the path (internal C++ interface under the C ABI).
+#pragma once
+
+#include "common.hpp"
+
+namespace elemamd {
+
+enum : int { kTrailingF64 = 0, kTrailingF32 = 1, kTrailingBf16x3 = 2 };
+
+// cholesky::LVar3 on device memory; T = double (FP64 path) or float (the
+// low-precision factor, diagonal blocks still factored in double).
+template <typename T>
+int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s);
+
+// cholesky::SolveAfter(LOWER, NORMAL): B := inv(L^T) inv(L) B.
+template <typename T>
+int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, hipStream_t s);
+
+struct MixedOpts {
+    int nb = 128;
+    int trailing = kTrailingF32;
+    int max_iter = 30;
+    int gmres_restart = 20;
+    int force_gmres = 0;
+    int allow_shift = 1;
+    double tol = 1e-14;
+};
+
+struct MixedReport {
+    int info;
+    int lowprec_info;
+    int iterations;
+    int gmres_iterations;
+    int used_gmres;
+    int shifted;
+    int converged;
+    double backward_error;
+    double anorm_inf;
+};
+
+size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o);
+int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o, void* work,
+                    size_t work_bytes, MixedReport* rep, hipStream_t s);
+
+}  // namespace elemamd

@@ -1,0 +1,188 @@
+/*
+ * elemental_amd.h -- C ABI of the MI355X (gfx950) HPD Cholesky factor-and-solve
+ * library.  It is the drop-in boundary for Elemental's lower Cholesky /
+ * HPDSolve path on one GPU's local matrix.
+ *
+ * Conventions
+ *   - Matrices are column-major with a leading dimension in elements, exactly
+ *     like elem::Matrix<F> (include/elemental/core/matrix.hpp:16-183, element
+ *     (i,j) at i + j*ldim).  All matrix pointers are DEVICE pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Calls are asynchronous on that stream unless documented otherwise.
+ *   - Return value: ELEMAMD_OK (0) when the work was enqueued (or done), a
+ *     negative ELEMAMD_ERR_* code on an invalid argument or a HIP failure.
+ *   - Factorizations report the reference's non-HPD condition through a
+ *     device int `dinfo`: 0 on success, otherwise the 1-based global column
+ *     whose pivot was <= 0 -- the column at which the reference's
+ *     cholesky::LVar3Unb throws std::logic_error("A was not numerically HPD")
+ *     (include/elemental/lapack-like/Cholesky/LVar3.hpp:37-38).  Work after a
+ *     failing block is skipped, so A is left as the reference leaves it.  As
+ *     in the reference, a NaN pivot is not detected.
+ *   - Character arguments follow the Fortran BLAS conventions the reference
+ *     forwards to (src/core/imports/blas.cpp): uplo 'L'/'U', trans 'N'/'T'/'C'
+ *     ('C' == 'T' for real data), side 'L'/'R', diag 'N'/'U'.
+ *   - Every entry point is re-entrant: no global mutable state; block size,
+ *     precision and refinement parameters are per call.
+ */
+#ifndef ELEMENTAL_AMD_H
+#define ELEMENTAL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ELEMAMD_OK 0
+#define ELEMAMD_ERR_ARG (-1)
+#define ELEMAMD_ERR_HIP (-2)
+#define ELEMAMD_ERR_NOT_CONVERGED (-3)
+#define ELEMAMD_ERR_WORKSPACE (-4)
+
+/* Trailing-update arithmetic of the low-precision factor. */
+#define ELEMAMD_TRAILING_F64 0
+#define ELEMAMD_TRAILING_F32 1
+#define ELEMAMD_TRAILING_BF16X3 2
+
+/* Library version string. */
+const char *elemamd_version(void);
+
+/* Synchronize `stream` and return the value of the device int *dinfo
+ * (a negative ELEMAMD_ERR_* if the copy fails). */
+int elemamd_info_read(const int *dinfo, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Local kernels of the blocked factorization.                        */
+
+/* Unblocked lower Cholesky of an n x n block, n <= 128, in place.
+ * Replaces cholesky::LVar3Unb (include/elemental/lapack-like/Cholesky/LVar3.hpp:19-49),
+ * reached from LocalCholesky (include/elemental/lapack-like/Cholesky.hpp:29-37).
+ * Bit-identical to the reference loop in IEEE double.  *dinfo must be 0 on
+ * entry (a nonzero value makes the call a no-op, as after an earlier failure). */
+int elemamd_potrf_unb_f64(int n, double *A, int64_t lda, int *dinfo, void *stream);
+
+/* B := alpha * op(A)^{-1} B (side 'L') or alpha * B op(A)^{-1} (side 'R'),
+ * A lower triangular, non-unit diagonal.  Replaces elem::Trsm(Matrix)
+ * (include/elemental/blas-like/level3/Trsm.hpp:49-86) -> blas::Trsm ->
+ * dtrsm_ (src/core/imports/blas.cpp:1203-1210).
+ * Supported: side 'R', trans 'T'/'C' with n <= 128 (the Cholesky panel
+ * solve, bit-identical to the column-oriented reference BLAS loop); side 'L',
+ * trans 'N'/'T'/'C', any size (the SolveAfter solves).  Other combinations
+ * return ELEMAMD_ERR_ARG.  alpha != 1 pre-scales B. */
+int elemamd_trsm_f64(char side, char uplo, char trans, char diag, int m, int n, double alpha,
+                     const double *A, int64_t lda, double *B, int64_t ldb, void *stream);
+
+/* Lower C := alpha*A*A^T + beta*C, A n x k (trans 'N').  Replaces
+ * elem::Herk / Syrk(Matrix) (include/elemental/blas-like/level3/Herk.hpp:17-27,
+ * Syrk.hpp:20-57) -> blas::Herk -> dsyrk_ (src/core/imports/blas.cpp:1043-1050).
+ * The strictly upper triangle of C is neither read nor written. */
+int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const double *A, int64_t lda,
+                     double beta, double *C, int64_t ldc, void *stream);
+
+/* Float-storage trailing update of the mixed-precision factor (no reference
+ * counterpart): mode ELEMAMD_TRAILING_F32 (FP32 MFMA) or
+ * ELEMAMD_TRAILING_BF16X3 (split-BF16x3, FP32 accumulation). */
+int elemamd_herk_f32(int mode, int n, int k, float alpha, const float *A, int64_t lda, float beta, float *C,
+                     int64_t ldc, void *stream);
+
+/* Lower trapezoid of C := alpha * op(A) * op(B) + beta*C with A and B k x n
+ * (orientA 'T'/'C', orientB 'N').  Replaces elem::Trrk(Matrix) /
+ * LocalTrrk's TN path (src/blas-like/Trrk.cpp:19-38,
+ * src/blas-like/Trrk/Local.hpp:733-775 and :931-976). */
+int elemamd_trrk_f64(char uplo, char orientA, char orientB, int n, int k, double alpha, const double *A,
+                     int64_t lda, const double *B, int64_t ldb, double beta, double *C, int64_t ldc, void *stream);
+
+/* x := op(L)^{-1} x for one vector (incx must be 1).  Replaces elem::Trsv
+ * (include/elemental/blas-like/level2/Trsv.hpp:20-44) -> dtrsv_
+ * (src/core/imports/blas.cpp:901-904). */
+int elemamd_trsv_f64(char uplo, char trans, char diag, int n, const double *A, int64_t lda, double *x,
+                     int64_t incx, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* The path.                                                          */
+
+/* Blocked right-looking lower Cholesky (variant 3) with block size nb
+ * (1..128; Elemental's default Blocksize() is 128, src/core/global.cpp:194).
+ * Replaces elem::Cholesky(LOWER, Matrix<F>&)
+ * (include/elemental/lapack-like/Cholesky.hpp:49-62) -> cholesky::LVar3
+ * (include/elemental/lapack-like/Cholesky/LVar3.hpp:83-123).  The lower
+ * triangle of A becomes L; the strictly upper triangle is untouched.
+ * Asynchronous: the outcome is written to *dinfo. */
+int elemamd_cholesky_f64(char uplo, int n, double *A, int64_t lda, int nb, int *dinfo, void *stream);
+
+/* B := A^{-1} B given the factor L in the lower triangle of A.  Replaces
+ * cholesky::SolveAfter(uplo, orientation, A, B)
+ * (include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75): two Trsv
+ * for one column, two Trsm otherwise.  uplo 'L'; orientation 'N', 'T' or
+ * 'C' (identical for real data). */
+int elemamd_solve_after_f64(char uplo, char orientation, int n, int k, const double *A, int64_t lda, double *B,
+                            int64_t ldb, void *stream);
+
+/* Cholesky then SolveAfter.  Replaces elem::HPDSolve(uplo, orientation,
+ * Matrix<F>& A, Matrix<F>& B) (include/elemental/lapack-like/HPDSolve.hpp:18-28).
+ * Asynchronous; the solve is skipped when *dinfo becomes nonzero. */
+int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double *A, int64_t lda, double *B,
+                          int64_t ldb, int nb, int *dinfo, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Mixed precision with FP64 iterative refinement (no reference         */
+/* counterpart; results are held to the FP64 backward-error target).    */
+
+typedef struct {
+    int nb;               /* factorization block size, 1..128 (0 -> 128) */
+    int trailing;         /* ELEMAMD_TRAILING_F32 or ELEMAMD_TRAILING_BF16X3 */
+    int max_iter;         /* refinement steps before giving up (0 -> 30) */
+    int gmres_restart;    /* Krylov dimension of a GMRES-IR cycle: 0 -> 20, negative disables the fallback */
+    int force_gmres;      /* 1: use GMRES-IR from the first correction */
+    int allow_shift;      /* 1: on low-precision breakdown, retry the factor of A + sigma*diag(A) */
+    double tol;           /* target normwise backward error (inf-norms); 0 -> 1e-14 */
+} elemamd_ir_opts;
+
+typedef struct {
+    int info;             /* reserved, 0 */
+    int lowprec_info;     /* failing column of the last low-precision factor attempt (0 = none) */
+    int iterations;       /* refinement steps (FP64 residuals computed) */
+    int gmres_iterations; /* total inner GMRES iterations */
+    int used_gmres;       /* 1 if GMRES-IR was used */
+    int shifted;          /* 1 if the low-precision factor needed a diagonal shift */
+    int converged;        /* 1 if backward_error <= tol */
+    double backward_error;/* max over columns of ||b-Ax||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+    double anorm_inf;     /* ||A||_inf */
+} elemamd_ir_report;
+
+/* Workspace bytes needed by elemamd_hpd_solve_mixed_f64. */
+size_t elemamd_hpd_solve_mixed_workspace(int n, int k, const elemamd_ir_opts *opts);
+
+/* Solve A X = B for symmetric positive definite A (lower triangle read, A is
+ * NOT modified) to FP64 backward error: low-precision factor (FP64 diagonal
+ * potrf, FP32-stored panels, FP32 or split-BF16x3 trailing update) then FP64
+ * residual iterative refinement, falling back to GMRES-IR preconditioned by
+ * the low-precision factor when classic refinement stagnates.  B is
+ * overwritten with X.  Synchronous (it reads convergence norms each step).
+ * `work` must hold elemamd_hpd_solve_mixed_workspace() bytes of device memory.
+ * Returns ELEMAMD_OK, ELEMAMD_ERR_NOT_CONVERGED (X holds the best iterate),
+ * or another error; *report (may be NULL) receives the details. */
+int elemamd_hpd_solve_mixed_f64(int n, int k, const double *A, int64_t lda, double *B, int64_t ldb,
+                                const elemamd_ir_opts *opts, void *work, size_t work_bytes,
+                                elemamd_ir_report *report, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Utilities.                                                          */
+
+/* Fill a rows x cols column-major block with N(0,1) numbers: entry t in
+ * column-major order is number offset+t of the splitmix64 / Box-Muller
+ * stream `seed`. */
+int elemamd_fill_normal_f64(double *out, int64_t rows, int64_t cols, int64_t ld, uint64_t seed, int64_t offset,
+                            void *stream);
+
+/* R := B - A*X with A symmetric, lower triangle read (the FP64 refinement
+ * residual), n x n; X, B, R n x k. */
+int elemamd_sym_residual_f64(int n, int k, const double *A, int64_t lda, const double *X, int64_t ldx,
+                             const double *B, int64_t ldb, double *R, int64_t ldr, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ELEMENTAL_AMD_H */

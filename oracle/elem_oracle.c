@@ -1,0 +1,490 @@
+/*
+ * elem_oracle.c -- CPU restatement of Elemental's lower HPD Cholesky
+ * factor-and-solve path.  TEST INFRASTRUCTURE ONLY: it is the checker that
+ * the HIP library is compared against, and the CPU baseline that bench.py
+ * times beside the GPU.  Nothing in the product library links, loads or
+ * calls this file.
+ *
+ * What it restates (all file:line citations are into aterrel/Elemental):
+ *   eo_lvar3unb        cholesky::LVar3Unb     include/elemental/lapack-like/Cholesky/LVar3.hpp:19-49
+ *   eo_lvar3           cholesky::LVar3        include/elemental/lapack-like/Cholesky/LVar3.hpp:83-123
+ *                      block clamp            include/elemental/core/partition_impl.hpp:403-424
+ *   eo_solve_after     cholesky::SolveAfter   include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75
+ *   eo_hpd_solve       HPDSolve               include/elemental/lapack-like/HPDSolve.hpp:18-28
+ *   eo_trrk_tn         internal::TrrkTN       src/blas-like/Trrk/Local.hpp:733-775 (leaf :452-490)
+ *   eo_local_trrk_tn   LocalTrrk (1x1 grid)   src/blas-like/Trrk/Local.hpp:931-976 (leaf :492-536)
+ *   eo_scale_trapezoid ScaleTrapezoid         include/elemental/blas-like/level1/ScaleTrapezoid.hpp:15-84
+ *   eo_axpy_triangle   AxpyTriangle           include/elemental/blas-like/level1/AxpyTriangle.hpp:16-38
+ *
+ * The reference hands its level-3/level-2 arithmetic to an external,
+ * unpinned Fortran BLAS (CMakeLists.txt:281-315; call sites
+ * src/core/imports/blas.cpp:901-904 dtrsv, :930-940 dgemm, :1043-1050 dsyrk,
+ * :1203-1210 dtrsm).  The survey container resolved it to Intel MKL 2021.4.
+ * This file restates the published algorithm of that dependency: the Netlib
+ * reference BLAS (the column-oriented loops of dtrsm/dsyrk/dgemm/dtrsv as
+ * published with LAPACK 3.x), written here as eo_dtrsm/eo_dsyrk/eo_dgemm/
+ * eo_dtrsv.  For the CPU baseline at full size the same blocked loop can
+ * instead call an optimised host BLAS bound at run time (eo_bind_blas);
+ * that is what the reference itself does.
+ *
+ * Build with -ffp-contract=off so every multiply and add rounds separately,
+ * as in the Fortran reference loops.
+ *
+ * Parity pins (tests/test_oracle.py): integer-exact known answers
+ * (A = L0*L0^T with integer L0 and power-of-two diagonal, and the Pascal
+ * matrix whose factor is the binomial triangle), LAPACK dpotrf/dpotrs and
+ * the host BLAS via scipy, the splitmix64 published output sequence, and
+ * the invariants L*L^T = A, diag(L) > 0 and backward error.
+ */
+#include <dlfcn.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t i64;
+
+#define AT(A, i, j, ld) ((A)[(i64)(i) + (i64)(j) * (i64)(ld)])
+
+/* ------------------------------------------------------------------ */
+/* Portable seeded generator: splitmix64 (Steele, Lea, Flood 2014) as a
+ * counter-based stream, and Box-Muller normals.  The same definitions are
+ * used by the HIP library's generator and by the Python tests, so inputs
+ * are identical on CPU and GPU up to libm rounding of log/cos.            */
+
+uint64_t eo_splitmix64(uint64_t *state) {
+    uint64_t z = (*state += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* i-th output of the stream started at `seed` (counter form). */
+static uint64_t sm64_at(uint64_t seed, uint64_t i) {
+    uint64_t s = seed + i * 0x9E3779B97F4A7C15ULL;
+    return eo_splitmix64(&s);
+}
+
+/* uniform in (0,1]: 53 high bits, offset by one ulp so log() is finite */
+static double u01(uint64_t x) { return ((double)(x >> 11) + 1.0) * 0x1.0p-53; }
+
+/* out[t] for t in [0,count): standard normal number (offset + t) of the
+ * stream `seed`.  Pair p = (offset+t)/2 uses outputs 2p and 2p+1. */
+void eo_fill_normal(double *out, i64 count, uint64_t seed, i64 offset) {
+    const double two_pi = 6.283185307179586476925286766559;
+    for (i64 t = 0; t < count; ++t) {
+        i64 idx = offset + t;
+        uint64_t p = (uint64_t)(idx >> 1);
+        double u1 = u01(sm64_at(seed, 2 * p));
+        double u2 = u01(sm64_at(seed, 2 * p + 1));
+        double r = sqrt(-2.0 * log(u1));
+        out[t] = (idx & 1) ? r * sin(two_pi * u2) : r * cos(two_pi * u2);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Netlib reference BLAS restatements (the published algorithms).       */
+
+static int lsame(char a, char b) {
+    if (a >= 'a' && a <= 'z') a = (char)(a - 32);
+    if (b >= 'a' && b <= 'z') b = (char)(b - 32);
+    return a == b;
+}
+
+/* C := alpha*op(A)*op(B) + beta*C, op in {N,T}. */
+void eo_dgemm(char ta, char tb, int m, int n, int k, double alpha,
+              const double *A, int lda, const double *B, int ldb,
+              double beta, double *C, int ldc) {
+    int nota = lsame(ta, 'N'), notb = lsame(tb, 'N');
+    if (m == 0 || n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return;
+    if (alpha == 0.0) {
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < m; ++i)
+                AT(C, i, j, ldc) = (beta == 0.0) ? 0.0 : beta * AT(C, i, j, ldc);
+        return;
+    }
+    if (notb) {
+        if (nota) {
+            for (int j = 0; j < n; ++j) {
+                if (beta == 0.0) for (int i = 0; i < m; ++i) AT(C, i, j, ldc) = 0.0;
+                else if (beta != 1.0) for (int i = 0; i < m; ++i) AT(C, i, j, ldc) = beta * AT(C, i, j, ldc);
+                for (int l = 0; l < k; ++l) {
+                    double temp = alpha * AT(B, l, j, ldb);
+                    for (int i = 0; i < m; ++i) AT(C, i, j, ldc) += temp * AT(A, i, l, lda);
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < m; ++i) {
+                    double temp = 0.0;
+                    for (int l = 0; l < k; ++l) temp += AT(A, l, i, lda) * AT(B, l, j, ldb);
+                    AT(C, i, j, ldc) = (beta == 0.0) ? alpha * temp : alpha * temp + beta * AT(C, i, j, ldc);
+                }
+        }
+    } else {
+        if (nota) {
+            for (int j = 0; j < n; ++j) {
+                if (beta == 0.0) for (int i = 0; i < m; ++i) AT(C, i, j, ldc) = 0.0;
+                else if (beta != 1.0) for (int i = 0; i < m; ++i) AT(C, i, j, ldc) = beta * AT(C, i, j, ldc);
+                for (int l = 0; l < k; ++l) {
+                    double temp = alpha * AT(B, j, l, ldb);
+                    for (int i = 0; i < m; ++i) AT(C, i, j, ldc) += temp * AT(A, i, l, lda);
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < m; ++i) {
+                    double temp = 0.0;
+                    for (int l = 0; l < k; ++l) temp += AT(A, l, i, lda) * AT(B, j, l, ldb);
+                    AT(C, i, j, ldc) = (beta == 0.0) ? alpha * temp : alpha * temp + beta * AT(C, i, j, ldc);
+                }
+        }
+    }
+}
+
+/* Lower-triangle SYRK: C := alpha*A*A^T + beta*C (trans N, A n x k) or
+ * C := alpha*A^T*A + beta*C (trans T, A k x n).  Upper triangle untouched. */
+void eo_dsyrk(char uplo, char trans, int n, int k, double alpha,
+              const double *A, int lda, double beta, double *C, int ldc) {
+    (void)uplo; /* lower only; the only uplo on this path */
+    if (n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return;
+    if (lsame(trans, 'N')) {
+        for (int j = 0; j < n; ++j) {
+            if (beta == 0.0) for (int i = j; i < n; ++i) AT(C, i, j, ldc) = 0.0;
+            else if (beta != 1.0) for (int i = j; i < n; ++i) AT(C, i, j, ldc) = beta * AT(C, i, j, ldc);
+            if (alpha == 0.0) continue;
+            for (int l = 0; l < k; ++l) {
+                if (AT(A, j, l, lda) != 0.0) {
+                    double temp = alpha * AT(A, j, l, lda);
+                    for (int i = j; i < n; ++i) AT(C, i, j, ldc) += temp * AT(A, i, l, lda);
+                }
+            }
+        }
+    } else {
+        for (int j = 0; j < n; ++j)
+            for (int i = j; i < n; ++i) {
+                double temp = 0.0;
+                for (int l = 0; l < k; ++l) temp += AT(A, l, i, lda) * AT(A, l, j, lda);
+                AT(C, i, j, ldc) = (beta == 0.0) ? alpha * temp : alpha * temp + beta * AT(C, i, j, ldc);
+            }
+    }
+}
+
+/* Lower-triangular TRSM, non-unit or unit diagonal.
+ *   side L, trans N : B := alpha*inv(A)*B
+ *   side L, trans T : B := alpha*inv(A^T)*B
+ *   side R, trans T : B := alpha*B*inv(A^T)
+ *   side R, trans N : B := alpha*B*inv(A)
+ * A is m x m (side L) or n x n (side R), lower. */
+void eo_dtrsm(char side, char uplo, char transa, char diag, int m, int n,
+              double alpha, const double *A, int lda, double *B, int ldb) {
+    (void)uplo; /* lower only */
+    int nounit = lsame(diag, 'N');
+    if (m == 0 || n == 0) return;
+    if (alpha == 0.0) {
+        for (int j = 0; j < n; ++j) for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = 0.0;
+        return;
+    }
+    if (lsame(side, 'L')) {
+        if (lsame(transa, 'N')) {
+            for (int j = 0; j < n; ++j) {
+                if (alpha != 1.0) for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = alpha * AT(B, i, j, ldb);
+                for (int k = 0; k < m; ++k) {
+                    if (AT(B, k, j, ldb) != 0.0) {
+                        if (nounit) AT(B, k, j, ldb) = AT(B, k, j, ldb) / AT(A, k, k, lda);
+                        for (int i = k + 1; i < m; ++i)
+                            AT(B, i, j, ldb) -= AT(B, k, j, ldb) * AT(A, i, k, lda);
+                    }
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j)
+                for (int i = m - 1; i >= 0; --i) {
+                    double temp = alpha * AT(B, i, j, ldb);
+                    for (int k = i + 1; k < m; ++k) temp -= AT(A, k, i, lda) * AT(B, k, j, ldb);
+                    if (nounit) temp = temp / AT(A, i, i, lda);
+                    AT(B, i, j, ldb) = temp;
+                }
+        }
+    } else {
+        if (lsame(transa, 'N')) {
+            for (int j = n - 1; j >= 0; --j) {
+                if (alpha != 1.0) for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = alpha * AT(B, i, j, ldb);
+                for (int k = j + 1; k < n; ++k)
+                    if (AT(A, k, j, lda) != 0.0)
+                        for (int i = 0; i < m; ++i) AT(B, i, j, ldb) -= AT(A, k, j, lda) * AT(B, i, k, ldb);
+                if (nounit) {
+                    double temp = 1.0 / AT(A, j, j, lda);
+                    for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = temp * AT(B, i, j, ldb);
+                }
+            }
+        } else {
+            for (int k = 0; k < n; ++k) {
+                if (nounit) {
+                    double temp = 1.0 / AT(A, k, k, lda);
+                    for (int i = 0; i < m; ++i) AT(B, i, k, ldb) = temp * AT(B, i, k, ldb);
+                }
+                for (int j = k + 1; j < n; ++j) {
+                    if (AT(A, j, k, lda) != 0.0) {
+                        double temp = AT(A, j, k, lda);
+                        for (int i = 0; i < m; ++i) AT(B, i, j, ldb) -= temp * AT(B, i, k, ldb);
+                    }
+                }
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) AT(B, i, k, ldb) = alpha * AT(B, i, k, ldb);
+            }
+        }
+    }
+}
+
+/* Lower-triangular TRSV with unit stride, trans N or T. */
+void eo_dtrsv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x) {
+    (void)uplo;
+    int nounit = lsame(diag, 'N');
+    if (lsame(trans, 'N')) {
+        for (int j = 0; j < n; ++j) {
+            if (x[j] != 0.0) {
+                if (nounit) x[j] = x[j] / AT(A, j, j, lda);
+                double temp = x[j];
+                for (int i = j + 1; i < n; ++i) x[i] -= temp * AT(A, i, j, lda);
+            }
+        }
+    } else {
+        for (int j = n - 1; j >= 0; --j) {
+            double temp = x[j];
+            for (int i = n - 1; i >= j + 1; --i) temp -= AT(A, i, j, lda) * x[i];
+            if (nounit) temp = temp / AT(A, j, j, lda);
+            x[j] = temp;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Optional optimised host BLAS, bound at run time (CPU baseline).      */
+
+typedef void (*fp_dtrsm)(const char *, const char *, const char *, const char *,
+                         const int *, const int *, const double *, const double *,
+                         const int *, double *, const int *);
+typedef void (*fp_dsyrk)(const char *, const char *, const int *, const int *,
+                         const double *, const double *, const int *,
+                         const double *, double *, const int *);
+typedef void (*fp_dtrsv)(const char *, const char *, const char *, const int *,
+                         const double *, const int *, double *, const int *);
+typedef void (*fp_setnt)(int);
+
+static fp_dtrsm g_dtrsm;
+static fp_dsyrk g_dsyrk;
+static fp_dtrsv g_dtrsv;
+
+/* Bind dtrsm/dsyrk/dtrsv from a shared BLAS whose Fortran symbols carry
+ * `prefix` (e.g. "scipy_" for scipy's OpenBLAS, "" for MKL).  Returns 0 on
+ * success.  `setnt_name` optionally names a set-num-threads function. */
+int eo_bind_blas(const char *path, const char *prefix, const char *setnt_name, int nthreads) {
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return 1;
+    char name[128];
+    snprintf(name, sizeof name, "%sdtrsm_", prefix);
+    g_dtrsm = (fp_dtrsm)dlsym(h, name);
+    snprintf(name, sizeof name, "%sdsyrk_", prefix);
+    g_dsyrk = (fp_dsyrk)dlsym(h, name);
+    snprintf(name, sizeof name, "%sdtrsv_", prefix);
+    g_dtrsv = (fp_dtrsv)dlsym(h, name);
+    if (!g_dtrsm || !g_dsyrk || !g_dtrsv) return 2;
+    if (setnt_name && setnt_name[0]) {
+        fp_setnt f = (fp_setnt)dlsym(h, setnt_name);
+        if (!f) return 3;
+        f(nthreads);
+    }
+    return 0;
+}
+
+int eo_blas_bound(void) { return g_dtrsm && g_dsyrk && g_dtrsv; }
+
+static void x_dtrsm(int use_host, char side, char uplo, char ta, char diag, int m, int n,
+                    double alpha, const double *A, int lda, double *B, int ldb) {
+    if (use_host) g_dtrsm(&side, &uplo, &ta, &diag, &m, &n, &alpha, A, &lda, B, &ldb);
+    else eo_dtrsm(side, uplo, ta, diag, m, n, alpha, A, lda, B, ldb);
+}
+static void x_dsyrk(int use_host, char uplo, char trans, int n, int k, double alpha,
+                    const double *A, int lda, double beta, double *C, int ldc) {
+    if (use_host) g_dsyrk(&uplo, &trans, &n, &k, &alpha, A, &lda, &beta, C, &ldc);
+    else eo_dsyrk(uplo, trans, n, k, alpha, A, lda, beta, C, ldc);
+}
+static void x_dtrsv(int use_host, char uplo, char trans, char diag, int n,
+                    const double *A, int lda, double *x) {
+    int one = 1;
+    if (use_host) g_dtrsv(&uplo, &trans, &diag, &n, A, &lda, x, &one);
+    else eo_dtrsv(uplo, trans, diag, n, A, lda, x);
+}
+
+/* ------------------------------------------------------------------ */
+/* Elemental's algorithms.                                              */
+
+/* LVar3Unb.  Returns 0, or j+1 for the first column whose pivot is <= 0
+ * (the reference throws logic_error("A was not numerically HPD") there,
+ * leaving columns < j factored).  A NaN pivot is not detected, exactly as
+ * in the reference (NaN <= 0 is false). */
+int eo_lvar3unb(int n, double *A, int lda) {
+    for (int j = 0; j < n; ++j) {
+        double alpha = AT(A, j, j, lda);
+        if (alpha <= 0.0) return j + 1;
+        alpha = sqrt(alpha);
+        AT(A, j, j, lda) = alpha;
+        for (int k = j + 1; k < n; ++k) AT(A, k, j, lda) /= alpha;
+        for (int k = j + 1; k < n; ++k)
+            for (int i = k; i < n; ++i)
+                AT(A, i, k, lda) -= AT(A, i, j, lda) * AT(A, k, j, lda);
+    }
+    return 0;
+}
+
+/* LVar3: blocked right-looking lower Cholesky with block size nb
+ * (Elemental's default Blocksize() is 128).  Each step clamps the diagonal
+ * block to the remaining height.  Returns 0 or the global failing column+1. */
+static int lvar3_impl(int use_host, int n, double *A, int lda, int nb) {
+    if (nb < 1) nb = 1;
+    for (int k = 0; k < n; k += nb) {
+        int b = (n - k < nb) ? n - k : nb;
+        int m2 = n - k - b;
+        double *A11 = &AT(A, k, k, lda);
+        int info = eo_lvar3unb(b, A11, lda);
+        if (info) return k + info;
+        if (m2 > 0) {
+            double *A21 = &AT(A, k + b, k, lda);
+            double *A22 = &AT(A, k + b, k + b, lda);
+            x_dtrsm(use_host, 'R', 'L', 'T', 'N', m2, b, 1.0, A11, lda, A21, lda);
+            x_dsyrk(use_host, 'L', 'N', m2, b, -1.0, A21, lda, 1.0, A22, lda);
+        }
+    }
+    return 0;
+}
+
+int eo_lvar3(int n, double *A, int lda, int nb) { return lvar3_impl(0, n, A, lda, nb); }
+int eo_lvar3_hostblas(int n, double *A, int lda, int nb) { return lvar3_impl(1, n, A, lda, nb); }
+
+/* cholesky::SolveAfter(LOWER, NORMAL, L, B): B := inv(L^T)*inv(L)*B.
+ * One column -> two Trsv; otherwise two Trsm.  For real data TRANSPOSE
+ * orientation only conjugates, which is the identity. */
+static void solve_after_impl(int use_host, int n, int k, const double *L, int ldl, double *B, int ldb) {
+    if (k == 1) {
+        x_dtrsv(use_host, 'L', 'N', 'N', n, L, ldl, B);
+        x_dtrsv(use_host, 'L', 'T', 'N', n, L, ldl, B);
+    } else {
+        x_dtrsm(use_host, 'L', 'L', 'N', 'N', n, k, 1.0, L, ldl, B, ldb);
+        x_dtrsm(use_host, 'L', 'L', 'T', 'N', n, k, 1.0, L, ldl, B, ldb);
+    }
+}
+
+void eo_solve_after(int n, int k, const double *L, int ldl, double *B, int ldb) {
+    solve_after_impl(0, n, k, L, ldl, B, ldb);
+}
+void eo_solve_after_hostblas(int n, int k, const double *L, int ldl, double *B, int ldb) {
+    solve_after_impl(1, n, k, L, ldl, B, ldb);
+}
+
+int eo_hpd_solve(int n, int k, double *A, int lda, double *B, int ldb, int nb) {
+    int info = eo_lvar3(n, A, lda, nb);
+    if (info) return info;
+    eo_solve_after(n, k, A, lda, B, ldb);
+    return 0;
+}
+
+int eo_hpd_solve_hostblas(int n, int k, double *A, int lda, double *B, int ldb, int nb) {
+    int info = eo_lvar3_hostblas(n, A, lda, nb);
+    if (info) return info;
+    eo_solve_after_hostblas(n, k, A, lda, B, ldb);
+    return 0;
+}
+
+/* ScaleTrapezoid(alpha, LOWER, A, offset=0, side=LEFT). */
+void eo_scale_trapezoid_lower(double alpha, int m, int n, double *A, int lda) {
+    for (int j = 0; j < n; ++j) {
+        int nz = j > 0 ? j : 0;
+        for (int i = nz; i < m; ++i) AT(A, i, j, lda) *= alpha;
+    }
+}
+
+/* AxpyTriangle(LOWER, alpha, X, Y): per column j, Y(j:n,j) += alpha*X(j:n,j)
+ * (a daxpy per column; daxpy is y := y + alpha*x elementwise). */
+void eo_axpy_triangle_lower(int n, double alpha, const double *X, int ldx, double *Y, int ldy) {
+    for (int j = 0; j < n; ++j)
+        for (int i = j; i < n; ++i) AT(Y, i, j, ldy) += alpha * AT(X, i, j, ldx);
+}
+
+/* Leaf of TrrkTN: lower trapezoid of C := alpha*A^T*B + beta*C,
+ * A and B are k x n, C is n x n.  Elemental's leaf scales the trapezoid by
+ * beta, computes the off-diagonal quarter with one GEMM and each diagonal
+ * quarter as a full square GEMM into a temporary that is then
+ * triangle-added (Trrk/Local.hpp:452-490 and 492-536). */
+static void trrk_tn_leaf(int k, int n, double alpha, const double *A, int lda,
+                         const double *B, int ldb, double beta, double *C, int ldc) {
+    int half = n / 2;
+    eo_scale_trapezoid_lower(beta, n, n, C, ldc);
+    /* CBL += alpha * AR^T * BL   (AR = A(:,half:n), BL = B(:,0:half)) */
+    eo_dgemm('T', 'N', n - half, half, k, alpha, &AT(A, 0, half, lda), lda,
+             B, ldb, 1.0, &AT(C, half, 0, ldc), ldc);
+    /* DTL := alpha*AL^T*BL ; CTL_lower += DTL_lower */
+    if (half > 0) {
+        double *D = (double *)malloc(sizeof(double) * (size_t)half * (size_t)half);
+        eo_dgemm('T', 'N', half, half, k, alpha, A, lda, B, ldb, 0.0, D, half);
+        eo_axpy_triangle_lower(half, 1.0, D, half, C, ldc);
+        free(D);
+    }
+    int h2 = n - half;
+    if (h2 > 0) {
+        double *D = (double *)malloc(sizeof(double) * (size_t)h2 * (size_t)h2);
+        eo_dgemm('T', 'N', h2, h2, k, alpha, &AT(A, 0, half, lda), lda,
+                 &AT(B, 0, half, ldb), ldb, 0.0, D, h2);
+        eo_axpy_triangle_lower(h2, 1.0, D, h2, &AT(C, half, half, ldc), ldc);
+        free(D);
+    }
+}
+
+/* Recursive halving of TrrkTN / LocalTrrk with leaf threshold `bsize`
+ * (LocalTrrkBlocksize<double>() = 64 by default, global.cpp:48-51; the
+ * 1x1 grid makes g.Width() = 1). */
+void eo_trrk_tn(int k, int n, double alpha, const double *A, int lda,
+                const double *B, int ldb, double beta, double *C, int ldc, int bsize) {
+    if (n < bsize) {
+        trrk_tn_leaf(k, n, alpha, A, lda, B, ldb, beta, C, ldc);
+        return;
+    }
+    int half = n / 2;
+    eo_dgemm('T', 'N', n - half, half, k, alpha, &AT(A, 0, half, lda), lda,
+             B, ldb, beta, &AT(C, half, 0, ldc), ldc);
+    eo_trrk_tn(k, half, alpha, A, lda, B, ldb, beta, C, ldc, bsize);
+    eo_trrk_tn(k, n - half, alpha, &AT(A, 0, half, lda), lda, &AT(B, 0, half, ldb), ldb,
+               beta, &AT(C, half, half, ldc), ldc, bsize);
+}
+
+/* ------------------------------------------------------------------ */
+/* Norms used by the checks.                                             */
+
+/* Frobenius norm of the symmetric matrix whose lower triangle is stored. */
+double eo_sym_lower_fro(int n, const double *A, int lda) {
+    double s = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double d = AT(A, j, j, lda);
+        s += d * d;
+        for (int i = j + 1; i < n; ++i) { double v = AT(A, i, j, lda); s += 2.0 * v * v; }
+    }
+    return sqrt(s);
+}
+
+/* R := B - A*X with A symmetric from its lower triangle (n x n), X, B n x k. */
+void eo_sym_lower_residual(int n, int k, const double *A, int lda, const double *X, int ldx,
+                           const double *B, int ldb, double *R, int ldr) {
+    for (int c = 0; c < k; ++c) {
+        for (int i = 0; i < n; ++i) AT(R, i, c, ldr) = AT(B, i, c, ldb);
+        for (int j = 0; j < n; ++j) {
+            double xj = AT(X, j, c, ldx);
+            AT(R, j, c, ldr) -= AT(A, j, j, lda) * xj;
+            for (int i = j + 1; i < n; ++i) {
+                double a = AT(A, i, j, lda);
+                AT(R, i, c, ldr) -= a * xj;
+                AT(R, j, c, ldr) -= a * AT(X, i, c, ldx);
+            }
+        }
+    }
+}

@@ -1,0 +1,220 @@
+"""numpy front end of the CPU oracle (``oracle/elem_oracle.c``).
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` import this module, and only as the
+checker (or the timed CPU baseline) -- never as the product path.
+
+All matrices are column-major (Fortran-ordered) float64 arrays, like
+Elemental's ``Matrix<double>`` (``include/elemental/core/matrix.hpp:16-183``,
+element (i, j) at ``i + j*ldim``).  Functions work in place where the
+reference does and return LAPACK-style ``info`` for the factorization.
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libelem_oracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "libelem_oracle.so"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(
+                f"oracle library {_LIB_PATH} is not built; run `make -C oracle` "
+                "or __graft_entry__.build()")
+        L = ctypes.CDLL(_LIB_PATH)
+        i, d, c, vp = ctypes.c_int, ctypes.c_double, ctypes.c_char, ctypes.c_void_p
+        i64, u64 = ctypes.c_int64, ctypes.c_uint64
+        L.eo_splitmix64.argtypes = [ctypes.POINTER(u64)]
+        L.eo_splitmix64.restype = u64
+        L.eo_fill_normal.argtypes = [vp, i64, u64, i64]
+        L.eo_dgemm.argtypes = [c, c, i, i, i, d, vp, i, vp, i, d, vp, i]
+        L.eo_dsyrk.argtypes = [c, c, i, i, d, vp, i, d, vp, i]
+        L.eo_dtrsm.argtypes = [c, c, c, c, i, i, d, vp, i, vp, i]
+        L.eo_dtrsv.argtypes = [c, c, c, i, vp, i, vp]
+        L.eo_lvar3unb.argtypes = [i, vp, i]
+        L.eo_lvar3unb.restype = i
+        L.eo_lvar3.argtypes = [i, vp, i, i]
+        L.eo_lvar3.restype = i
+        L.eo_lvar3_hostblas.argtypes = [i, vp, i, i]
+        L.eo_lvar3_hostblas.restype = i
+        L.eo_solve_after.argtypes = [i, i, vp, i, vp, i]
+        L.eo_solve_after_hostblas.argtypes = [i, i, vp, i, vp, i]
+        L.eo_hpd_solve.argtypes = [i, i, vp, i, vp, i, i]
+        L.eo_hpd_solve.restype = i
+        L.eo_hpd_solve_hostblas.argtypes = [i, i, vp, i, vp, i, i]
+        L.eo_hpd_solve_hostblas.restype = i
+        L.eo_scale_trapezoid_lower.argtypes = [d, i, i, vp, i]
+        L.eo_axpy_triangle_lower.argtypes = [i, d, vp, i, vp, i]
+        L.eo_trrk_tn.argtypes = [i, i, d, vp, i, vp, i, d, vp, i, i]
+        L.eo_sym_lower_fro.argtypes = [i, vp, i]
+        L.eo_sym_lower_fro.restype = d
+        L.eo_sym_lower_residual.argtypes = [i, i, vp, i, vp, i, vp, i, vp, i]
+        L.eo_bind_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i]
+        L.eo_bind_blas.restype = i
+        L.eo_blas_bound.restype = i
+        _lib = L
+    return _lib
+
+
+def _f(a: np.ndarray) -> np.ndarray:
+    if a.dtype != np.float64 or not a.flags.f_contiguous:
+        raise TypeError("oracle expects Fortran-ordered float64 arrays")
+    return a
+
+
+def _p(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _ld(a: np.ndarray) -> int:
+    return max(1, a.shape[0])
+
+
+# ---------------------------------------------------------------- generator
+def splitmix64_sequence(seed: int, count: int) -> list[int]:
+    s = ctypes.c_uint64(seed)
+    return [lib().eo_splitmix64(ctypes.byref(s)) for _ in range(count)]
+
+
+def normal(shape, seed: int, offset: int = 0) -> np.ndarray:
+    """Column-major N(0,1) array; entry t (column-major order) is number
+    ``offset + t`` of the splitmix64/Box-Muller stream ``seed``."""
+    shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+    out = np.empty(shape, dtype=np.float64, order="F")
+    lib().eo_fill_normal(_p(out), out.size, seed, offset)
+    return out
+
+
+# ------------------------------------------------------- reference BLAS loops
+def dgemm(ta, tb, alpha, A, B, beta, C):
+    _f(A), _f(B), _f(C)
+    m, n = C.shape
+    k = A.shape[0] if ta.upper() == "T" else A.shape[1]
+    lib().eo_dgemm(ta.encode(), tb.encode(), m, n, k, alpha, _p(A), _ld(A), _p(B), _ld(B),
+                   beta, _p(C), _ld(C))
+    return C
+
+
+def dsyrk(trans, alpha, A, beta, C):
+    _f(A), _f(C)
+    n = C.shape[0]
+    k = A.shape[1] if trans.upper() == "N" else A.shape[0]
+    lib().eo_dsyrk(b"L", trans.encode(), n, k, alpha, _p(A), _ld(A), beta, _p(C), _ld(C))
+    return C
+
+
+def dtrsm(side, transa, alpha, L, B, diag="N"):
+    _f(L), _f(B)
+    m, n = B.shape
+    lib().eo_dtrsm(side.encode(), b"L", transa.encode(), diag.encode(), m, n, alpha,
+                   _p(L), _ld(L), _p(B), _ld(B))
+    return B
+
+
+def dtrsv(trans, L, x, diag="N"):
+    _f(L), _f(x)
+    lib().eo_dtrsv(b"L", trans.encode(), diag.encode(), L.shape[0], _p(L), _ld(L), _p(x))
+    return x
+
+
+# ----------------------------------------------------------- Elemental path
+def lvar3unb(A) -> int:
+    _f(A)
+    return lib().eo_lvar3unb(A.shape[0], _p(A), _ld(A))
+
+
+def lvar3(A, nb: int = 128, host_blas: bool = False) -> int:
+    _f(A)
+    fn = lib().eo_lvar3_hostblas if host_blas else lib().eo_lvar3
+    if host_blas and not lib().eo_blas_bound():
+        raise RuntimeError("host BLAS not bound; call bind_host_blas() first")
+    return fn(A.shape[0], _p(A), _ld(A), nb)
+
+
+def solve_after(L, B, host_blas: bool = False):
+    _f(L), _f(B)
+    fn = lib().eo_solve_after_hostblas if host_blas else lib().eo_solve_after
+    k = B.shape[1] if B.ndim == 2 else 1
+    fn(L.shape[0], k, _p(L), _ld(L), _p(B), _ld(B))
+    return B
+
+
+def hpd_solve(A, B, nb: int = 128, host_blas: bool = False) -> int:
+    _f(A), _f(B)
+    fn = lib().eo_hpd_solve_hostblas if host_blas else lib().eo_hpd_solve
+    k = B.shape[1] if B.ndim == 2 else 1
+    return fn(A.shape[0], k, _p(A), _ld(A), _p(B), _ld(B), nb)
+
+
+def scale_trapezoid_lower(alpha, A):
+    _f(A)
+    lib().eo_scale_trapezoid_lower(alpha, A.shape[0], A.shape[1], _p(A), _ld(A))
+    return A
+
+
+def axpy_triangle_lower(alpha, X, Y):
+    _f(X), _f(Y)
+    lib().eo_axpy_triangle_lower(X.shape[0], alpha, _p(X), _ld(X), _p(Y), _ld(Y))
+    return Y
+
+
+def trrk_tn(alpha, A, B, beta, C, bsize: int = 64):
+    """Lower trapezoid of C := alpha*A^T*B + beta*C by Elemental's recursive
+    halving (TrrkTN / LocalTrrk on a 1x1 grid)."""
+    _f(A), _f(B), _f(C)
+    k, n = A.shape
+    lib().eo_trrk_tn(k, n, alpha, _p(A), _ld(A), _p(B), _ld(B), beta, _p(C), _ld(C), bsize)
+    return C
+
+
+def sym_lower_fro(A) -> float:
+    _f(A)
+    return lib().eo_sym_lower_fro(A.shape[0], _p(A), _ld(A))
+
+
+def sym_lower_residual(A, X, B) -> np.ndarray:
+    _f(A), _f(X), _f(B)
+    R = np.empty_like(B, order="F")
+    k = B.shape[1] if B.ndim == 2 else 1
+    lib().eo_sym_lower_residual(A.shape[0], k, _p(A), _ld(A), _p(X), _ld(X), _p(B), _ld(B),
+                                _p(R), _ld(R))
+    return R
+
+
+# ------------------------------------------------------- host BLAS (baseline)
+def find_host_blas():
+    """Locate an optimised LP64 host BLAS: scipy's bundled OpenBLAS (symbols
+    prefixed ``scipy_``).  Returns (path, prefix, set_threads_symbol, name)."""
+    import scipy
+
+    libs = os.path.join(os.path.dirname(os.path.dirname(scipy.__file__)), "scipy.libs")
+    cands = sorted(glob.glob(os.path.join(libs, "libscipy_openblas*.so")))
+    if cands:
+        return cands[0], "scipy_", "scipy_openblas_set_num_threads", "OpenBLAS (scipy bundle)"
+    raise RuntimeError("no host BLAS found")
+
+
+def bind_host_blas(nthreads: int) -> str:
+    path, prefix, setnt, name = find_host_blas()
+    rc = lib().eo_bind_blas(path.encode(), prefix.encode(), setnt.encode(), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"binding host BLAS {path} failed (rc={rc})")
+    return f"{name}: {os.path.basename(path)}"
