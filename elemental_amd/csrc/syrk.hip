@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int n, int k, doubl
             const int i = i0 + wi + 16 * nb + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int j = j0 + wj + 16 * mb + 4 * (lane >> 4) + v;
+                const int j = j0 + wj + 16 * mb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
                 if (i < n && j <= i) {
                     double* p = C + (i64)i + (i64)j * ldc;
                     const double upd = alpha * acc[mb][nb][v];
