@@ -24,9 +24,10 @@ SOURCES = [
     "trsm_panel.hip",
     "syrk.hip",
     "gemm.hip",
-    "trsm_solve.hip",
+    "solve.hip",
     "blas1.hip",
     "lapack.cpp",
+    "runtime.cpp",
     "capi.cpp",
 ]
 
@@ -59,6 +60,15 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the C++ drop-in API test program (host code only, links the library)
+    test_src = os.path.join(ROOT, "tests", "cpp", "elem_api_test.cpp")
+    test_bin = os.path.join(ROOT, "tests", "cpp", "elem_api_test")
+    if os.path.exists(test_src):
+        cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), test_src, "-o", test_bin,
+               "-L" + HERE, "-lelemental_amd", "-Wl,-rpath,$ORIGIN/../../elemental_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {test_src} failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -124,86 +124,137 @@ hipError_t launch_copy_f64(int m, int n, const double* X, i64 ldx, double* Y, i6
 
 // ------------------------------------------- symmetric product (lower read)
 // out[p, c] = beta*B[p, c] + alpha * sum_q Asym[p, q] * X[q, c]
-// A workgroup owns 64 rows and up to 32 columns and walks every 64x64 tile
-// of its block row of the symmetric matrix; tiles right of the diagonal are
-// read as the transpose of the stored lower tile.  ABS = true uses |A| and
-// X = ones (row sums for the infinity norm).
+// with Asym the symmetric matrix whose lower triangle is stored in A.
+//
+// A workgroup owns 64 output rows and up to 32 right-hand sides and walks
+// the 64x64 tiles of its block row of the symmetric matrix (tiles right of
+// the diagonal are read as the transpose of the stored lower tile, the
+// diagonal tile is mirrored).  Tiles and the matching 64 x 32 slab of X are
+// staged through double-buffered LDS; the product runs on
+// v_mfma_f64_16x16x4_f64 with MFMA rows = right-hand sides and MFMA columns =
+// output rows, so each 16 lanes store 128 contiguous bytes of a column.
+// At k = 32 the FP64 work (2 n^2 k flops) and the bytes read (8 n^2) are
+// balanced on MI355X, so the kernel sits near both roofs.
+// ABS = true reads |A| and X = ones: out[p] = sum_q |Asym[p, q]| (row sums
+// for the infinity norm).
 constexpr int kSymT = 64;
 constexpr int kSymCols = 32;
+constexpr int kSymLd = kSymT + 1;
 
 template <bool ABS>
 __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double alpha, const double* __restrict__ A,
                                                          i64 lda, const double* __restrict__ X, i64 ldx, double beta,
                                                          const double* __restrict__ B, i64 ldb, double* __restrict__ R,
                                                          i64 ldr) {
-    __shared__ double sM[kSymT][kSymT + 1];  // sM[q][p] = Asym(p0+p, q0+q)
-    __shared__ double sX[kSymT][kSymCols + 1];
-    const int t = threadIdx.x;
+    __shared__ double sM[2][kSymT][kSymLd];     // sM[q][p] = Asym(p0+p, q0+q)
+    __shared__ double sX[2][kSymT][kSymCols + 1];  // sX[q][c] = X(q0+q, c_base+c)
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int I = blockIdx.x;
     const i64 p0 = (i64)I * kSymT;
     const int c_base = blockIdx.y * kSymCols;
     const int kcl = ABS ? 1 : min(kSymCols, k - c_base);
-    const int pl = t & 63, cg = t >> 6;
-    double acc[kSymCols / 4];
-#pragma unroll
-    for (int u = 0; u < kSymCols / 4; ++u) acc[u] = 0.0;
     const int nT = (int)ceil_div(n, kSymT);
-    for (int J = 0; J < nT; ++J) {
+
+    double gm[16], gx[8];
+    // stage tile J into registers
+    auto gload = [&](int J) {
         const i64 q0 = (i64)J * kSymT;
-        __syncthreads();
+        if (J < I) {  // stored lower tile: Asym(p, q) = A(p, q); lanes along p
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int qq = w + 4 * s;
+                const i64 p = p0 + lane;
+                double v = (p < n) ? A[p + (q0 + qq) * lda] : 0.0;
+                gm[s] = ABS ? fabs(v) : v;
+            }
+        } else if (J > I) {  // Asym(p, q) = A(q, p); lanes along q
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int pp = w + 4 * s;
+                const i64 q = q0 + lane, p = p0 + pp;
+                double v = (q < n && p < n) ? A[q + p * lda] : 0.0;
+                gm[s] = ABS ? fabs(v) : v;
+            }
+        } else {  // diagonal tile, lower part (mirrored at store time)
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int qq = w + 4 * s;
+                const i64 p = p0 + lane, q = q0 + qq;
+                double v = (p < n && lane >= qq) ? A[p + q * lda] : 0.0;
+                gm[s] = ABS ? fabs(v) : v;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int c = w + 4 * s;
+            const i64 q = q0 + lane;
+            gx[s] = ABS ? ((c == 0 && q < n) ? 1.0 : 0.0)
+                        : ((q < n && c < kcl) ? X[q + (i64)(c_base + c) * ldx] : 0.0);
+        }
+    };
+    auto sstore = [&](int J, int buf) {
         if (J < I) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int qq = (t >> 6) + 4 * s;
-                const i64 p = p0 + (t & 63), q = q0 + qq;
-                double v = (p < n) ? A[p + q * lda] : 0.0;
-                sM[qq][t & 63] = ABS ? fabs(v) : v;
-            }
+            for (int s = 0; s < 16; ++s) sM[buf][w + 4 * s][lane] = gm[s];
         } else if (J > I) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int pp = (t >> 6) + 4 * s;
-                const i64 q = q0 + (t & 63), p = p0 + pp;
-                double v = (q < n && p < n) ? A[q + p * lda] : 0.0;
-                sM[t & 63][pp] = ABS ? fabs(v) : v;
-            }
+            for (int s = 0; s < 16; ++s) sM[buf][lane][w + 4 * s] = gm[s];
         } else {
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
-                const int qq = (t >> 6) + 4 * s;
-                const int pp = t & 63;
-                if (pp >= qq) {
-                    const i64 p = p0 + pp, q = q0 + qq;
-                    double v = (p < n) ? A[p + q * lda] : 0.0;
-                    if (ABS) v = fabs(v);
-                    sM[qq][pp] = v;
-                    sM[pp][qq] = v;
+                const int qq = w + 4 * s;
+                if (lane >= qq) {
+                    sM[buf][qq][lane] = gm[s];
+                    sM[buf][lane][qq] = gm[s];
                 }
             }
         }
-        for (int idx = t; idx < kSymT * kSymCols; idx += 256) {
-            const int qq = idx % kSymT, c = idx / kSymT;
-            const i64 q = q0 + qq;
-            sX[qq][c] = ABS ? ((c == 0 && q < n) ? 1.0 : 0.0) : ((q < n && c < kcl) ? X[q + (i64)(c_base + c) * ldx] : 0.0);
-        }
-        __syncthreads();
-        for (int qq = 0; qq < kSymT; ++qq) {
-            const double m = sM[qq][pl];
 #pragma unroll
-            for (int u = 0; u < kSymCols / 4; ++u) acc[u] += m * sX[qq][cg + 4 * u];
+        for (int s = 0; s < 8; ++s) sX[buf][lane][w + 4 * s] = gx[s];
+    };
+
+    d4_t acc[2];
+    acc[0] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    acc[1] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    gload(0);
+    sstore(0, 0);
+    __syncthreads();
+    for (int J = 0; J < nT; ++J) {
+        const int buf = J & 1;
+        // branch-free body: the last iteration restages its own tile into the
+        // spare buffer, which is never read again
+        gload((J + 1) < nT ? J + 1 : J);
+        // wave w: output rows p = 16w .. 16w+15, both 16-column blocks of c
+#pragma unroll
+        for (int ks = 0; ks < kSymT / 4; ++ks) {
+            const int q = 4 * ks + (lane >> 4);
+            const double bfrag = sM[buf][q][16 * w + (lane & 15)];
+            const double a0 = sX[buf][q][(lane & 15)];
+            const double a1 = sX[buf][q][16 + (lane & 15)];
+            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag, acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag, acc[1], 0, 0, 0);
         }
+        sstore((J + 1) < nT ? J + 1 : J, buf ^ 1);
+        __syncthreads();
     }
-    const i64 p = p0 + pl;
+    // dependent zero-operand MFMAs right before the accumulators are read, so
+    // the reads follow an MFMA in the same basic block (the compiler's
+    // wait-state padding across the loop back edge is short for FP64 MFMA)
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, acc[1], 0, 0, 0);
+    const i64 p = p0 + 16 * w + (lane & 15);
     if (p < n) {
 #pragma unroll
-        for (int u = 0; u < kSymCols / 4; ++u) {
-            const int c = cg + 4 * u;
-            if (c < kcl) {
-                const i64 cc = c_base + c;
-                const double bv = (beta == 0.0) ? 0.0 : beta * B[p + cc * ldb];
-                R[p + cc * ldr] = bv + alpha * acc[u];
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 16 * cb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
+                if (c < kcl) {
+                    const i64 cc = c_base + c;
+                    const double bv = (beta == 0.0) ? 0.0 : beta * B[p + cc * ldb];
+                    R[p + cc * ldr] = bv + alpha * acc[cb][v];
+                }
             }
-        }
     }
 }
 

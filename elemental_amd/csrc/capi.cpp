@@ -5,6 +5,7 @@
 
 #include "kernels.hpp"
 #include "lapack.hpp"
+#include "runtime.hpp"
 
 using namespace elemamd;
 
@@ -14,6 +15,15 @@ inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 inline char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 32) : c; }
 inline bool is_trans(char c) { c = up(c); return c == 'T' || c == 'C'; }
 inline bool valid_orient(char c) { c = up(c); return c == 'N' || c == 'T' || c == 'C'; }
+
+// One triangular solve with a lower L (forward, or backward with L^T) on
+// the calling stream; the per-stream scratch holds the persistent-solve flags.
+int solve_left_lower(bool trans, int n, int k, const double* L, i64 ldl, double* B, i64 ldb, hipStream_t s) {
+    void* work = stream_scratch(solve_work_bytes<double>(n, k), s);
+    if (work == nullptr) return ELEMAMD_ERR_HIP;
+    int* ctrl = (int*)((char*)work + ((solve_dinv_count(n) * sizeof(double) + 255) & ~(size_t)255));
+    return solve_apply<double>(!trans, trans, n, k, L, ldl, B, ldb, nullptr, ctrl, nullptr, s);
+}
 
 }  // namespace
 
@@ -47,13 +57,11 @@ int elemamd_trsm_f64(char side, char uplo, char trans, char diag, int m, int n, 
     if (alpha != 1.0 && launch_scale_f64(m, n, alpha, B, ldb, S(stream)) != hipSuccess) return ELEMAMD_ERR_HIP;
     if (side == 'R') {
         if (!is_trans(trans) || n > 128) return ELEMAMD_ERR_ARG;
-        return launch_trsm_rlt<double>(m, n, A, lda, B, ldb, nullptr, S(stream)) == hipSuccess ? ELEMAMD_OK
+        return launch_trsm_rlt<double>(m, n, A, lda, B, ldb, nullptr, kNoLimit, S(stream)) == hipSuccess ? ELEMAMD_OK
                                                                                                : ELEMAMD_ERR_HIP;
     }
     if (side != 'L') return ELEMAMD_ERR_ARG;
-    return launch_trsm_left_lower<double>(is_trans(trans), m, n, A, lda, B, ldb, nullptr, S(stream)) == hipSuccess
-               ? ELEMAMD_OK
-               : ELEMAMD_ERR_HIP;
+    return solve_left_lower(is_trans(trans), m, n, A, lda, B, ldb, S(stream));
 }
 
 int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const double* A, int64_t lda, double beta,
@@ -61,7 +69,8 @@ int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const do
     if (up(uplo) != 'L' || up(trans) != 'N') return ELEMAMD_ERR_ARG;
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
-    return launch_syrk_ln_f64(n, k, alpha, A, lda, beta, C, ldc, nullptr, S(stream)) == hipSuccess ? ELEMAMD_OK
+    return launch_syrk_ln_f64(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr, kNoLimit, S(stream)) == hipSuccess
+               ? ELEMAMD_OK
                                                                                                   : ELEMAMD_ERR_HIP;
 }
 
@@ -70,8 +79,8 @@ int elemamd_herk_f32(int mode, int n, int k, float alpha, const float* A, int64_
     if (mode != ELEMAMD_TRAILING_F32 && mode != ELEMAMD_TRAILING_BF16X3) return ELEMAMD_ERR_ARG;
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
-    return launch_syrk_ln_f32(mode == ELEMAMD_TRAILING_BF16X3, n, k, alpha, A, lda, beta, C, ldc, nullptr,
-                              S(stream)) == hipSuccess
+    return launch_syrk_ln_f32(mode == ELEMAMD_TRAILING_BF16X3, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr,
+                              kNoLimit, S(stream)) == hipSuccess
                ? ELEMAMD_OK
                : ELEMAMD_ERR_HIP;
 }
@@ -92,10 +101,7 @@ int elemamd_trsv_f64(char uplo, char trans, char diag, int n, const double* A, i
     if (up(uplo) != 'L' || up(diag) != 'N' || !valid_orient(trans) || incx != 1) return ELEMAMD_ERR_ARG;
     if (n < 0 || lda < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
-    return launch_trsm_left_lower<double>(is_trans(trans), n, 1, A, lda, x, std::max(1, n), nullptr, S(stream)) ==
-                   hipSuccess
-               ? ELEMAMD_OK
-               : ELEMAMD_ERR_HIP;
+    return solve_left_lower(is_trans(trans), n, 1, A, lda, x, std::max(1, n), S(stream));
 }
 
 int elemamd_cholesky_f64(char uplo, int n, double* A, int64_t lda, int nb, int* dinfo, void* stream) {
@@ -106,16 +112,21 @@ int elemamd_cholesky_f64(char uplo, int n, double* A, int64_t lda, int nb, int* 
 int elemamd_solve_after_f64(char uplo, char orientation, int n, int k, const double* A, int64_t lda, double* B,
                             int64_t ldb, void* stream) {
     if (up(uplo) != 'L' || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
-    return solve_after_lower<double>(n, k, A, lda, B, ldb, nullptr, S(stream));
+    if (n < 0 || k < 0) return ELEMAMD_ERR_ARG;
+    void* work = stream_scratch(solve_work_bytes<double>(std::max(n, 1), std::max(k, 1)), S(stream));
+    if (work == nullptr) return ELEMAMD_ERR_HIP;
+    return solve_after_lower<double>(n, k, A, lda, B, ldb, nullptr, work, S(stream));
 }
 
 int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double* A, int64_t lda, double* B, int64_t ldb,
                           int nb, int* dinfo, void* stream) {
     if (up(uplo) != 'L' || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
     if (k < 0 || ldb < std::max(1, n)) return ELEMAMD_ERR_ARG;
+    void* work = stream_scratch(solve_work_bytes<double>(std::max(n, 1), std::max(k, 1)), S(stream));
+    if (work == nullptr) return ELEMAMD_ERR_HIP;
     int rc = cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, S(stream));
     if (rc) return rc;
-    return solve_after_lower<double>(n, k, A, lda, B, ldb, dinfo, S(stream));
+    return solve_after_lower<double>(n, k, A, lda, B, ldb, dinfo, work, S(stream));
 }
 
 static MixedOpts to_opts(const elemamd_ir_opts* o) {

@@ -5,29 +5,44 @@
 
 namespace elemamd {
 
-// potrf.hip -- LVar3Unb on an n x n (n <= 128) diagonal block.
+// Factorization kernels skip their work when *info != 0 && *info <= limit
+// (info is the 1-based failing column): a kernel belonging to the step whose
+// diagonal block ends at column c passes limit = c, so a failure in a later
+// step never cancels it and A ends in the state the reference leaves.
+constexpr int kNoLimit = 0x7fffffff;
+
+// potrf.hip -- LVar3Unb on an n x n (n <= 128) diagonal block starting at
+// global column col_offset (skips if an earlier column failed).
 template <typename T>
 hipError_t launch_potrf_unb(int n, T* A, i64 lda, int* info, int col_offset, hipStream_t s);
 
 // trsm_panel.hip -- B := B * inv(L)^T, L nb x nb lower (nb <= 128), B m x nb.
 template <typename T>
-hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, hipStream_t s);
+hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
+                           hipStream_t s);
 
-// syrk.hip -- lower C := beta*C + alpha*A*A^T.
-hipError_t launch_syrk_ln_f64(int n, int k, double alpha, const double* A, i64 lda, double beta, double* C, i64 ldc,
-                              const int* info, hipStream_t s);
+// syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
+hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
+                              double beta, double* C, i64 ldc, const int* info, int limit, hipStream_t s);
 // float storage; bf16x3 = false: FP32 MFMA, true: split-BF16x3 MFMA with FP32 accumulation.
-hipError_t launch_syrk_ln_f32(bool bf16x3, int n, int k, float alpha, const float* A, i64 lda, float beta, float* C,
-                              i64 ldc, const int* info, hipStream_t s);
+hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, const float* A, i64 lda, const float* B,
+                              i64 ldb, float beta, float* C, i64 ldc, const int* info, int limit, hipStream_t s);
 
 // gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
                               double beta, double* C, i64 ldc, bool lower_only, hipStream_t s);
 
-// trsm_solve.hip -- B := inv(L) B (trans=false) or inv(L^T) B (trans=true).
+// solve.hip -- persistent triangular solves with precomputed inverses of the
+// 64x64 diagonal blocks.  Dinv holds solve_dinv_count(n) elements, ctrl
+// solve_ctrl_count(n, kc) ints (ctrl[1] != 0 after a solve means a bounded wait
+// expired).
+size_t solve_ctrl_count(int n, int kc);
+size_t solve_dinv_count(int n);
 template <typename T>
-hipError_t launch_trsm_left_lower(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const int* info,
-                                  hipStream_t s);
+hipError_t launch_trtri_diag64(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s);
+template <typename T>
+hipError_t launch_trsm_persist(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
+                               int* ctrl, const int* info, hipStream_t s);
 
 // blas1.hip -- elementwise helpers, conversions, residual and norms.
 hipError_t launch_fill_int(int* p, int value, int count, hipStream_t s);

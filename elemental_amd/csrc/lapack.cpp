@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "runtime.hpp"
 
 namespace elemamd {
 
@@ -17,30 +18,67 @@ namespace elemamd {
 // while the bottom-right block is nonempty, take a diagonal block of
 // min(nb, remaining) (partition_impl.hpp:403-424), factor it with LVar3Unb,
 // solve the panel below it, and apply the symmetric rank-nb update to the
-// trailing matrix.  Every launch checks *dinfo so that work after a failing
-// block is skipped.
+// trailing matrix.
+//
+// Scheduled with a look-ahead of one block: after panel j is solved, the
+// caller's stream updates only the next block column with panel j and goes
+// straight on to that block's potrf and panel solve, while an auxiliary
+// stream applies panel j to the rest of the trailing matrix.  The next
+// block-column update waits for the previous rest-update (they write the same
+// elements); everything joins back onto the caller's stream at the end.
+// Every element still receives the updates of panels 0, 1, 2, ... in order.
+template <typename T>
+static hipError_t syrk_launch(int trailing, int m, int n, int k, const T* A, i64 lda, const T* B, T* C, i64 ldc,
+                              const int* info, int limit, hipStream_t s) {
+    if constexpr (sizeof(T) == 8)
+        return launch_syrk_ln_f64(m, n, k, -1.0, (const double*)A, lda, (const double*)B, lda, 1.0, (double*)C, ldc,
+                                  info, limit, s);
+    else
+        return launch_syrk_ln_f32(trailing == kTrailingBf16x3, m, n, k, -1.0f, (const float*)A, lda, (const float*)B,
+                                  lda, 1.0f, (float*)C, ldc, info, limit, s);
+}
+
 template <typename T>
 int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s) {
     if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr) return kErrArg;
     ELEMAMD_HIP_TRY(launch_fill_int(dinfo, 0, 1, s));
+    if (n == 0) return kOk;
+    StreamPair* sp = stream_pair(s);
+    if (sp == nullptr) return kErrHip;
+    // step 0 panel
+    {
+        const int b = std::min(nb, n);
+        ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, A, lda, dinfo, 0, s));
+        if (n > b) ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(n - b, b, A, lda, A + b, lda, dinfo, b, s));
+    }
+    bool rest_pending = false;
     for (int k = 0; k < n; k += nb) {
         const int b = std::min(nb, n - k);
         const int m2 = n - k - b;
-        T* A11 = A + k + (i64)k * lda;
-        ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, A11, lda, dinfo, k, s));
-        if (m2 > 0) {
-            T* A21 = A + (k + b) + (i64)k * lda;
-            T* A22 = A + (k + b) + (i64)(k + b) * lda;
-            ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(m2, b, A11, lda, A21, lda, dinfo, s));
-            hipError_t e;
-            if constexpr (sizeof(T) == 8)
-                e = launch_syrk_ln_f64(m2, b, -1.0, (const double*)A21, lda, 1.0, (double*)A22, lda, dinfo, s);
-            else
-                e = launch_syrk_ln_f32(trailing == kTrailingBf16x3, m2, b, -1.0f, (const float*)A21, lda, 1.0f,
-                                       (float*)A22, lda, dinfo, s);
-            ELEMAMD_HIP_TRY(e);
+        if (m2 <= 0) break;
+        const int limit = k + b;  // this step's work is cancelled only by failures at columns <= k+b-1
+        const T* P = A + (k + b) + (i64)k * lda;  // panel j, m2 x b
+        T* C = A + (k + b) + (i64)(k + b) * lda;  // trailing matrix, m2 x m2
+        const int bn = std::min(nb, m2);          // next block column
+        // the rest-update of the previous step writes the next block column too
+        if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
+        // fork: the rest of the trailing matrix is updated with panel j on aux
+        const bool rest = m2 > bn;
+        if (rest) {
+            ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
+            ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
+            ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2 - bn, m2 - bn, b, P + bn, lda, P + bn, C + bn + (i64)bn * lda,
+                                           lda, dinfo, limit, sp->aux));
+            ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_b, sp->aux));
         }
+        rest_pending = rest;
+        // critical chain on the caller's stream: next block column, its potrf and panel
+        ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2, bn, b, P, lda, P, C, lda, dinfo, limit, s));
+        ELEMAMD_HIP_TRY(launch_potrf_unb<T>(bn, C, lda, dinfo, k + b, s));
+        if (m2 > bn)
+            ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(m2 - bn, bn, C, lda, C + bn, lda, dinfo, k + b + bn, s));
     }
+    if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
     return kOk;
 }
 
@@ -50,18 +88,57 @@ template int cholesky_lower<float>(int, float*, i64, int, int, int*, hipStream_t
 // ----------------------------------------------------------- SolveAfter
 // cholesky::SolveAfter(LOWER, NORMAL) (SolveAfter.hpp:20-75): forward then
 // backward triangular solve with the lower factor.  One column is the Trsv
-// pair in the reference; the blocked kernels handle k = 1 the same way.
+// pair in the reference; the persistent kernels handle k = 1 the same way.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 template <typename T>
-int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, hipStream_t s) {
-    if (n < 0 || k < 0 || ldl < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
-    if (n == 0 || k == 0) return kOk;
-    ELEMAMD_HIP_TRY(launch_trsm_left_lower<T>(false, n, k, L, ldl, B, ldb, dinfo, s));
-    ELEMAMD_HIP_TRY(launch_trsm_left_lower<T>(true, n, k, L, ldl, B, ldb, dinfo, s));
+size_t solve_work_bytes(int n, int k) {
+    return align256(solve_dinv_count(n) * sizeof(T)) + align256(solve_ctrl_count(n, k) * sizeof(int));
+}
+
+// The FP64 solves substitute inside the diagonal blocks (no inverses); the
+// float solves of the refinement use the precomputed inverses.
+template <typename T>
+static constexpr bool kSolveUsesInverse = (sizeof(T) == 4);
+
+template <typename T>
+int solve_prepare(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
+    ELEMAMD_HIP_TRY(launch_trtri_diag64<T>(n, L, ldl, Dinv, s));
     return kOk;
 }
 
-template int solve_after_lower<double>(int, int, const double*, i64, double*, i64, const int*, hipStream_t);
-template int solve_after_lower<float>(int, int, const float*, i64, float*, i64, const int*, hipStream_t);
+template <typename T>
+int solve_apply(bool forward, bool backward, int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
+                int* ctrl, const int* dinfo, hipStream_t s) {
+    if (forward) ELEMAMD_HIP_TRY(launch_trsm_persist<T>(false, n, k, L, ldl, B, ldb, Dinv, ctrl, dinfo, s));
+    if (backward) ELEMAMD_HIP_TRY(launch_trsm_persist<T>(true, n, k, L, ldl, B, ldb, Dinv, ctrl, dinfo, s));
+    return kOk;
+}
+
+template <typename T>
+int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s) {
+    if (n < 0 || k < 0 || ldl < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
+    if (n == 0 || k == 0) return kOk;
+    if (work == nullptr) return kErrWorkspace;
+    T* Dinv = kSolveUsesInverse<T> ? (T*)work : nullptr;
+    int* ctrl = (int*)((char*)work + align256(solve_dinv_count(n) * sizeof(T)));
+    if (kSolveUsesInverse<T>) {
+        int rc = solve_prepare<T>(n, L, ldl, Dinv, s);
+        if (rc) return rc;
+    }
+    return solve_apply<T>(true, true, n, k, L, ldl, B, ldb, Dinv, ctrl, dinfo, s);
+}
+
+template size_t solve_work_bytes<double>(int, int);
+template size_t solve_work_bytes<float>(int, int);
+template int solve_prepare<double>(int, const double*, i64, double*, hipStream_t);
+template int solve_prepare<float>(int, const float*, i64, float*, hipStream_t);
+template int solve_apply<double>(bool, bool, int, int, const double*, i64, double*, i64, const double*, int*,
+                                 const int*, hipStream_t);
+template int solve_apply<float>(bool, bool, int, int, const float*, i64, float*, i64, const float*, int*, const int*,
+                                hipStream_t);
+template int solve_after_lower<double>(int, int, const double*, i64, double*, i64, const int*, void*, hipStream_t);
+template int solve_after_lower<float>(int, int, const float*, i64, float*, i64, const int*, void*, hipStream_t);
 
 // ------------------------------------------------------ mixed precision
 namespace {
@@ -89,6 +166,8 @@ struct MixedWork {
     double* V;     // (m+1) blocks of n x k: Krylov basis
     double* nrm;   // 8k + n + 8 doubles: norms, row sums, scratch
     double* h;     // (m+1) x k Gram-Schmidt coefficients
+    float* Dinv;   // inverses of the 64x64 diagonal blocks of S
+    int* ctrl;     // persistent-solve tickets and flags
     int* dinfo;
     size_t bytes;
 };
@@ -106,6 +185,8 @@ MixedWork carve_mixed(void* work, size_t cap, int n, int k, int m) {
     w.V = c.take<double>(m > 0 ? nk * (size_t)(m + 1) : 0);
     w.nrm = c.take<double>((size_t)8 * k + n + 8);
     w.h = c.take<double>((size_t)(m + 2) * k);
+    w.Dinv = c.take<float>(solve_dinv_count(n));
+    w.ctrl = c.take<int>(solve_ctrl_count(n, k));
     w.dinfo = c.take<int>(4);
     w.bytes = c.off + 256;
     return w;
@@ -127,7 +208,7 @@ struct Ctx {
 int precond_apply(Ctx& c, const double* y, double* z, const double* colscale) {
     const int n = c.n, k = c.k;
     ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, y, n, colscale, c.w.Rf, n, c.s));
-    int rc = solve_after_lower<float>(n, k, c.w.S, n, c.w.Rf, n, nullptr, c.s);
+    int rc = solve_apply<float>(true, true, n, k, c.w.S, n, c.w.Rf, n, c.w.Dinv, c.w.ctrl, nullptr, c.s);
     if (rc) return rc;
     ELEMAMD_HIP_TRY(launch_copy_f32_to_f64_scaled(n, k, c.w.Rf, n, colscale, z, n, c.s));
     return kOk;
@@ -299,6 +380,10 @@ int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, 
         r.shifted = 1;
     }
     r.anorm_inf = c.anorm;
+    if (info == 0) {
+        int rc = solve_prepare<float>(n, w.S, n, w.Dinv, s);
+        if (rc) return rc;
+    }
 
     if (info != 0) {
         // The low-precision factor broke down even after shifting: report it
@@ -341,7 +426,7 @@ int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, 
         if (!gmres) {
             // d = M^{-1} r in float, x += d
             ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, w.R, n, w.nrm + k, w.Rf, n, s));
-            rc = solve_after_lower<float>(n, k, w.S, n, w.Rf, n, nullptr, s);
+            rc = solve_apply<float>(true, true, n, k, w.S, n, w.Rf, n, w.Dinv, w.ctrl, nullptr, s);
             if (rc) return rc;
             ELEMAMD_HIP_TRY(launch_axpy_f32_to_f64_scaled(n, k, w.Rf, n, w.nrm + k, w.X, n, s));
         } else {

@@ -13,8 +13,18 @@ template <typename T>
 int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s);
 
 // cholesky::SolveAfter(LOWER, NORMAL): B := inv(L^T) inv(L) B.
+// `work` holds solve_work_bytes<T>(n, k) bytes of device memory.
 template <typename T>
-int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, hipStream_t s);
+size_t solve_work_bytes(int n, int k);
+template <typename T>
+int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s);
+// The two halves of it for repeated solves with one factor: the diagonal
+// block inverses are computed once (solve_prepare) and reused (solve_apply).
+template <typename T>
+int solve_prepare(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s);
+template <typename T>
+int solve_apply(bool forward, bool backward, int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
+                int* ctrl, const int* dinfo, hipStream_t s);
 
 struct MixedOpts {
     int nb = 128;

@@ -1,15 +1,21 @@
 // Diagonal-block Cholesky: Elemental's cholesky::LVar3Unb
 // (include/elemental/lapack-like/Cholesky/LVar3.hpp:19-49) on an n x n block,
-// n <= 128, held entirely in LDS (128 x 128 doubles = 128 KiB of gfx950's
-// 160 KiB per CU).
+// n <= 128, by one 512-thread workgroup.
 //
 // Rounding order matches the reference loop exactly: per column j the pivot
 // is sqrt(a_jj), the column below it is divided by that pivot, and every
 // trailing element takes a_ik := a_ik - a_ij*a_kj as a separately rounded
 // product and difference, in increasing j.  Contraction into FMA is switched
-// off in this kernel so that the block result is bit-identical to the
-// reference loop run in IEEE double on the CPU.
+// off, so the block result is bit-identical to the reference loop run in IEEE
+// double on the CPU.
 //
+// Layout: the block lives in registers.  Thread (tr, tc) = (t % 32, t / 32)
+// owns the 32 elements (i, k) = (tr + 32a, tc + 16b), a < 4, b < 8, a 2-D
+// cyclic distribution that keeps the shrinking trailing triangle balanced.
+// Column j's owners are the 32 consecutive threads with tc = j % 16 (one
+// wave): they take the pivot from its owner with a lane read, divide their
+// elements and publish the finished column in LDS.  One barrier per column,
+// then every thread applies the rank-1 update to its active elements.
 // Storage may be double or float (the mixed-precision factor keeps its
 // working matrix in float); arithmetic is always double.
 #include "common.hpp"
@@ -20,56 +26,110 @@ namespace elemamd {
 constexpr int kPotrfMaxN = 128;
 constexpr int kPotrfThreads = 512;
 
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Columns j = 16*A + r, r = 0..15, with A fixed at compile time so that
+// the owner's register indices (a = A/2, b = A) are static.
+template <int A>
+__device__ __forceinline__ bool potrf_col_group(double (&e)[4][8], int n, int tr, int tc, int wave, int lane,
+                                                double (*col)[kPotrfMaxN], int* sfail) {
+#pragma clang fp contract(off)
+    constexpr int a_own = A / 2;  // row block of the pivot (j / 32)
+    constexpr int b_own = A;      // column block of column j (j / 16)
+    for (int r = 0; r < 16; ++r) {
+        const int j = 16 * A + r;
+        if (j >= n) return true;
+        double* cj = col[j & 1];
+        // owners of column j: tc == r, threads 32r .. 32r+31, wave r/2
+        if (wave == (r >> 1)) {
+            const int own_lane = ((r & 1) << 5) + (j & 31);  // thread tr = j%32, tc = r
+            const double ajj = readlane_d(e[a_own][b_own], own_lane);
+            if (tc == r) {
+                if (ajj <= 0.0) {
+                    if (tr == 0) *sfail = j + 1;
+                } else {
+                    const double alpha = sqrt(ajj);
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const int i = tr + 32 * a;
+                        if (i > j && i < n) {
+                            e[a][b_own] = e[a][b_own] / alpha;
+                            cj[i] = e[a][b_own];
+                        } else if (i == j) {
+                            e[a][b_own] = alpha;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (*sfail != 0) return false;
+        // rank-1 update of the elements with j < k <= i < n.  All column-j
+        // values this thread needs are read from LDS first (one wait), then
+        // the masked products are applied from registers.
+        double ri[4], ck[8];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) ri[a] = cj[tr + 32 * a];
+#pragma unroll
+        for (int b = A; b < 8; ++b) ck[b] = cj[tc + 16 * b];
+#pragma unroll
+        for (int b = A; b < 8; ++b) {
+            const int k = tc + 16 * b;
+            if (16 * b + 15 <= j) continue;  // whole column block finished (uniform)
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                if (32 * a + 31 <= j) continue;  // whole row block finished (uniform)
+                const int i = tr + 32 * a;
+                if (k > j && i >= k && i < n) e[a][b] = e[a][b] - ri[a] * ck[b];
+            }
+        }
+    }
+    return true;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kPotrfThreads) void potrf_unb_kernel(int n, T* __restrict__ A, i64 lda,
                                                                   int* __restrict__ info, int col_offset) {
 #pragma clang fp contract(off)
-    __shared__ double s[kPotrfMaxN * kPotrfMaxN];
-    if (info != nullptr && *info != 0) return;  // an earlier block failed
+    __shared__ double col[2][kPotrfMaxN];
+    __shared__ int sfail;
+    if (info != nullptr && *info != 0 && *info <= col_offset) return;  // an earlier block failed
     const int t = threadIdx.x;
-    const int ld = n;
-
-    for (int idx = t; idx < n * n; idx += kPotrfThreads) {
-        const int i = idx % n, j = idx / n;
-        if (i >= j) s[i + j * ld] = (double)A[i + (i64)j * lda];
-    }
+    const int tr = t & 31, tc = t >> 5;
+    const int wave = t >> 6, lane = t & 63;
+    if (t == 0) sfail = 0;
+    double e[4][8];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int i = tr + 32 * a, k = tc + 16 * b;
+            e[a][b] = (i < n && k <= i) ? (double)A[i + (i64)k * lda] : 0.0;
+        }
     __syncthreads();
-
-    // Threads 0..127 own rows; the four groups t/128 split the columns k of
-    // the rank-1 update round-robin, so a wave touches 64 consecutive rows
-    // of one column (conflict-free LDS access, broadcast of a_kj).
-    const int r = t & (kPotrfMaxN - 1);
-    const int q = t >> 7;
-    int fail = 0;
-    int done = n;
-    for (int j = 0; j < n; ++j) {
-        const double ajj = s[j + j * ld];
-        if (ajj <= 0.0) {  // uniform: every thread read the same value
-            fail = j + 1;
-            done = j;
-            break;
-        }
-        const double alpha = sqrt(ajj);
-        __syncthreads();  // all reads of a_jj precede its overwrite
-        if (t == 0) s[j + j * ld] = alpha;
-        for (int i = j + 1 + t; i < n; i += kPotrfThreads) s[i + j * ld] = s[i + j * ld] / alpha;
-        __syncthreads();
-        const int i = j + 1 + r;
-        if (i < n) {
-            const double aij = s[i + j * ld];
-            for (int k = j + 1 + q; k <= i; k += 4) s[i + k * ld] = s[i + k * ld] - aij * s[k + j * ld];
-        }
-        __syncthreads();
-    }
-    (void)done;
+    bool ok = potrf_col_group<0>(e, n, tr, tc, wave, lane, col, &sfail);
+    if (ok) ok = potrf_col_group<1>(e, n, tr, tc, wave, lane, col, &sfail);
+    if (ok) ok = potrf_col_group<2>(e, n, tr, tc, wave, lane, col, &sfail);
+    if (ok) ok = potrf_col_group<3>(e, n, tr, tc, wave, lane, col, &sfail);
+    if (ok) ok = potrf_col_group<4>(e, n, tr, tc, wave, lane, col, &sfail);
+    if (ok) ok = potrf_col_group<5>(e, n, tr, tc, wave, lane, col, &sfail);
+    if (ok) ok = potrf_col_group<6>(e, n, tr, tc, wave, lane, col, &sfail);
+    if (ok) ok = potrf_col_group<7>(e, n, tr, tc, wave, lane, col, &sfail);
     // Write back the lower triangle.  On failure at column j this is the
     // state the reference leaves behind when LVar3Unb throws: columns < j
     // factored and the trailing triangle updated by them.
-    for (int idx = t; idx < n * n; idx += kPotrfThreads) {
-        const int i = idx % n, j = idx / n;
-        if (i >= j) A[i + (i64)j * lda] = (T)s[i + j * ld];
-    }
-    if (fail && t == 0 && info != nullptr) *info = col_offset + fail;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int i = tr + 32 * a, k = tc + 16 * b;
+            if (i < n && k <= i) A[i + (i64)k * lda] = (T)e[a][b];
+        }
+    if (!ok && t == 0 && info != nullptr) *info = col_offset + sfail;
 }
 
 template <typename T>

@@ -1,0 +1,24 @@
+// Per-(device, stream, thread) device scratch for entry points whose C ABI
+// takes no workspace (the drop-in Trsm / SolveAfter / HPDSolve calls).
+#pragma once
+
+#include "common.hpp"
+
+namespace elemamd {
+
+// Device memory of at least `bytes`, owned by the calling thread and reserved
+// for work enqueued on `stream`; grows on demand (the old buffer is released
+// only after `stream` has drained).  Returns nullptr on allocation failure.
+void* stream_scratch(size_t bytes, hipStream_t stream);
+
+// An auxiliary stream (default priority) and two timing-free events paired
+// with `stream` for the look-ahead factorization: work forked onto `aux`
+// after an event recorded on `stream` and joined back the same way.
+struct StreamPair {
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+};
+// Owned by the calling thread; nullptr members on failure.
+StreamPair* stream_pair(hipStream_t stream);
+
+}  // namespace elemamd

@@ -1,0 +1,351 @@
+// Persistent triangular solves with the Cholesky factor, one launch per
+// direction:
+//   forward  B := inv(L) B     (Trsm(LEFT, LOWER, NORMAL,  NON_UNIT) -> dtrsm_('L','L','N','N'))
+//   backward B := inv(L^T) B   (Trsm(LEFT, LOWER, ADJOINT, NON_UNIT) -> dtrsm_('L','L','T','N'))
+// as used by cholesky::SolveAfter (include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75).
+//
+// The rows are cut into 64-row blocks.  Each workgroup owns one block row
+// and one 16-column slab of the right-hand sides; it takes a ticket from a
+// device counter, so it only ever waits for workgroups that started before
+// it.  It accumulates the contributions of the already-solved blocks in the
+// order they become available (flags in device memory), each 64x64 tile of L
+// staged through double-buffered LDS and multiplied with MFMA; then it solves
+// its 64x64 diagonal block, publishes the solved block and raises its flag.
+// L is read from HBM once per slab.
+//
+// Diagonal block, two variants:
+//   USE_INV = false (the FP64 solves): substitution, one wave per four
+//     right-hand sides, lanes = rows, pivots applied as reciprocal products.
+//     Backward stable like the reference BLAS loop, and exact on integer
+//     problems whose diagonal is a power of two.
+//   USE_INV = true (the FP32 correction solves of iterative refinement):
+//     a product with the precomputed inverse of the diagonal block (MFMA),
+//     which shortens the sequential critical path; refinement absorbs its
+//     larger rounding error.
+//
+// Every wait is bounded: after kSpinLimit polls the workgroup records an error
+// in ctrl[1] and continues, so a lost flag gives a wrong answer and an error
+// code, never a hang.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace elemamd {
+
+constexpr int kSB = 64;        // block rows
+constexpr int kSC = 16;        // right-hand-side columns per workgroup
+constexpr int kSLd = kSB + 1;
+constexpr long kSpinLimit = 1L << 26;
+
+// MFMA 16x16x4 of the storage type; output row index of accumulator v.
+__device__ inline d4_t mfma16(double a, double b, d4_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline f4_t mfma16(float a, float b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+template <typename T>
+__device__ inline int mfma_row(int lane, int v) {
+    return sizeof(T) == 8 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+}
+__device__ inline double wave_bcast(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline float wave_bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<double> {
+    typedef d4_t type;
+};
+template <>
+struct Acc4<float> {
+    typedef f4_t type;
+};
+
+// ------------------------------------------------ inverse of diagonal blocks
+// Dinv + I*64*64 (column-major, ld 64) := inv(L(I*64.., I*64..)), upper part 0.
+// Column c is built row by row: W(i,c) = -(sum_{c<=k<i} L(i,k) W(k,c)) / L(i,i),
+// the dot product split over four thread groups and reduced in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void trtri_diag64_kernel(int n, const T* __restrict__ L, i64 ldl,
+                                                           T* __restrict__ Dinv) {
+    __shared__ T sL[kSB][kSLd];  // sL[i][k]
+    __shared__ T sW[kSB][kSLd];  // sW[k][c]
+    __shared__ T red[4][kSB];
+    const int I = blockIdx.x, t = threadIdx.x;
+    const i64 r0 = (i64)I * kSB;
+    const int bs = (int)min<i64>(kSB, n - r0);
+    for (int idx = t; idx < kSB * kSB; idx += 256) {
+        const int i = idx % kSB, k = idx / kSB;
+        sL[i][k] = (i < bs && k < bs && k <= i) ? L[r0 + i + (r0 + k) * ldl] : (i == k ? T(1) : T(0));
+        sW[i][k] = T(0);
+    }
+    __syncthreads();
+    const int c = t & 63, part = t >> 6;
+    if (part == 0) sW[c][c] = T(1) / sL[c][c];
+    __syncthreads();
+    for (int i = 1; i < kSB; ++i) {
+        T s = T(0);
+        if (c < i)
+            for (int k = c + part; k < i; k += 4) s += sL[i][k] * sW[k][c];
+        red[part][c] = s;
+        __syncthreads();
+        if (part == 0 && c < i) sW[i][c] = -(red[0][c] + red[1][c] + red[2][c] + red[3][c]) / sL[i][i];
+        __syncthreads();
+    }
+    T* out = Dinv + (i64)I * kSB * kSB;
+    for (int idx = t; idx < kSB * kSB; idx += 256) {
+        const int i = idx % kSB, k = idx / kSB;
+        out[idx] = (i < bs && k < bs) ? sW[i][k] : T(0);
+    }
+}
+
+// ------------------------------------------------------- persistent solve
+template <typename T, bool TRANS, bool USE_INV>
+__global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const T* __restrict__ L, i64 ldl,
+                                                           T* __restrict__ B, i64 ldb, const T* __restrict__ Dinv,
+                                                           int* __restrict__ ctrl, const int* __restrict__ info) {
+    typedef typename Acc4<T>::type acc_t;
+    __shared__ T sM[2][kSB][kSLd];  // sM[q][p]: contraction index q, output row p
+    __shared__ T sY[2][kSB][kSC + 1];
+    __shared__ int s_ord;
+    if (info != nullptr && *info != 0) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int nT = (int)ceil_div(n, kSB);
+    const int CY = (int)ceil_div(kc, kSC);
+    if (t == 0) s_ord = atomicAdd(&ctrl[0], 1);
+    __syncthreads();
+    const int ord = s_ord;
+    const int cy = ord % CY;
+    const int I = TRANS ? (nT - 1 - ord / CY) : (ord / CY);
+    const i64 r0 = (i64)I * kSB;
+    const int c_base = cy * kSC;
+    const int kcl = min(kSC, kc - c_base);
+    int* flags = ctrl + 2;
+
+    // contraction tile of block J (rows q of the solved block, rows p of ours)
+    T gm[16], gy[4];
+    auto gload = [&](int J) {
+        const i64 q0 = (i64)J * kSB;
+        if (!TRANS) {  // M[q][p] = L(r0+p, q0+q), lanes along p
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int qq = w + 4 * s;
+                const i64 p = r0 + lane;
+                gm[s] = (p < n) ? L[p + (q0 + qq) * ldl] : T(0);
+            }
+        } else {  // M[q][p] = L(q0+q, r0+p), lanes along q
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int pp = w + 4 * s;
+                const i64 q = q0 + lane;
+                gm[s] = (q < n) ? L[q + (r0 + pp) * ldl] : T(0);
+            }
+        }
+    };
+    auto gload_y = [&](int J) {
+        const i64 q0 = (i64)J * kSB;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int c = w + 4 * s;
+            const i64 q = q0 + lane;
+            // agent-coherent load: the block was written by another workgroup,
+            // possibly on another XCD (its L2 may hold older lines of it)
+            gy[s] = (q < n && c < kcl)
+                        ? __hip_atomic_load(&B[q + (i64)(c_base + c) * ldb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : T(0);
+        }
+    };
+    auto sstore = [&](int buf) {
+        if (!TRANS) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) sM[buf][w + 4 * s][lane] = gm[s];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) sM[buf][lane][w + 4 * s] = gm[s];
+        }
+    };
+    auto sstore_y = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) sY[buf][lane][w + 4 * s] = gy[s];
+    };
+    auto wait_flag = [&](int J) {
+        if (t == 0) {
+            int* f = &flags[J * CY + cy];
+            long spins = 0;
+            // relaxed polling (no cache invalidation per poll), one acquire after
+            while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSpinLimit) {
+                    atomicOr(&ctrl[1], 1);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+    };
+
+    // D[c][p]: MFMA rows = right-hand sides, columns = output rows; wave w owns rows 16w..16w+15
+    acc_t acc = {};
+    const int nJ = TRANS ? (nT - 1 - I) : I;
+    if (nJ > 0) {
+        gload(TRANS ? nT - 1 : 0);
+        sstore(0);
+    }
+    for (int jj = 0; jj < nJ; ++jj) {
+        const int J = TRANS ? (nT - 1 - jj) : jj;
+        const int buf = jj & 1;
+        // next tile (the last iteration reloads its own tile into the spare
+        // buffer, which is never read again: the loop body stays branch-free)
+        const int Jn = ((jj + 1) < nJ) ? (TRANS ? J - 1 : J + 1) : J;
+        gload(Jn);
+        wait_flag(J);  // includes a barrier: sM[buf] is complete too
+        gload_y(J);
+        sstore_y(buf);
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kSB / 4; ++ks) {
+            const int q = 4 * ks + (lane >> 4);
+            acc = mfma16(sY[buf][q][lane & 15], sM[buf][q][16 * w + (lane & 15)], acc);
+        }
+        sstore(buf ^ 1);
+    }
+    // A dependent zero-operand MFMA right before the accumulator is read: the
+    // reads below then follow an MFMA in the same basic block, where the
+    // compiler's wait-state padding for MFMA results is applied correctly
+    // (across the loop's back edge it was found to be short for FP64 MFMA).
+    acc = mfma16(T(0), T(0), acc);
+    // R = B_I - acc into sY[0] as [p][c] -> contraction operand of the inverse
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = mfma_row<T>(lane, v);
+        const int p = 16 * w + (lane & 15);
+        const i64 row = r0 + p;
+        const T bv = (row < n && c < kcl) ? B[row + (i64)(c_base + c) * ldb] : T(0);
+        sY[0][p][c] = bv - acc[v];
+    }
+    if constexpr (USE_INV) {
+        // forward Y(p) = sum_q Dinv(p,q) R(q); backward X(p) = sum_q Dinv(q,p) R(q)
+        const T* Di = Dinv + (i64)I * kSB * kSB;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int qq = w + 4 * s;
+            if (!TRANS) sM[1][qq][lane] = Di[lane + qq * kSB];  // M[q][p] = Dinv(p, q)
+            else sM[1][qq][lane] = Di[qq + lane * kSB];         // M[q][p] = Dinv(q, p)
+        }
+        __syncthreads();
+        acc_t y = {};
+#pragma unroll
+        for (int ks = 0; ks < kSB / 4; ++ks) {
+            const int q = 4 * ks + (lane >> 4);
+            y = mfma16(sY[0][q][lane & 15], sM[1][q][16 * w + (lane & 15)], y);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = mfma_row<T>(lane, v);
+            const i64 row = r0 + 16 * w + (lane & 15);
+            if (row < n && c < kcl) B[row + (i64)(c_base + c) * ldb] = y[v];
+        }
+    } else {
+        // Lower-triangular system of the block in sM[1][k][i] = M(i, k) with
+        // identity on padded rows.  Forward: M = L_II.  Backward: L_II^T x = r
+        // is solved as the same lower-triangular sweep on reversed indices,
+        // M(a, b) = L_II(63-b, 63-a), lane a holding row 63-a.
+        const int bs = (int)min<i64>(kSB, n - r0);
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int kk = w + 4 * s;
+            const int gi = TRANS ? (kSB - 1 - kk) : lane;  // row of L_II
+            const int gk = TRANS ? (kSB - 1 - lane) : kk;  // column of L_II
+            T v = T(0);
+            if (gi < bs && gk < bs) {
+                if (lane >= kk) v = L[r0 + gi + (r0 + gk) * ldl];
+            } else if (lane == kk) {
+                v = T(1);
+            }
+            sM[1][kk][lane] = v;
+        }
+        __syncthreads();
+        if (t < kSB) sY[1][t][0] = T(1) / sM[1][t][t];
+        __syncthreads();
+        const int myrow = TRANS ? (kSB - 1 - lane) : lane;
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = sY[0][myrow][4 * w + u];
+        for (int kk = 0; kk < kSB; ++kk) {
+            const T rk = sY[1][kk][0];
+            const T lik = sM[1][kk][lane];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T y = wave_bcast(v[u], kk) * rk;
+                if (lane == kk) v[u] = y;
+                else if (lane > kk) v[u] -= lik * y;
+            }
+        }
+        const i64 row = r0 + myrow;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = 4 * w + u;
+            if (row < n && c < kcl) B[row + (i64)(c_base + c) * ldb] = v[u];
+        }
+    }
+    // every thread's stores complete (workgroup-scope release), then one
+    // agent-scope release by the flag writer makes the block visible to all XCDs
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&flags[I * CY + cy], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+size_t solve_ctrl_ints(int n, int kc) {
+    return 2 + (size_t)ceil_div(n, kSB) * ceil_div(kc, kSC);
+}
+
+template <typename T>
+hipError_t launch_trtri_diag64(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(trtri_diag64_kernel<T>, dim3((unsigned)ceil_div(n, kSB)), dim3(256), 0, s, n, L, ldl, Dinv);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_trsm_persist(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
+                               int* ctrl, const int* info, hipStream_t s) {
+    if (n <= 0 || kc <= 0) return hipSuccess;
+    const int nctrl = (int)(2 + ceil_div(n, kSB) * ceil_div(kc, kSC));
+    hipError_t e = launch_fill_int(ctrl, 0, nctrl, s);
+    if (e != hipSuccess) return e;
+    const unsigned grid = (unsigned)(ceil_div(n, kSB) * ceil_div(kc, kSC));
+    const bool inv = Dinv != nullptr;
+    if (trans) {
+        if (inv)
+            hipLaunchKernelGGL((trsm_persist_kernel<T, true, true>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B, ldb,
+                               Dinv, ctrl, info);
+        else
+            hipLaunchKernelGGL((trsm_persist_kernel<T, true, false>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B,
+                               ldb, Dinv, ctrl, info);
+    } else {
+        if (inv)
+            hipLaunchKernelGGL((trsm_persist_kernel<T, false, true>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B,
+                               ldb, Dinv, ctrl, info);
+        else
+            hipLaunchKernelGGL((trsm_persist_kernel<T, false, false>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B,
+                               ldb, Dinv, ctrl, info);
+    }
+    return hipGetLastError();
+}
+
+size_t solve_ctrl_count(int n, int kc) { return 2 + (size_t)ceil_div(n, kSB) * ceil_div(kc, kSC); }
+size_t solve_dinv_count(int n) { return (size_t)ceil_div(n, kSB) * kSB * kSB; }
+
+template hipError_t launch_trtri_diag64<double>(int, const double*, i64, double*, hipStream_t);
+template hipError_t launch_trtri_diag64<float>(int, const float*, i64, float*, hipStream_t);
+template hipError_t launch_trsm_persist<double>(bool, int, int, const double*, i64, double*, i64, const double*, int*,
+                                                const int*, hipStream_t);
+template hipError_t launch_trsm_persist<float>(bool, int, int, const float*, i64, float*, i64, const float*, int*,
+                                               const int*, hipStream_t);
+
+}  // namespace elemamd

@@ -1,14 +1,18 @@
 // Trailing update of the blocked Cholesky (the dominant kernel):
-//   C := beta*C + alpha*A*A^T, lower triangle of C only, A n x k.
+//   C := beta*C + alpha*A*B^T on the lower trapezoid of C (entries with
+//   j <= i), C m x n with n <= m, A m x k, B n x k.  With B = A and m = n this is
+//   the lower SYRK; with n = nb it is the look-ahead update of the next block
+//   column, whose top nb x nb block is itself lower triangular.
 // This is Herk(LOWER, NORMAL, -1, A21, 1, A22)
 // (include/elemental/lapack-like/Cholesky/LVar3.hpp:114) which the reference
 // hands to dsyrk_('L','N') (include/elemental/blas-like/level3/Syrk.hpp:45-48,
 // src/core/imports/blas.cpp:1043-1050).  The strictly upper triangle of C is
 // never read or written.
 //
-// CDNA4 mapping: one 256-thread workgroup per 128x128 lower tile of C (tiles
-// above the diagonal are never launched: the grid enumerates the T(T+1)/2
-// lower tiles), four waves each owning a 64x64 quadrant as 4x4 MFMA blocks
+// CDNA4 mapping: one 256-thread workgroup per 128x128 lower tile of C (for a
+// square C the grid enumerates only the T(T+1)/2 lower tiles; otherwise it is
+// Tm x Tn and tiles above the diagonal exit), four waves each owning a 64x64
+// quadrant as 4x4 MFMA blocks
 // of 16x16.  K is streamed through double-buffered LDS in chunks of 16.
 // MFMA "rows" index C's columns and MFMA "columns" index C's rows, so each
 // lane's four accumulator values land in four columns at one row and the 16
@@ -28,6 +32,9 @@ constexpr int kSyrkTile = 128;
 constexpr int kSyrkKC = 16;
 constexpr int kSyrkLdp = kSyrkTile + 16;  // 144 doubles: k-rows start 128 B apart in bank space
 
+// Tile coordinates of this workgroup; false if the tile lies above the diagonal.
+__device__ inline bool syrk_tile(int m, int n, int& ti, int& tj);
+
 __device__ inline void tri_tile(int t, int& ti, int& tj) {
     // t = ti*(ti+1)/2 + tj, 0 <= tj <= ti
     int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
@@ -37,16 +44,25 @@ __device__ inline void tri_tile(int t, int& ti, int& tj) {
     tj = t - (int)((i64)r * (r + 1) / 2);
 }
 
+__device__ inline bool syrk_tile(int m, int n, int& ti, int& tj) {
+    if (m == n) {
+        tri_tile(blockIdx.x, ti, tj);
+        return true;
+    }
+    ti = blockIdx.x;
+    tj = blockIdx.y;
+    return ti >= tj;
+}
+
 // ---------------------------------------------------------------- FP64
-__global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int n, int k, double alpha,
-                                                             const double* __restrict__ A, i64 lda,
-                                                             double beta, double* __restrict__ C, i64 ldc,
-                                                             const int* __restrict__ info) {
-    if (info != nullptr && *info != 0) return;
+__global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
+        const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
+        double* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
+    if (info != nullptr && *info != 0 && *info <= limit) return;
     __shared__ double sI[2][kSyrkKC][kSyrkLdp];
     __shared__ double sJ[2][kSyrkKC][kSyrkLdp];
     int ti, tj;
-    tri_tile(blockIdx.x, ti, tj);
+    if (!syrk_tile(m, n, ti, tj)) return;
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64;  // MFMA M offset = C column offset
@@ -56,9 +72,9 @@ __global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int n, int k, doubl
     // global -> register staging: thread owns tile row lr, k rows lk + 2q
     const int lr = tid & 127;
     const int lk = tid >> 7;
-    const bool rowI = (i0 + lr) < n, rowJ = (j0 + lr) < n;
+    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
     const double* pI = A + (i64)(i0 + lr);
-    const double* pJ = A + (i64)(j0 + lr);
+    const double* pJ = B + (i64)(j0 + lr);
     double gI[8], gJ[8];
 
     auto gload = [&](int kc0) {
@@ -121,7 +137,7 @@ __global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int n, int k, doubl
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int j = j0 + wj + 16 * mb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
-                if (i < n && j <= i) {
+                if (i < m && j < n && j <= i) {
                     double* p = C + (i64)i + (i64)j * ldc;
                     const double upd = alpha * acc[mb][nb][v];
                     *p = (beta == 0.0) ? upd : beta * (*p) + upd;
@@ -131,13 +147,17 @@ __global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int n, int k, doubl
     }
 }
 
-hipError_t launch_syrk_ln_f64(int n, int k, double alpha, const double* A, i64 lda, double beta, double* C,
-                              i64 ldc, const int* info, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    const i64 T = ceil_div(n, kSyrkTile);
-    const i64 tiles = T * (T + 1) / 2;
-    hipLaunchKernelGGL(syrk_ln_f64_kernel, dim3((unsigned)tiles), dim3(256), 0, s, n, k, alpha, A, lda, beta, C,
-                       ldc, info);
+static dim3 syrk_grid(int m, int n) {
+    const i64 Tm = ceil_div(m, kSyrkTile), Tn = ceil_div(n, kSyrkTile);
+    if (m == n) return dim3((unsigned)(Tm * (Tm + 1) / 2));
+    return dim3((unsigned)Tm, (unsigned)Tn);
+}
+
+hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
+                              double beta, double* C, i64 ldc, const int* info, int limit, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(syrk_ln_f64_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                       ldc, info, limit);
     return hipGetLastError();
 }
 
@@ -145,22 +165,22 @@ hipError_t launch_syrk_ln_f64(int n, int k, double alpha, const double* A, i64 l
 // ---------------------------------------------------------------- FP32
 constexpr int kSyrkLdpF = kSyrkTile + 16;  // 144 floats: k-rows 16 banks apart
 
-__global__ __launch_bounds__(256, 2) void syrk_ln_f32_kernel(int n, int k, float alpha, const float* __restrict__ A,
-                                                             i64 lda, float beta, float* __restrict__ C, i64 ldc,
-                                                             const int* __restrict__ info) {
-    if (info != nullptr && *info != 0) return;
+__global__ __launch_bounds__(256, 2) void syrk_ln_f32_kernel(int m, int n, int k, float alpha,
+        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
+    if (info != nullptr && *info != 0 && *info <= limit) return;
     __shared__ float sI[2][kSyrkKC][kSyrkLdpF];
     __shared__ float sJ[2][kSyrkKC][kSyrkLdpF];
     int ti, tj;
-    tri_tile(blockIdx.x, ti, tj);
+    if (!syrk_tile(m, n, ti, tj)) return;
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
     const bool skip = (ti == tj) && (wj > wi);
     const int lr = tid & 127, lk = tid >> 7;
-    const bool rowI = (i0 + lr) < n, rowJ = (j0 + lr) < n;
+    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
     const float* pI = A + (i64)(i0 + lr);
-    const float* pJ = A + (i64)(j0 + lr);
+    const float* pJ = B + (i64)(j0 + lr);
     float gI[8], gJ[8];
     auto gload = [&](int kc0) {
 #pragma unroll
@@ -219,7 +239,7 @@ __global__ __launch_bounds__(256, 2) void syrk_ln_f32_kernel(int n, int k, float
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int j = j0 + wj + 16 * mb + 4 * (lane >> 4) + v;
-                if (i < n && j <= i) {
+                if (i < m && j < n && j <= i) {
                     float* p = C + (i64)i + (i64)j * ldc;
                     const float upd = alpha * acc[mb][nb][v];
                     *p = (beta == 0.0f) ? upd : beta * (*p) + upd;
@@ -239,24 +259,24 @@ constexpr int kBfLd = kBfKC + 8;  // 40 bf16 = 80 B per row: 8 lanes of 16 B hit
 
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
-__global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3_kernel(int n, int k, float alpha, const float* __restrict__ A,
-                                                                i64 lda, float beta, float* __restrict__ C, i64 ldc,
-                                                                const int* __restrict__ info) {
-    if (info != nullptr && *info != 0) return;
+__global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3_kernel(int m, int n, int k, float alpha,
+        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
+    if (info != nullptr && *info != 0 && *info <= limit) return;
     __shared__ __attribute__((aligned(16))) unsigned short sIh[kSyrkTile][kBfLd];
     __shared__ __attribute__((aligned(16))) unsigned short sIl[kSyrkTile][kBfLd];
     __shared__ __attribute__((aligned(16))) unsigned short sJh[kSyrkTile][kBfLd];
     __shared__ __attribute__((aligned(16))) unsigned short sJl[kSyrkTile][kBfLd];
     int ti, tj;
-    tri_tile(blockIdx.x, ti, tj);
+    if (!syrk_tile(m, n, ti, tj)) return;
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
     const bool skip = (ti == tj) && (wj > wi);
     const int lr = tid & 127, kh = tid >> 7;  // row, k half (16 values)
-    const bool rowI = (i0 + lr) < n, rowJ = (j0 + lr) < n;
+    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
     const float* pI = A + (i64)(i0 + lr);
-    const float* pJ = A + (i64)(j0 + lr);
+    const float* pJ = B + (i64)(j0 + lr);
     float gI[16], gJ[16];
     auto gload = [&](int kc0) {
 #pragma unroll
@@ -329,7 +349,7 @@ __global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3_kernel(int n, int k, fl
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int j = j0 + wj + 16 * mb + 4 * (lane >> 4) + v;
-                if (i < n && j <= i) {
+                if (i < m && j < n && j <= i) {
                     float* p = C + (i64)i + (i64)j * ldc;
                     const float upd = alpha * acc[mb][nb][v];
                     *p = (beta == 0.0f) ? upd : beta * (*p) + upd;
@@ -338,17 +358,15 @@ __global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3_kernel(int n, int k, fl
         }
 }
 
-hipError_t launch_syrk_ln_f32(bool bf16x3, int n, int k, float alpha, const float* A, i64 lda, float beta, float* C,
-                              i64 ldc, const int* info, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    const i64 T = ceil_div(n, kSyrkTile);
-    const i64 tiles = T * (T + 1) / 2;
+hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, const float* A, i64 lda, const float* B,
+                              i64 ldb, float beta, float* C, i64 ldc, const int* info, int limit, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
     if (bf16x3)
-        hipLaunchKernelGGL(syrk_ln_bf16x3_kernel, dim3((unsigned)tiles), dim3(256), 0, s, n, k, alpha, A, lda, beta, C,
-                           ldc, info);
+        hipLaunchKernelGGL(syrk_ln_bf16x3_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
+                           C, ldc, info, limit);
     else
-        hipLaunchKernelGGL(syrk_ln_f32_kernel, dim3((unsigned)tiles), dim3(256), 0, s, n, k, alpha, A, lda, beta, C,
-                           ldc, info);
+        hipLaunchKernelGGL(syrk_ln_f32_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
+                           ldc, info, limit);
     return hipGetLastError();
 }
 

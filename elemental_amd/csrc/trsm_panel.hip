@@ -60,11 +60,12 @@ struct RltSweep<32, FULL> {
 
 template <typename T, bool FULL>
 __global__ __launch_bounds__(256) void trsm_rlt_kernel(int m, int nb, const T* __restrict__ L, i64 ldl,
-                                                       T* __restrict__ B, i64 ldb, const int* __restrict__ info) {
+                                                       T* __restrict__ B, i64 ldb, const int* __restrict__ info,
+                                                       int limit) {
 #pragma clang fp contract(off)
     __shared__ double sL[kTrsmMaxN * kTrsmMaxN];
     __shared__ double sRinv[kTrsmMaxN];
-    if (info != nullptr && *info != 0) return;
+    if (info != nullptr && *info != 0 && *info <= limit) return;
     const int t = threadIdx.x;
     const int n = nb;
     for (int idx = t; idx < n * n; idx += 256) {
@@ -94,17 +95,18 @@ __global__ __launch_bounds__(256) void trsm_rlt_kernel(int m, int nb, const T* _
 }
 
 template <typename T>
-hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, hipStream_t s) {
+hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
+                           hipStream_t s) {
     if (m <= 0 || nb <= 0) return hipSuccess;
     dim3 grid((unsigned)ceil_div(m, kTrsmRowsPerBlock));
     if (nb == kTrsmMaxN)
-        hipLaunchKernelGGL((trsm_rlt_kernel<T, true>), grid, dim3(256), 0, s, m, nb, L, ldl, B, ldb, info);
+        hipLaunchKernelGGL((trsm_rlt_kernel<T, true>), grid, dim3(256), 0, s, m, nb, L, ldl, B, ldb, info, limit);
     else
-        hipLaunchKernelGGL((trsm_rlt_kernel<T, false>), grid, dim3(256), 0, s, m, nb, L, ldl, B, ldb, info);
+        hipLaunchKernelGGL((trsm_rlt_kernel<T, false>), grid, dim3(256), 0, s, m, nb, L, ldl, B, ldb, info, limit);
     return hipGetLastError();
 }
 
-template hipError_t launch_trsm_rlt<double>(int, int, const double*, i64, double*, i64, const int*, hipStream_t);
-template hipError_t launch_trsm_rlt<float>(int, int, const float*, i64, float*, i64, const int*, hipStream_t);
+template hipError_t launch_trsm_rlt<double>(int, int, const double*, i64, double*, i64, const int*, int, hipStream_t);
+template hipError_t launch_trsm_rlt<float>(int, int, const float*, i64, float*, i64, const int*, int, hipStream_t);
 
 }  // namespace elemamd

@@ -317,3 +317,16 @@ def test_device_normal_generator_matches_host():
     a = inputs.normal_device(301, 7, seed=42, offset=3)
     b = inputs.normal_host(301, 7, seed=42, offset=3)
     assert np.max(np.abs(host(a) - b)) <= 8 * EPS * np.abs(b).max()
+
+
+def test_cpp_dropin_api():
+    """The C++ drop-in header (elem::Matrix, elem::HPDSolve, elem::Cholesky on a
+    view, the non-HPD logic_error, elem::Trsm) driven by a compiled program."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "elem_api_test")
+    assert os.path.exists(exe), "build() compiles tests/cpp/elem_api_test"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL OK" in r.stdout, r.stdout + r.stderr
