@@ -1,0 +1,88 @@
+// Device code of the panel triangular solve B := B inv(L)^T, shared by the
+// single-matrix and batched launches.
+#pragma once
+
+#include "common.hpp"
+
+namespace elemamd {
+
+constexpr int kTrsmMaxN = 128;
+constexpr int kTrsmRowsPerBlock = 64;
+
+// One column k = 4*K4 + R of the right-looking sweep.  All register indices
+// are compile-time constants (the sweep is instantiated per column), so the
+// row slice stays in VGPRs.
+template <int K4, int R, bool FULL>
+__device__ __forceinline__ void rlt_column(double (&b)[32], int q, int n, const double* sL, const double* sRinv) {
+#pragma clang fp contract(off)
+    constexpr int k = 4 * K4 + R;
+    // A uniform branch per column (also when n == 128 is known to the caller)
+    // ends the basic block, so each column is scheduled on its own and the
+    // sweep's LDS reads are not all hoisted into registers.
+    if (k >= n) return;
+    const double xv = sRinv[k] * b[K4];
+    const double x = quad_bcast<R>(xv);
+    if (q == R) b[K4] = x;
+#pragma unroll
+    for (int s = K4; s < 32; ++s) {
+        const int j = 4 * s + q;
+        if (s > K4 || q > R) {
+            if (FULL || j < n) b[s] = b[s] - sL[j + k * kTrsmMaxN] * x;
+        }
+    }
+}
+
+template <int K4, bool FULL>
+struct RltSweep {
+    static __device__ __forceinline__ void run(double (&b)[32], int q, int n, const double* sL, const double* sRinv) {
+        rlt_column<K4, 0, FULL>(b, q, n, sL, sRinv);
+        rlt_column<K4, 1, FULL>(b, q, n, sL, sRinv);
+        rlt_column<K4, 2, FULL>(b, q, n, sL, sRinv);
+        rlt_column<K4, 3, FULL>(b, q, n, sL, sRinv);
+        RltSweep<K4 + 1, FULL>::run(b, q, n, sL, sRinv);
+    }
+};
+template <bool FULL>
+struct RltSweep<32, FULL> {
+    static __device__ __forceinline__ void run(double (&)[32], int, int, const double*, const double*) {}
+};
+
+// Rows [64*row_block, 64*row_block + 64) of the panel by the calling
+// 256-thread workgroup.
+template <typename T, bool FULL>
+__device__ __forceinline__ void trsm_rlt_rows(int m, int nb, const T* __restrict__ L, i64 ldl, T* __restrict__ B,
+                                              i64 ldb, const int* __restrict__ info, int limit, int row_block) {
+#pragma clang fp contract(off)
+    __shared__ double sL[kTrsmMaxN * kTrsmMaxN];
+    __shared__ double sRinv[kTrsmMaxN];
+    if (info != nullptr && *info != 0 && *info <= limit) return;
+    const int t = threadIdx.x;
+    const int n = nb;
+    for (int idx = t; idx < n * n; idx += 256) {
+        const int i = idx % n, j = idx / n;
+        sL[i + j * kTrsmMaxN] = (i >= j) ? (double)L[i + (i64)j * ldl] : 0.0;
+    }
+    for (int k = t; k < n; k += 256) sRinv[k] = 1.0 / (double)L[k + (i64)k * ldl];
+    __syncthreads();
+
+    const int q = t & 3;
+    const i64 row = (i64)row_block * kTrsmRowsPerBlock + (t >> 2);
+    const bool active = row < m;
+    double b[32];
+#pragma unroll
+    for (int s = 0; s < 32; ++s) {
+        const int j = 4 * s + q;
+        b[s] = (active && (FULL || j < n)) ? (double)B[row + (i64)j * ldb] : 0.0;
+    }
+    RltSweep<0, FULL>::run(b, q, n, sL, sRinv);
+    if (active) {
+#pragma unroll
+        for (int s = 0; s < 32; ++s) {
+            const int j = 4 * s + q;
+            if (FULL || j < n) B[row + (i64)j * ldb] = (T)b[s];
+        }
+    }
+}
+
+
+}  // namespace elemamd

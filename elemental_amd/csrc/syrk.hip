@@ -25,126 +25,35 @@
 //            hi*hi + hi*lo + lo*hi with v_mfma_f32_16x16x32_bf16, fp32 accumulate
 #include "common.hpp"
 #include "kernels.hpp"
+#include "syrk_dev.hpp"
 
 namespace elemamd {
 
-constexpr int kSyrkTile = 128;
-constexpr int kSyrkKC = 16;
-constexpr int kSyrkLdp = kSyrkTile + 16;  // 144 doubles: k-rows start 128 B apart in bank space
-
-// Tile coordinates of this workgroup; false if the tile lies above the diagonal.
-__device__ inline bool syrk_tile(int m, int n, int& ti, int& tj);
-
-__device__ inline void tri_tile(int t, int& ti, int& tj) {
-    // t = ti*(ti+1)/2 + tj, 0 <= tj <= ti
-    int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while ((i64)(r + 1) * (r + 2) / 2 <= t) ++r;
-    while ((i64)r * (r + 1) / 2 > t) --r;
-    ti = r;
-    tj = t - (int)((i64)r * (r + 1) / 2);
-}
-
-__device__ inline bool syrk_tile(int m, int n, int& ti, int& tj) {
-    if (m == n) {
-        tri_tile(blockIdx.x, ti, tj);
-        return true;
-    }
-    ti = blockIdx.x;
-    tj = blockIdx.y;
-    return ti >= tj;
-}
-
-// ---------------------------------------------------------------- FP64
 __global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
         double* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
     if (info != nullptr && *info != 0 && *info <= limit) return;
-    __shared__ double sI[2][kSyrkKC][kSyrkLdp];
-    __shared__ double sJ[2][kSyrkKC][kSyrkLdp];
     int ti, tj;
     if (!syrk_tile(m, n, ti, tj)) return;
-    const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wj = (w >> 1) * 64;  // MFMA M offset = C column offset
-    const int wi = (w & 1) * 64;   // MFMA N offset = C row offset
-    const bool skip = (ti == tj) && (wj > wi);
+    syrk_tile_f64(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
+}
 
-    // global -> register staging: thread owns tile row lr, k rows lk + 2q
-    const int lr = tid & 127;
-    const int lk = tid >> 7;
-    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
-    const double* pI = A + (i64)(i0 + lr);
-    const double* pJ = B + (i64)(j0 + lr);
-    double gI[8], gJ[8];
+__global__ __launch_bounds__(256, 2) void syrk_ln_f32_kernel(int m, int n, int k, float alpha,
+        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
+    if (info != nullptr && *info != 0 && *info <= limit) return;
+    int ti, tj;
+    if (!syrk_tile(m, n, ti, tj)) return;
+    syrk_tile_f32(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
+}
 
-    auto gload = [&](int kc0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int kk = kc0 + lk + 2 * q;
-            const bool kin = kk < k;
-            gI[q] = (rowI && kin) ? pI[(i64)kk * lda] : 0.0;
-            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * lda] : 0.0;
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            sI[buf][lk + 2 * q][lr] = gI[q];
-            sJ[buf][lk + 2 * q][lr] = gJ[q];
-        }
-    };
-
-    d4_t acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
-
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    const int nchunks = (int)ceil_div(k, kSyrkKC);
-    for (int c = 0; c < nchunks; ++c) {
-        const int buf = c & 1;
-        const bool more = (c + 1) < nchunks;
-        if (more) gload((c + 1) * kSyrkKC);
-        if (!skip) {
-#pragma unroll
-            for (int ks = 0; ks < kSyrkKC / 4; ++ks) {
-                const int kr = ks * 4 + (lane >> 4);
-                double fa[4], fb[4];
-#pragma unroll
-                for (int mb = 0; mb < 4; ++mb) fa[mb] = sJ[buf][kr][wj + 16 * mb + (lane & 15)];
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb) fb[nb] = sI[buf][kr][wi + 16 * nb + (lane & 15)];
-#pragma unroll
-                for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-                    for (int nb = 0; nb < 4; ++nb)
-                        acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mb], fb[nb], acc[mb][nb], 0, 0, 0);
-            }
-        }
-        if (more) sstore(buf ^ 1);
-        __syncthreads();
-    }
-    if (skip) return;
-
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const int i = i0 + wi + 16 * nb + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int j = j0 + wj + 16 * mb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
-                if (i < m && j < n && j <= i) {
-                    double* p = C + (i64)i + (i64)j * ldc;
-                    const double upd = alpha * acc[mb][nb][v];
-                    *p = (beta == 0.0) ? upd : beta * (*p) + upd;
-                }
-            }
-        }
-    }
+__global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3_kernel(int m, int n, int k, float alpha,
+        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
+    if (info != nullptr && *info != 0 && *info <= limit) return;
+    int ti, tj;
+    if (!syrk_tile(m, n, ti, tj)) return;
+    syrk_tile_bf16x3(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
 }
 
 static dim3 syrk_grid(int m, int n) {
@@ -162,201 +71,6 @@ hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A
 }
 
 
-// ---------------------------------------------------------------- FP32
-constexpr int kSyrkLdpF = kSyrkTile + 16;  // 144 floats: k-rows 16 banks apart
-
-__global__ __launch_bounds__(256, 2) void syrk_ln_f32_kernel(int m, int n, int k, float alpha,
-        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
-        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
-    if (info != nullptr && *info != 0 && *info <= limit) return;
-    __shared__ float sI[2][kSyrkKC][kSyrkLdpF];
-    __shared__ float sJ[2][kSyrkKC][kSyrkLdpF];
-    int ti, tj;
-    if (!syrk_tile(m, n, ti, tj)) return;
-    const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
-    const bool skip = (ti == tj) && (wj > wi);
-    const int lr = tid & 127, lk = tid >> 7;
-    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
-    const float* pI = A + (i64)(i0 + lr);
-    const float* pJ = B + (i64)(j0 + lr);
-    float gI[8], gJ[8];
-    auto gload = [&](int kc0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int kk = kc0 + lk + 2 * q;
-            const bool kin = kk < k;
-            gI[q] = (rowI && kin) ? pI[(i64)kk * lda] : 0.0f;
-            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * lda] : 0.0f;
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            sI[buf][lk + 2 * q][lr] = gI[q];
-            sJ[buf][lk + 2 * q][lr] = gJ[q];
-        }
-    };
-    f4_t acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    const int nchunks = (int)ceil_div(k, kSyrkKC);
-    for (int c = 0; c < nchunks; ++c) {
-        const int buf = c & 1;
-        const bool more = (c + 1) < nchunks;
-        if (more) gload((c + 1) * kSyrkKC);
-        if (!skip) {
-#pragma unroll
-            for (int ks = 0; ks < kSyrkKC / 4; ++ks) {
-                const int kr = ks * 4 + (lane >> 4);
-                float fa[4], fb[4];
-#pragma unroll
-                for (int mb = 0; mb < 4; ++mb) fa[mb] = sJ[buf][kr][wj + 16 * mb + (lane & 15)];
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb) fb[nb] = sI[buf][kr][wi + 16 * nb + (lane & 15)];
-#pragma unroll
-                for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-                    for (int nb = 0; nb < 4; ++nb)
-                        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[mb], fb[nb], acc[mb][nb], 0, 0, 0);
-            }
-        }
-        if (more) sstore(buf ^ 1);
-        __syncthreads();
-    }
-    if (skip) return;
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const int i = i0 + wi + 16 * nb + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int j = j0 + wj + 16 * mb + 4 * (lane >> 4) + v;
-                if (i < m && j < n && j <= i) {
-                    float* p = C + (i64)i + (i64)j * ldc;
-                    const float upd = alpha * acc[mb][nb][v];
-                    *p = (beta == 0.0f) ? upd : beta * (*p) + upd;
-                }
-            }
-        }
-}
-
-// ------------------------------------------------------- split-BF16x3
-// Each float x of the panel is split as hi = bf16(x), lo = bf16(x - hi);
-// products use hi*hi + hi*lo + lo*hi (the lo*lo term, ~2^-16 relative, is
-// dropped), each a v_mfma_f32_16x16x32_bf16 with fp32 accumulation.
-// LDS holds row-major [row][k] planes so one lane fetches its 8 consecutive
-// k values (16 bytes) per operand.
-constexpr int kBfKC = 32;
-constexpr int kBfLd = kBfKC + 8;  // 40 bf16 = 80 B per row: 8 lanes of 16 B hit disjoint banks
-
-typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
-
-__global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3_kernel(int m, int n, int k, float alpha,
-        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
-        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
-    if (info != nullptr && *info != 0 && *info <= limit) return;
-    __shared__ __attribute__((aligned(16))) unsigned short sIh[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sIl[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sJh[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sJl[kSyrkTile][kBfLd];
-    int ti, tj;
-    if (!syrk_tile(m, n, ti, tj)) return;
-    const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
-    const bool skip = (ti == tj) && (wj > wi);
-    const int lr = tid & 127, kh = tid >> 7;  // row, k half (16 values)
-    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
-    const float* pI = A + (i64)(i0 + lr);
-    const float* pJ = B + (i64)(j0 + lr);
-    float gI[16], gJ[16];
-    auto gload = [&](int kc0) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int kk = kc0 + kh * 16 + q;
-            const bool kin = kk < k;
-            gI[q] = (rowI && kin) ? pI[(i64)kk * lda] : 0.0f;
-            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * lda] : 0.0f;
-        }
-    };
-    auto sstore = [&]() {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const unsigned short hI = f32_to_bf16_rne(gI[q]);
-            const unsigned short hJ = f32_to_bf16_rne(gJ[q]);
-            sIh[lr][kh * 16 + q] = hI;
-            sIl[lr][kh * 16 + q] = f32_to_bf16_rne(gI[q] - bf16_to_f32(hI));
-            sJh[lr][kh * 16 + q] = hJ;
-            sJl[lr][kh * 16 + q] = f32_to_bf16_rne(gJ[q] - bf16_to_f32(hJ));
-        }
-    };
-    f4_t acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
-    const int nchunks = (int)ceil_div(k, kBfKC);
-    gload(0);
-    for (int c = 0; c < nchunks; ++c) {
-        __syncthreads();  // previous chunk's LDS reads are done
-        sstore();
-        __syncthreads();
-        if (c + 1 < nchunks) gload((c + 1) * kBfKC);
-        if (!skip) {
-            const int kofs = 8 * (lane >> 4);
-            s8_t ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) {
-                const int r = wj + 16 * mb + (lane & 15);
-                ah[mb] = *reinterpret_cast<const s8_t*>(&sJh[r][kofs]);
-                al[mb] = *reinterpret_cast<const s8_t*>(&sJl[r][kofs]);
-            }
-#pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                const int r = wi + 16 * nb + (lane & 15);
-                bh[nb] = *reinterpret_cast<const s8_t*>(&sIh[r][kofs]);
-                bl[nb] = *reinterpret_cast<const s8_t*>(&sIl[r][kofs]);
-            }
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-                for (int nb = 0; nb < 4; ++nb) {
-                    f4_t a = acc[mb][nb];
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, al[mb]),
-                                                                __builtin_bit_cast(bf8_t, bh[nb]), a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, ah[mb]),
-                                                                __builtin_bit_cast(bf8_t, bl[nb]), a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, ah[mb]),
-                                                                __builtin_bit_cast(bf8_t, bh[nb]), a, 0, 0, 0);
-                    acc[mb][nb] = a;
-                }
-        }
-    }
-    if (skip) return;
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-        for (int nb = 0; nb < 4; ++nb) {
-            const int i = i0 + wi + 16 * nb + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int j = j0 + wj + 16 * mb + 4 * (lane >> 4) + v;
-                if (i < m && j < n && j <= i) {
-                    float* p = C + (i64)i + (i64)j * ldc;
-                    const float upd = alpha * acc[mb][nb][v];
-                    *p = (beta == 0.0f) ? upd : beta * (*p) + upd;
-                }
-            }
-        }
-}
 
 hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, const float* A, i64 lda, const float* B,
                               i64 ldb, float beta, float* C, i64 ldc, const int* info, int limit, hipStream_t s) {
