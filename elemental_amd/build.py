@@ -27,6 +27,8 @@ SOURCES = [
     "solve.hip",
     "blas1.hip",
     "lapack.cpp",
+    "batch.hip",
+    "batch_plan.cpp",
     "runtime.cpp",
     "capi.cpp",
 ]

@@ -1,0 +1,423 @@
+// Batched HPD factor-and-solve for many independent matrices of uneven size
+// (no reference counterpart: Elemental factors one matrix per call).
+//
+// Each matrix is the same computation as the single-matrix path: LVar3 with
+// block size nb (the same potrf / panel-solve / trailing-update device code,
+// so a matrix gives the same result alone and inside a batch), then the
+// forward and backward solves.  Matrices are processed in order of decreasing
+// size; step j of the factorization of every matrix that has a step j runs in
+// one launch per kernel type, and a workgroup finds its (matrix, tile) from a
+// per-step prefix sum of work counts, so small and large matrices share
+// launches without idle workgroups.  The whole sequence is captured in a
+// hipGraph by the host driver and replayed as one submission.
+//
+// Layout: matrix i is column-major with ld = n_i at element offset a_off[i] of
+// one buffer; its right-hand sides (n_i x k_i, ld = n_i) at b_off[i].
+#include "batch.hpp"
+#include "common.hpp"
+#include "kernels.hpp"
+#include "panel_dev.hpp"
+#include "potrf_dev.hpp"
+#include "syrk_dev.hpp"
+
+namespace elemamd {
+
+// index p with prefix[p] <= w < prefix[p+1]
+__device__ inline int find_item(const int* prefix, int cnt, int w) {
+    int lo = 0, hi = cnt;  // prefix[0] = 0 <= w < prefix[cnt]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------- factorization
+template <typename T>
+__global__ __launch_bounds__(512) void batch_potrf_kernel(BatchArgs a, T* A, int step, int nb) {
+    const int p = blockIdx.x;
+    const int n = a.n[p];
+    const int c0 = step * nb;
+    if (c0 >= n) return;
+    if (a.skip != nullptr && a.skip[p]) return;
+    const int b = min(nb, n - c0);
+    potrf_unb_block<T>(b, A + a.a_off[p] + c0 + (i64)c0 * n, n, a.info + p, c0);
+}
+
+template <typename T, bool FULL>
+__global__ __launch_bounds__(256) void batch_trsm_kernel(BatchArgs a, T* A, int step, int nb, const int* prefix,
+                                                         int cnt) {
+    const int p = find_item(prefix, cnt, blockIdx.x);
+    const int rb = blockIdx.x - prefix[p];
+    const int n = a.n[p];
+    const int c0 = step * nb;
+    const int m = n - c0 - nb;
+    if (m <= 0) return;  // (prefix gives such matrices no workgroups)
+    if (a.skip != nullptr && a.skip[p]) return;
+    T* L = A + a.a_off[p] + c0 + (i64)c0 * n;
+    trsm_rlt_rows<T, FULL>(m, nb, L, n, L + nb, n, a.info + p, c0 + nb, rb);
+}
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(256, 2) void batch_syrk_kernel(BatchArgs a, T* A, int step, int nb, const int* prefix,
+                                                            int cnt) {
+    const int p = find_item(prefix, cnt, blockIdx.x);
+    const int t = blockIdx.x - prefix[p];
+    const int n = a.n[p];
+    const int c0 = step * nb;
+    const int m = n - c0 - nb;
+    if (m <= 0) return;
+    if (a.skip != nullptr && a.skip[p]) return;
+    const int* info = a.info + p;
+    if (*info != 0 && *info <= c0 + nb) return;
+    int ti, tj;
+    tri_tile(t, ti, tj);
+    const T* P = A + a.a_off[p] + (c0 + nb) + (i64)c0 * n;
+    T* C = A + a.a_off[p] + (c0 + nb) + (i64)(c0 + nb) * n;
+    if constexpr (MODE == 0)
+        syrk_tile_f64(m, m, nb, -1.0, (const double*)P, n, (const double*)P, n, 1.0, (double*)C, n, ti, tj);
+    else if constexpr (MODE == 1)
+        syrk_tile_f32(m, m, nb, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj);
+    else
+        syrk_tile_bf16x3(m, m, nb, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj);
+}
+
+// ------------------------------------------------------------------ solve
+// One workgroup per matrix: forward then backward solve in 64-row blocks,
+// 16 right-hand sides at a time.  Contributions of solved blocks are staged
+// through LDS and applied with fused multiply-adds; the 64x64 diagonal block
+// is solved by substitution (one wave per four right-hand sides, lanes =
+// rows), the backward sweep as the forward sweep on reversed indices.
+constexpr int kBS = 64;
+constexpr int kBC = 16;
+
+template <typename T>
+__device__ inline T bcast_lane(T v, int lane);
+template <>
+__device__ inline double bcast_lane<double>(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+template <>
+__device__ inline float bcast_lane<float>(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void batch_solve_kernel(BatchArgs a, const T* L_all, T* B_all) {
+    __shared__ T sL[kBS][kBS + 1];   // contraction tile / diagonal block
+    __shared__ T sX[kBS][kBC + 1];   // solved block / right-hand sides
+    __shared__ T sR[kBS][kBC + 1];   // block being solved
+    __shared__ T sRinv[kBS];
+    const int p = blockIdx.x;
+    if (a.info[p] != 0) return;
+    if (a.skip != nullptr && a.skip[p]) return;
+    const int n = a.n[p], k = a.k[p];
+    const T* L = L_all + a.a_off[p];
+    T* B = B_all + a.b_off[p];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int nT = (n + kBS - 1) / kBS;
+    for (int cs = 0; cs < k; cs += kBC) {
+        const int kc = min(kBC, k - cs);
+        for (int dir = 0; dir < 2; ++dir) {
+            const bool back = dir == 1;
+            for (int ii = 0; ii < nT; ++ii) {
+                const int I = back ? (nT - 1 - ii) : ii;
+                const int r0 = I * kBS;
+                const int bs = min(kBS, n - r0);
+                // thread (row pl = t % 64, column group cg = t / 64: columns cg, cg+4, cg+8, cg+12)
+                const int pl = t & 63, cg = t >> 6;
+                T acc[4] = {T(0), T(0), T(0), T(0)};
+                for (int jj = 0; jj < ii; ++jj) {
+                    const int J = back ? (nT - 1 - jj) : jj;
+                    const int q0 = J * kBS;
+                    const int qs = min(kBS, n - q0);
+                    __syncthreads();
+                    // tile M[q][p]: forward L(r0+p, q0+q); backward L(q0+q, r0+p)
+                    for (int idx = t; idx < kBS * kBS; idx += 256) {
+                        const int x = idx & 63, y = idx >> 6;
+                        if (!back) {  // lanes along p (rows of L)
+                            sL[y][x] = (x < bs && y < qs) ? L[(r0 + x) + (i64)(q0 + y) * n] : T(0);
+                        } else {  // lanes along q
+                            sL[x][y] = (x < qs && y < bs) ? L[(q0 + x) + (i64)(r0 + y) * n] : T(0);
+                        }
+                    }
+                    for (int idx = t; idx < kBS * kBC; idx += 256) {
+                        const int q = idx & 63, c = idx >> 6;
+                        sX[q][c] = (q < qs && c < kc) ? B[(q0 + q) + (i64)(cs + c) * n] : T(0);
+                    }
+                    __syncthreads();
+                    for (int q = 0; q < qs; ++q) {
+                        const T mv = sL[q][pl];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) acc[u] += mv * sX[q][cg + 4 * u];
+                    }
+                }
+                __syncthreads();
+                // R = B_I - acc
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int c = cg + 4 * u;
+                    sR[pl][c] = (pl < bs && c < kc) ? B[(r0 + pl) + (i64)(cs + c) * n] - acc[u] : T(0);
+                }
+                // diagonal block in lower-triangular form on (possibly reversed) indices
+                for (int idx = t; idx < kBS * kBS; idx += 256) {
+                    const int x = idx & 63, y = idx >> 6;  // M(x, y) into sL[y][x]
+                    const int gi = back ? (kBS - 1 - y) : x;
+                    const int gk = back ? (kBS - 1 - x) : y;
+                    T v = T(0);
+                    if (gi < bs && gk < bs) {
+                        if (x >= y) v = L[(r0 + gi) + (i64)(r0 + gk) * n];
+                    } else if (x == y) {
+                        v = T(1);
+                    }
+                    sL[y][x] = v;
+                }
+                __syncthreads();
+                if (t < kBS) sRinv[t] = T(1) / sL[t][t];
+                __syncthreads();
+                const int myrow = back ? (kBS - 1 - lane) : lane;
+                T v[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[u] = sR[myrow][4 * w + u];
+                for (int kk = 0; kk < kBS; ++kk) {
+                    const T rk = sRinv[kk];
+                    const T lik = sL[kk][lane];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const T y = bcast_lane<T>(v[u], kk) * rk;
+                        if (lane == kk) v[u] = y;
+                        else if (lane > kk) v[u] -= lik * y;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int c = 4 * w + u;
+                    if (myrow < bs && c < kc) B[(r0 + myrow) + (i64)(cs + c) * n] = v[u];
+                }
+                __threadfence_block();
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// -------------------------------------------------- refinement helpers
+// S_i := float(A_i) (lower triangle), one workgroup row per (matrix, column block)
+__global__ void batch_to_float_kernel(BatchArgs a, const double* A, float* S) {
+    const int p = blockIdx.y;
+    const int n = a.n[p];
+    const i64 total = (i64)n * n;
+    const double* src = A + a.a_off[p];
+    float* dst = S + a.a_off[p];
+    for (i64 e = (i64)blockIdx.x * 256 + threadIdx.x; e < total; e += (i64)gridDim.x * 256) {
+        const int i = (int)(e % n), j = (int)(e / n);
+        if (i >= j) dst[e] = (float)src[e];
+    }
+}
+
+// R_i := B_i - A_i X_i (A_i symmetric, lower triangle read), FP64; then
+// eta_i = max_c ||r_c||_inf / (||A_i||_inf ||x_c||_inf + ||b_c||_inf), and
+// rscale_i(c) = ||r_c||_inf.  One workgroup per matrix; thread = row.
+__global__ __launch_bounds__(256) void batch_residual_kernel(BatchArgs a, const double* A, const double* X,
+                                                             const double* Bin, double* R, double* eta,
+                                                             double* rscale, int kmax) {
+    __shared__ double red[3][256];
+    __shared__ double s_anorm;
+    const int p = blockIdx.x;
+    if (a.skip != nullptr && a.skip[p]) return;
+    const int n = a.n[p], k = a.k[p];
+    const double* Ap = A + a.a_off[p];
+    const double* Xp = X + a.b_off[p];
+    const double* Bp = Bin + a.b_off[p];
+    double* Rp = R + a.b_off[p];
+    const int t = threadIdx.x;
+    // ||A||_inf (row sums of |Asym|)
+    double amax = 0.0;
+    for (int i = t; i < n; i += 256) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += fabs(j <= i ? Ap[i + (i64)j * n] : Ap[j + (i64)i * n]);
+        amax = fmax(amax, s);
+    }
+    red[0][t] = amax;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[0][t] = fmax(red[0][t], red[0][t + h]);
+        __syncthreads();
+    }
+    if (t == 0) s_anorm = red[0][0];
+    __syncthreads();
+    const double anorm = s_anorm;
+    double eta_max = 0.0;
+    for (int c = 0; c < k; ++c) {
+        double rmax = 0.0, xmax = 0.0, bmax = 0.0;
+        for (int i = t; i < n; i += 256) {
+            double s = Bp[i + (i64)c * n];
+            for (int j = 0; j < n; ++j) {
+                const double aij = j <= i ? Ap[i + (i64)j * n] : Ap[j + (i64)i * n];
+                s -= aij * Xp[j + (i64)c * n];
+            }
+            Rp[i + (i64)c * n] = s;
+            rmax = fmax(rmax, fabs(s));
+            xmax = fmax(xmax, fabs(Xp[i + (i64)c * n]));
+            bmax = fmax(bmax, fabs(Bp[i + (i64)c * n]));
+        }
+        __syncthreads();
+        red[0][t] = rmax;
+        red[1][t] = xmax;
+        red[2][t] = bmax;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (t < h) {
+                red[0][t] = fmax(red[0][t], red[0][t + h]);
+                red[1][t] = fmax(red[1][t], red[1][t + h]);
+                red[2][t] = fmax(red[2][t], red[2][t + h]);
+            }
+            __syncthreads();
+        }
+        const double den = anorm * red[1][0] + red[2][0];
+        const double e = den > 0.0 ? red[0][0] / den : (red[0][0] > 0.0 ? INFINITY : 0.0);
+        eta_max = fmax(eta_max, e != e ? INFINITY : e);
+        if (t == 0 && rscale != nullptr) rscale[(i64)p * kmax + c] = red[0][0];
+        __syncthreads();
+    }
+    if (t == 0) eta[p] = eta_max;
+}
+
+// refinement bookkeeping: mark converged matrices, scale residuals to float
+__global__ void batch_ir_prepare_kernel(BatchArgs a, const double* eta, double tol, const double* R,
+                                        const double* rscale, float* Rf, int kmax, int* iters) {
+    const int p = blockIdx.x;
+    if (a.skip[p]) return;
+    if (eta[p] <= tol) {  // uniform across the workgroup
+        if (threadIdx.x == 0) a.skip[p] = 1;
+        return;
+    }
+    if (threadIdx.x == 0) iters[p] += 1;
+    const int n = a.n[p], k = a.k[p];
+    for (int c = 0; c < k; ++c) {
+        const double sc = rscale[(i64)p * kmax + c];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const i64 e = a.b_off[p] + i + (i64)c * n;
+            Rf[e] = (float)(sc > 0.0 ? R[e] / sc : 0.0);
+        }
+    }
+}
+
+__global__ void batch_ir_update_kernel(BatchArgs a, const float* D, const double* rscale, double* X, int kmax) {
+    const int p = blockIdx.x;
+    if (a.skip[p]) return;
+    const int n = a.n[p], k = a.k[p];
+    for (int c = 0; c < k; ++c) {
+        const double sc = rscale[(i64)p * kmax + c];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const i64 e = a.b_off[p] + i + (i64)c * n;
+            X[e] += sc * (double)D[e];
+        }
+    }
+}
+
+// dst_i := src_i (FP64), fdst_i := float(src_i), or dst_i := 0 when src is null
+__global__ void batch_copy_kernel(BatchArgs a, const double* src, double* dst, float* fdst) {
+    const int p = blockIdx.x;
+    const int n = a.n[p], k = a.k[p];
+    const i64 o = a.b_off[p];
+    for (i64 e = threadIdx.x; e < (i64)n * k; e += blockDim.x) {
+        if (src == nullptr) dst[o + e] = 0.0;
+        else if (fdst != nullptr) fdst[o + e] = (float)src[o + e];
+        else dst[o + e] = src[o + e];
+    }
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t batch_potrf(const BatchArgs& a, void* A, bool f32, int step, int nb, int cnt, hipStream_t s) {
+    if (cnt <= 0) return hipSuccess;
+    if (f32) hipLaunchKernelGGL(batch_potrf_kernel<float>, dim3(cnt), dim3(512), 0, s, a, (float*)A, step, nb);
+    else hipLaunchKernelGGL(batch_potrf_kernel<double>, dim3(cnt), dim3(512), 0, s, a, (double*)A, step, nb);
+    return hipGetLastError();
+}
+
+hipError_t batch_trsm(const BatchArgs& a, void* A, bool f32, int step, int nb, const int* prefix, int cnt, int wgs,
+                      hipStream_t s) {
+    if (wgs <= 0) return hipSuccess;
+    const bool full = nb == kTrsmMaxN;
+    if (f32) {
+        if (full)
+            hipLaunchKernelGGL((batch_trsm_kernel<float, true>), dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb,
+                               prefix, cnt);
+        else
+            hipLaunchKernelGGL((batch_trsm_kernel<float, false>), dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb,
+                               prefix, cnt);
+    } else {
+        if (full)
+            hipLaunchKernelGGL((batch_trsm_kernel<double, true>), dim3(wgs), dim3(256), 0, s, a, (double*)A, step,
+                               nb, prefix, cnt);
+        else
+            hipLaunchKernelGGL((batch_trsm_kernel<double, false>), dim3(wgs), dim3(256), 0, s, a, (double*)A, step,
+                               nb, prefix, cnt);
+    }
+    return hipGetLastError();
+}
+
+hipError_t batch_syrk(const BatchArgs& a, void* A, int mode, int step, int nb, const int* prefix, int cnt, int wgs,
+                      hipStream_t s) {
+    if (wgs <= 0) return hipSuccess;
+    if (mode == 0)
+        hipLaunchKernelGGL((batch_syrk_kernel<double, 0>), dim3(wgs), dim3(256), 0, s, a, (double*)A, step, nb, prefix,
+                           cnt);
+    else if (mode == 1)
+        hipLaunchKernelGGL((batch_syrk_kernel<float, 1>), dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb, prefix,
+                           cnt);
+    else
+        hipLaunchKernelGGL((batch_syrk_kernel<float, 2>), dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb, prefix,
+                           cnt);
+    return hipGetLastError();
+}
+
+hipError_t batch_solve(const BatchArgs& a, const void* L, void* B, bool f32, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    if (f32)
+        hipLaunchKernelGGL(batch_solve_kernel<float>, dim3(a.count), dim3(256), 0, s, a, (const float*)L, (float*)B);
+    else
+        hipLaunchKernelGGL(batch_solve_kernel<double>, dim3(a.count), dim3(256), 0, s, a, (const double*)L,
+                           (double*)B);
+    return hipGetLastError();
+}
+
+hipError_t batch_to_float(const BatchArgs& a, const double* A, float* S, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_to_float_kernel, dim3(16, a.count), dim3(256), 0, s, a, A, S);
+    return hipGetLastError();
+}
+
+hipError_t batch_residual(const BatchArgs& a, const double* A, const double* X, const double* B, double* R,
+                          double* eta, double* rscale, int kmax, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_residual_kernel, dim3(a.count), dim3(256), 0, s, a, A, X, B, R, eta, rscale, kmax);
+    return hipGetLastError();
+}
+
+hipError_t batch_ir_prepare(const BatchArgs& a, const double* eta, double tol, const double* R, const double* rscale,
+                            float* Rf, int kmax, int* iters, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_ir_prepare_kernel, dim3(a.count), dim3(256), 0, s, a, eta, tol, R, rscale, Rf, kmax,
+                       iters);
+    return hipGetLastError();
+}
+
+hipError_t batch_ir_update(const BatchArgs& a, const float* D, const double* rscale, double* X, int kmax,
+                           hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_ir_update_kernel, dim3(a.count), dim3(256), 0, s, a, D, rscale, X, kmax);
+    return hipGetLastError();
+}
+
+hipError_t batch_copy_rhs(const BatchArgs& a, const double* src, double* dst, float* fdst, hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_copy_kernel, dim3(a.count), dim3(256), 0, s, a, src, dst, fdst);
+    return hipGetLastError();
+}
+
+}  // namespace elemamd

@@ -3,11 +3,16 @@
 
 #include <algorithm>
 
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "lapack.hpp"
 #include "runtime.hpp"
 
 using namespace elemamd;
+
+struct elemamd_batch_plan {
+    BatchPlan* p;
+};
 
 namespace {
 
@@ -184,6 +189,70 @@ int elemamd_sym_residual_f64(int n, int k, const double* A, int64_t lda, const d
     return launch_symm_lower_residual(n, k, A, lda, X, ldx, B, ldb, R, ldr, S(stream)) == hipSuccess
                ? ELEMAMD_OK
                : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_batch_plan_create(int count, const int* n, const int* k, int precision, int nbuckets,
+                              const int* bucket_max_n, const int* bucket_nb, elemamd_batch_plan** plan) {
+    if (plan == nullptr) return ELEMAMD_ERR_ARG;
+    *plan = nullptr;
+    BatchPlan* P = nullptr;
+    const int rc = batch_plan_create(count, n, k, precision, nbuckets, bucket_max_n, bucket_nb, &P);
+    if (rc != kOk) return rc;
+    *plan = new elemamd_batch_plan{P};
+    return ELEMAMD_OK;
+}
+
+int elemamd_batch_plan_destroy(elemamd_batch_plan* plan) {
+    if (plan == nullptr) return ELEMAMD_ERR_ARG;
+    delete plan->p;
+    delete plan;
+    return ELEMAMD_OK;
+}
+
+int elemamd_batch_plan_layout(const elemamd_batch_plan* plan, int64_t* a_off, int64_t* b_off, int64_t* a_total,
+                              int64_t* b_total) {
+    if (plan == nullptr) return ELEMAMD_ERR_ARG;
+    const BatchPlan* P = plan->p;
+    for (int i = 0; i < P->count; ++i) {
+        if (a_off) a_off[i] = P->a_off[i];
+        if (b_off) b_off[i] = P->b_off[i];
+    }
+    if (a_total) *a_total = P->a_total;
+    if (b_total) *b_total = P->b_total;
+    return ELEMAMD_OK;
+}
+
+int elemamd_batch_set_refinement(elemamd_batch_plan* plan, int sweeps, double tol) {
+    if (plan == nullptr || sweeps < 0 || !(tol > 0.0)) return ELEMAMD_ERR_ARG;
+    BatchPlan* P = plan->p;
+    if (P->ir_sweeps != sweeps || P->tol != tol) {
+        P->ir_sweeps = sweeps;
+        P->tol = tol;
+        P->g_A = nullptr;  // the captured sequence no longer matches: capture again
+    }
+    return ELEMAMD_OK;
+}
+
+size_t elemamd_batch_workspace(const elemamd_batch_plan* plan) {
+    return plan == nullptr ? 0 : batch_workspace_bytes(plan->p);
+}
+
+int elemamd_batch_hpd_solve(elemamd_batch_plan* plan, double* A, double* B, void* work, size_t work_bytes,
+                            int use_graph, void* stream) {
+    if (plan == nullptr) return ELEMAMD_ERR_ARG;
+    return batch_hpd_solve(plan->p, A, B, work, work_bytes, use_graph != 0, S(stream));
+}
+
+int elemamd_batch_results(const elemamd_batch_plan* plan, int* info, double* backward_error, int* iterations,
+                          void* stream) {
+    if (plan == nullptr) return ELEMAMD_ERR_ARG;
+    return batch_results(plan->p, info, backward_error, iterations, S(stream));
+}
+
+int elemamd_batch_backward_error(const elemamd_batch_plan* plan, const double* A, const double* X, const double* B,
+                                 double* R, double* eta, void* stream) {
+    if (plan == nullptr || A == nullptr || X == nullptr || B == nullptr || R == nullptr) return ELEMAMD_ERR_ARG;
+    return batch_backward_error(plan->p, A, X, B, R, eta, S(stream));
 }
 
 }  // extern "C"

@@ -102,6 +102,12 @@ __global__ __launch_bounds__(256) void trtri_diag64_kernel(int n, const T* __res
 }
 
 // ------------------------------------------------------- persistent solve
+// Cross-workgroup data (the solved blocks and their flags) moves only through
+// device-coherent (agent-scope) loads and stores, which bypass the per-XCD
+// caches that are not coherent across XCDs.  No cache invalidation or
+// write-back is ever needed: the writer waits for its own coherent stores to
+// be acknowledged (vmcnt) before raising the flag, and the reader only issues
+// its coherent loads of a block after it has seen the block's flag.
 template <typename T, bool TRANS, bool USE_INV>
 __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const T* __restrict__ L, i64 ldl,
                                                            T* __restrict__ B, i64 ldb, const T* __restrict__ Dinv,
@@ -109,7 +115,7 @@ __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const 
     typedef typename Acc4<T>::type acc_t;
     __shared__ T sM[2][kSB][kSLd];  // sM[q][p]: contraction index q, output row p
     __shared__ T sY[2][kSB][kSC + 1];
-    __shared__ int s_ord;
+    __shared__ int s_ord, s_cnt;
     if (info != nullptr && *info != 0) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int nT = (int)ceil_div(n, kSB);
@@ -120,9 +126,50 @@ __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const 
     const int cy = ord % CY;
     const int I = TRANS ? (nT - 1 - ord / CY) : (ord / CY);
     const i64 r0 = (i64)I * kSB;
+    const int bs = (int)min<i64>(kSB, n - r0);
     const int c_base = cy * kSC;
     const int kcl = min(kSC, kc - c_base);
     int* flags = ctrl + 2;
+    // contraction blocks in the order they are solved
+    const int nJ = TRANS ? (nT - 1 - I) : I;
+    auto blk = [&](int pos) { return TRANS ? nT - 1 - pos : pos; };
+
+    // ---- operands that do not depend on other workgroups, fetched up front
+    // own right-hand sides in MFMA output order: row p = 16w + lane%16, column mfma_row(lane, v)
+    T gb[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = mfma_row<T>(lane, v);
+        const i64 row = r0 + 16 * w + (lane & 15);
+        gb[v] = (row < n && c < kcl) ? B[row + (i64)(c_base + c) * ldb] : T(0);
+    }
+    // diagonal operand, staged as sM-style [q][p] rows
+    T gd[16];
+    if constexpr (USE_INV) {
+        const T* Di = Dinv + (i64)I * kSB * kSB;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int qq = w + 4 * s;
+            gd[s] = !TRANS ? Di[lane + qq * kSB] : Di[qq + lane * kSB];  // M[q][p] = Dinv(p,q) / Dinv(q,p)
+        }
+    } else {
+        // lower-triangular system M(i, k) at [k][i], identity on padded rows.
+        // Forward: M = L_II.  Backward: L_II^T x = r is the same sweep on
+        // reversed indices, M(a, b) = L_II(63-b, 63-a), lane a holding row 63-a.
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int kk = w + 4 * s;
+            const int gi = TRANS ? (kSB - 1 - kk) : lane;
+            const int gk = TRANS ? (kSB - 1 - lane) : kk;
+            T v = T(0);
+            if (gi < bs && gk < bs) {
+                if (lane >= kk) v = L[r0 + gi + (r0 + gk) * ldl];
+            } else if (lane == kk) {
+                v = T(1);
+            }
+            gd[s] = v;
+        }
+    }
 
     // contraction tile of block J (rows q of the solved block, rows p of ours)
     T gm[16], gy[4];
@@ -144,14 +191,12 @@ __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const 
             }
         }
     };
-    auto gload_y = [&](int J) {
+    auto gload_y = [&](int J) {  // a solved block: device-coherent loads
         const i64 q0 = (i64)J * kSB;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const int c = w + 4 * s;
             const i64 q = q0 + lane;
-            // agent-coherent load: the block was written by another workgroup,
-            // possibly on another XCD (its L2 may hold older lines of it)
             gy[s] = (q < n && c < kcl)
                         ? __hip_atomic_load(&B[q + (i64)(c_base + c) * ldb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                         : T(0);
@@ -165,111 +210,92 @@ __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const 
 #pragma unroll
             for (int s = 0; s < 16; ++s) sM[buf][lane][w + 4 * s] = gm[s];
         }
-    };
-    auto sstore_y = [&](int buf) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) sY[buf][lane][w + 4 * s] = gy[s];
     };
-    auto wait_flag = [&](int J) {
-        if (t == 0) {
-            int* f = &flags[J * CY + cy];
+    // Length of the run of solved blocks starting at position `done` (>= 1);
+    // wave 0 polls up to 64 flags per round trip.  Bounded: past kSpinLimit
+    // polls the error is recorded in ctrl[1] and the remaining blocks are
+    // taken as they are.
+    auto ready_run = [&](int done) -> int {
+        if (w == 0) {
             long spins = 0;
-            // relaxed polling (no cache invalidation per poll), one acquire after
-            while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-                __builtin_amdgcn_s_sleep(1);
+            int cnt;
+            for (;;) {
+                const int pos = done + lane;
+                int f = 0;
+                if (pos < nJ) f = __hip_atomic_load(&flags[blk(pos) * CY + cy], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long m = __ballot(f != 0);
+                cnt = (~m == 0ull) ? 64 : (int)__builtin_ctzll(~m);
+                if (cnt > 0) break;
                 if (++spins > kSpinLimit) {
-                    atomicOr(&ctrl[1], 1);
+                    if (lane == 0) atomicOr(&ctrl[1], 1);
+                    cnt = nJ - done;
                     break;
                 }
+                __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (lane == 0) s_cnt = cnt;
         }
         __syncthreads();
+        return s_cnt;
     };
 
     // D[c][p]: MFMA rows = right-hand sides, columns = output rows; wave w owns rows 16w..16w+15
     acc_t acc = {};
-    const int nJ = TRANS ? (nT - 1 - I) : I;
-    if (nJ > 0) {
-        gload(TRANS ? nT - 1 : 0);
-        sstore(0);
-    }
-    for (int jj = 0; jj < nJ; ++jj) {
-        const int J = TRANS ? (nT - 1 - jj) : jj;
-        const int buf = jj & 1;
-        // next tile (the last iteration reloads its own tile into the spare
-        // buffer, which is never read again: the loop body stays branch-free)
-        const int Jn = ((jj + 1) < nJ) ? (TRANS ? J - 1 : J + 1) : J;
-        gload(Jn);
-        wait_flag(J);  // includes a barrier: sM[buf] is complete too
-        gload_y(J);
-        sstore_y(buf);
-        __syncthreads();
+    int buf = 0;
+    if (nJ > 0) gload(blk(0));
+    int done = 0;
+    while (done < nJ) {
+        const int cnt = ready_run(done);
+        gload_y(blk(done));
+        for (int b = 0; b < cnt; ++b) {
+            const int pos = done + b;
+            sstore(buf);
+            // next tiles (clamped reloads keep the body branch-free; a
+            // reloaded tile is overwritten before it is staged again)
+            gload(blk(min(pos + 1, nJ - 1)));
+            gload_y(blk((b + 1 < cnt) ? pos + 1 : pos));
+            __syncthreads();
 #pragma unroll
-        for (int ks = 0; ks < kSB / 4; ++ks) {
-            const int q = 4 * ks + (lane >> 4);
-            acc = mfma16(sY[buf][q][lane & 15], sM[buf][q][16 * w + (lane & 15)], acc);
+            for (int ks = 0; ks < kSB / 4; ++ks) {
+                const int q = 4 * ks + (lane >> 4);
+                acc = mfma16(sY[buf][q][lane & 15], sM[buf][q][16 * w + (lane & 15)], acc);
+            }
+            buf ^= 1;
         }
-        sstore(buf ^ 1);
+        done += cnt;
     }
     // A dependent zero-operand MFMA right before the accumulator is read: the
     // reads below then follow an MFMA in the same basic block, where the
     // compiler's wait-state padding for MFMA results is applied correctly
-    // (across the loop's back edge it was found to be short for FP64 MFMA).
+    // (across a loop back edge it was found to be short for FP64 MFMA).
     acc = mfma16(T(0), T(0), acc);
-    // R = B_I - acc into sY[0] as [p][c] -> contraction operand of the inverse
     __syncthreads();
+    // R = B_I - acc into sY[0] as [p][c]; diagonal operand into sM[0]
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int c = mfma_row<T>(lane, v);
-        const int p = 16 * w + (lane & 15);
-        const i64 row = r0 + p;
-        const T bv = (row < n && c < kcl) ? B[row + (i64)(c_base + c) * ldb] : T(0);
-        sY[0][p][c] = bv - acc[v];
-    }
+    for (int v = 0; v < 4; ++v) sY[0][16 * w + (lane & 15)][mfma_row<T>(lane, v)] = gb[v] - acc[v];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) sM[0][w + 4 * s][lane] = gd[s];
+    __syncthreads();
     if constexpr (USE_INV) {
         // forward Y(p) = sum_q Dinv(p,q) R(q); backward X(p) = sum_q Dinv(q,p) R(q)
-        const T* Di = Dinv + (i64)I * kSB * kSB;
-#pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int qq = w + 4 * s;
-            if (!TRANS) sM[1][qq][lane] = Di[lane + qq * kSB];  // M[q][p] = Dinv(p, q)
-            else sM[1][qq][lane] = Di[qq + lane * kSB];         // M[q][p] = Dinv(q, p)
-        }
-        __syncthreads();
         acc_t y = {};
 #pragma unroll
         for (int ks = 0; ks < kSB / 4; ++ks) {
             const int q = 4 * ks + (lane >> 4);
-            y = mfma16(sY[0][q][lane & 15], sM[1][q][16 * w + (lane & 15)], y);
+            y = mfma16(sY[0][q][lane & 15], sM[0][q][16 * w + (lane & 15)], y);
         }
+        y = mfma16(T(0), T(0), y);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = mfma_row<T>(lane, v);
             const i64 row = r0 + 16 * w + (lane & 15);
-            if (row < n && c < kcl) B[row + (i64)(c_base + c) * ldb] = y[v];
+            if (row < n && c < kcl)
+                __hip_atomic_store(&B[row + (i64)(c_base + c) * ldb], y[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     } else {
-        // Lower-triangular system of the block in sM[1][k][i] = M(i, k) with
-        // identity on padded rows.  Forward: M = L_II.  Backward: L_II^T x = r
-        // is solved as the same lower-triangular sweep on reversed indices,
-        // M(a, b) = L_II(63-b, 63-a), lane a holding row 63-a.
-        const int bs = (int)min<i64>(kSB, n - r0);
-#pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const int kk = w + 4 * s;
-            const int gi = TRANS ? (kSB - 1 - kk) : lane;  // row of L_II
-            const int gk = TRANS ? (kSB - 1 - lane) : kk;  // column of L_II
-            T v = T(0);
-            if (gi < bs && gk < bs) {
-                if (lane >= kk) v = L[r0 + gi + (r0 + gk) * ldl];
-            } else if (lane == kk) {
-                v = T(1);
-            }
-            sM[1][kk][lane] = v;
-        }
-        __syncthreads();
-        if (t < kSB) sY[1][t][0] = T(1) / sM[1][t][t];
+        if (t < kSB) sY[1][t][0] = T(1) / sM[0][t][t];
         __syncthreads();
         const int myrow = TRANS ? (kSB - 1 - lane) : lane;
         T v[4];
@@ -277,7 +303,7 @@ __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const 
         for (int u = 0; u < 4; ++u) v[u] = sY[0][myrow][4 * w + u];
         for (int kk = 0; kk < kSB; ++kk) {
             const T rk = sY[1][kk][0];
-            const T lik = sM[1][kk][lane];
+            const T lik = sM[0][kk][lane];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const T y = wave_bcast(v[u], kk) * rk;
@@ -289,14 +315,15 @@ __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int c = 4 * w + u;
-            if (row < n && c < kcl) B[row + (i64)(c_base + c) * ldb] = v[u];
+            if (row < n && c < kcl)
+                __hip_atomic_store(&B[row + (i64)(c_base + c) * ldb], v[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
-    // every thread's stores complete (workgroup-scope release), then one
-    // agent-scope release by the flag writer makes the block visible to all XCDs
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    // every wave waits for the acknowledgement of its coherent stores, then
+    // one thread raises the flag with a coherent store
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    if (t == 0) __hip_atomic_store(&flags[I * CY + cy], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == 0) __hip_atomic_store(&flags[I * CY + cy], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>

@@ -168,6 +168,53 @@ int elemamd_hpd_solve_mixed_f64(int n, int k, const double *A, int64_t lda, doub
                                 elemamd_ir_report *report, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Batched factor-and-solve of many independent matrices of uneven     */
+/* size (no reference counterpart: Elemental factors one matrix per    */
+/* call; each matrix goes through the same LVar3 + SolveAfter steps).  */
+
+#define ELEMAMD_BATCH_F64 0         /* plain FP64, A_i overwritten with L_i */
+#define ELEMAMD_BATCH_F32_IR 1      /* FP32 factor + FP64-residual refinement, A_i read only */
+#define ELEMAMD_BATCH_BF16X3_IR 2   /* as 1 with split-BF16x3 trailing updates */
+
+typedef struct elemamd_batch_plan elemamd_batch_plan;
+
+/* Plan for `count` matrices of sizes n[i] with k[i] right-hand sides.
+ * Matrix i is assigned to the first size bucket b with n[i] <= bucket_max_n[b]
+ * (increasing; the last bucket also takes every larger matrix) and is
+ * factored with that bucket's block size bucket_nb[b] (1..128).  Inputs are
+ * packed: A_i column-major with ld = n[i] at element offset a_off[i] of one
+ * buffer, B_i (n[i] x k[i], ld = n[i]) at b_off[i] (see _layout). */
+int elemamd_batch_plan_create(int count, const int *n, const int *k, int precision, int nbuckets,
+                              const int *bucket_max_n, const int *bucket_nb, elemamd_batch_plan **plan);
+int elemamd_batch_plan_destroy(elemamd_batch_plan *plan);
+/* Element offsets of every A_i / B_i and the total element counts. */
+int elemamd_batch_plan_layout(const elemamd_batch_plan *plan, int64_t *a_off, int64_t *b_off, int64_t *a_total,
+                              int64_t *b_total);
+/* Refinement of the low-precision modes: a fixed number of sweeps (so the
+ * launch sequence can be captured and replayed) and the per-matrix target;
+ * a matrix whose backward error meets `tol` skips the remaining sweeps.
+ * Defaults: 4 sweeps, 1e-14. */
+int elemamd_batch_set_refinement(elemamd_batch_plan *plan, int sweeps, double tol);
+size_t elemamd_batch_workspace(const elemamd_batch_plan *plan);
+/* Factor and solve every matrix; B_i is overwritten with X_i.  With
+ * use_graph = 1 the launch sequence is captured in a hipGraph on the first
+ * call (and again whenever A, B, work or stream change) and replayed as one
+ * submission.  Asynchronous on `stream`. */
+int elemamd_batch_hpd_solve(elemamd_batch_plan *plan, double *A, double *B, void *work, size_t work_bytes,
+                            int use_graph, void *stream);
+/* Synchronize and fetch per-matrix results in the caller's order: factor
+ * status (0, or the 1-based failing column), normwise backward error (the
+ * refinement modes; NaN for plain FP64), refinement sweeps used.  Any
+ * pointer may be NULL. */
+int elemamd_batch_results(const elemamd_batch_plan *plan, int *info, double *backward_error, int *iterations,
+                          void *stream);
+/* eta[i] = max_c ||b_c - A_i x_c||_inf / (||A_i||_inf ||x_c||_inf + ||b_c||_inf)
+ * for packed A (symmetric, lower triangle read), X and B; R receives the
+ * residuals (b_total elements).  Synchronous. */
+int elemamd_batch_backward_error(const elemamd_batch_plan *plan, const double *A, const double *X, const double *B,
+                                 double *R, double *eta, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Utilities.                                                          */
 
 /* Fill a rows x cols column-major block with N(0,1) numbers: entry t in

@@ -1,0 +1,164 @@
+"""Batched uneven-size factor-and-solve through the C ABI.
+
+Bars: every matrix in a batch gives bit-for-bit the result it gives in a
+batch of its own (the batch shares launches, never arithmetic); the plain
+FP64 batch agrees with the single-matrix path (factor bit-exact: the same
+potrf / panel / trailing tile code; solution within 1e-12 relative, the
+solves accumulate in a different order) and with the oracle's HPDSolve
+(1e-12 relative); every matrix of the refinement modes reaches FP64 normwise
+backward error <= 1e-14; hipGraph replays equal the eager launch sequence;
+a non-HPD matrix reports its failing column and leaves the others intact.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+if not torch.cuda.is_available():  # pragma: no cover - CPU container
+    pytest.skip("no GPU", allow_module_level=True)
+
+from elemental_amd import batch as EB  # noqa: E402
+from elemental_amd import lib  # noqa: E402
+
+SEED = 7100
+
+
+def S():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def make(n, k, precision="f64", buckets=None, seed=SEED):
+    plan = EB.BatchPlan(n, k, precision, buckets)
+    A, B = plan.empty_a(), plan.empty_b()
+    EB.fill_gram(plan, A, B, seed)
+    torch.cuda.synchronize()
+    return plan, A, B
+
+
+def rel(a, b):
+    d = (a - b).abs().max().item()
+    s = b.abs().max().item()
+    return d / s if s > 0 else d
+
+
+SIZES = dict(count=48, lo=16, hi=300, kmax=8, seed=11)
+
+
+def sizes(**kw):
+    a = dict(SIZES)
+    a.update(kw)
+    return EB.uneven_sizes(a["count"], a["lo"], a["hi"], a["kmax"], a["seed"])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16x3"])
+def test_alone_equals_in_batch(precision):
+    n, k = sizes()
+    plan, A, B = make(n, k, precision)
+    A0 = A.clone()
+    plan.set_refinement(6)
+    plan.solve(A, B, use_graph=False)
+    info, eta, _ = plan.results()
+    assert (info == 0).all()
+    for i in [0, 5, 17, len(n) - 1]:
+        one = EB.BatchPlan(n[i:i + 1], k[i:i + 1], precision, plan.buckets)
+        one.set_refinement(6)
+        A1, B1 = one.empty_a(), one.empty_b()
+        EB.fill_gram(one, A1, B1, SEED + i)  # matrix i of the batch used stream SEED + i
+        assert torch.equal(one.a_view(A1, 0), plan.a_view(A0, i))
+        one.solve(A1, B1, use_graph=False)
+        torch.cuda.synchronize()
+        assert torch.equal(one.b_view(B1, 0), plan.b_view(B, i)), f"matrix {i} (n={n[i]}) differs alone"
+        assert torch.equal(one.a_view(A1, 0), plan.a_view(A, i))
+
+
+def test_f64_batch_matches_single_path_and_oracle(oracle):
+    n, k = sizes()
+    nb = 64
+    plan, A, B = make(n, k, "f64", buckets=[(1 << 30, nb)])
+    A0, B0 = A.clone(), B.clone()
+    plan.solve(A, B, use_graph=False)
+    info, _, _ = plan.results()
+    assert (info == 0).all()
+    eta = plan.backward_error(A0, B, B0)
+    assert eta.max() <= 1e-14, eta.max()
+    for i in range(0, len(n), 7):
+        ni, ki = int(n[i]), int(k[i])
+        Ai = plan.a_view(A0, i).t().contiguous().t()  # column-major copy, ld = n
+        Bi = plan.b_view(B0, i).t().contiguous().t()
+        d = torch.zeros(1, dtype=torch.int32, device="cuda")
+        rc = lib().elemamd_hpd_solve_f64(b"L", b"N", ni, ki, ctypes.c_void_p(Ai.data_ptr()), ni,
+                                         ctypes.c_void_p(Bi.data_ptr()), ni, nb, ctypes.c_void_p(d.data_ptr()), S())
+        assert rc == 0 and lib().elemamd_info_read(ctypes.c_void_p(d.data_ptr()), S()) == 0
+        Lb = torch.tril(plan.a_view(A, i))
+        assert torch.equal(torch.tril(Ai), Lb), f"factor of matrix {i} differs from the single path"
+        assert rel(plan.b_view(B, i), Bi) <= 1e-12
+        # oracle: the reference's HPDSolve on the host
+        Ah = np.asfortranarray(plan.a_view(A0, i).cpu().numpy())
+        Bh = np.asfortranarray(plan.b_view(B0, i).cpu().numpy())
+        assert oracle.hpd_solve(Ah, Bh, nb=nb) == 0
+        assert np.abs(plan.b_view(B, i).cpu().numpy() - Bh).max() <= 1e-12 * np.abs(Bh).max()
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16x3"])
+def test_refinement_reaches_fp64_backward_error(precision):
+    n, k = sizes(count=96, hi=1024)
+    plan, A, B = make(n, k, precision)
+    A0, B0 = A.clone(), B.clone()
+    plan.set_refinement(6, 1e-14)
+    plan.solve(A, B, use_graph=True)
+    info, eta, it = plan.results()
+    assert (info == 0).all()
+    assert eta.max() <= 1e-14, (eta.max(), int(np.argmax(eta)))
+    assert torch.equal(A, A0)  # the refinement modes read A only
+    eta2 = plan.backward_error(A0, B, B0)
+    assert eta2.max() <= 1e-14
+    assert it.min() >= 1
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_graph_replays_equal_eager(precision):
+    n, k = sizes(count=32)
+    plan, A, B = make(n, k, precision)
+    plan.set_refinement(5)
+    A0, B0 = A.clone(), B.clone()
+    plan.solve(A, B, use_graph=False)
+    torch.cuda.synchronize()
+    A_e, X_e = A.clone(), B.clone()
+    for rep in range(3):
+        A.copy_(A0)
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        plan.solve(A, B, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(B, X_e), f"replay {rep}"
+        assert torch.equal(A, A_e), f"replay {rep}"
+
+
+def test_non_hpd_member_reports_its_column():
+    n = np.array([40, 200, 130, 17], dtype=np.int32)
+    k = np.array([2, 3, 1, 4], dtype=np.int32)
+    plan, A, B = make(n, k, "f64", buckets=[(1 << 30, 32)])
+    bad, col = 1, 77
+    plan.a_view(A, bad)[col, col] = -5.0
+    ok_plan, A2, B2 = make(n, k, "f64", buckets=[(1 << 30, 32)])
+    plan.solve(A, B, use_graph=False)
+    ok_plan.solve(A2, B2, use_graph=False)
+    info, _, _ = plan.results()
+    assert info[bad] == col + 1
+    for i in range(len(n)):
+        if i != bad:
+            assert info[i] == 0
+            assert torch.equal(plan.b_view(B, i), ok_plan.b_view(B2, i))
+
+
+def test_plan_rejects_bad_arguments():
+    with pytest.raises(ValueError):
+        EB.BatchPlan([4, 0], [1, 1])
+    with pytest.raises(ValueError):
+        EB.BatchPlan([4], [1], buckets=[(64, 0)])
+    with pytest.raises(ValueError):
+        EB.BatchPlan([4], [1], buckets=[(64, 32), (32, 32)])
