@@ -11,7 +11,15 @@ The headline value is mixed+IR solves/s; plain FP64 is reported beside it,
 with p50/p99 latency, backward/forward errors, the dominant kernel's roofline
 and the oracle's CPU baseline.
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
+Other workloads (each prints its own single JSON line):
+  --mode batch : 8192 matrices, n log-uniform in [16, 1024], A_i = G G^T/n + 0.1 I,
+                 1-8 right-hand sides each, one captured batch (plain FP64 and FP32+IR)
+  --mode sweep : n = 32768, kappa in {1e2, 1e4, 1e6, 1e8}, B 32768 x 8: plain FP64 vs
+                 FP32 vs split-BF16x3 trailing update with FP64 refinement
+  --mode serve : Poisson arrivals of factor+solve requests, n log-uniform in [64, 8192],
+                 kappa = 1e4, 30% reuse of a resident factor, at 3 arrival rates
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--mode single|batch|sweep|serve]
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -44,6 +52,9 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline")
     p.add_argument("--cpu-runs", type=int, default=3)
+    p.add_argument("--mode", default="single", choices=["single", "batch", "sweep", "serve"])
+    p.add_argument("--count", type=int, default=8192, help="batch: number of matrices")
+    p.add_argument("--requests", type=int, default=160, help="serve: requests per arrival rate")
     return p.parse_args()
 
 
@@ -149,12 +160,238 @@ def cpu_baseline(A_host, B_host, runs):
                       f"{med:.3f} s; oracle LVar3(nb=128)+SolveAfter over {blas}"}
 
 
+# ------------------------------------------------------------ other modes
+def _common(metric, unit, value, args, ms, dtype, config, higher=True):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def bench_batch(args):
+    """Uneven batch: every matrix factored and solved in one captured launch
+    sequence, plain FP64 and FP32 + FP64 refinement."""
+    import torch
+
+    from elemental_amd import batch as EB
+
+    n, k = EB.uneven_sizes(args.count, 16, 1024, 8, args.seed)
+    flops = float(np.sum(n.astype(np.float64) ** 3 / 3 + 2.0 * n.astype(np.float64) ** 2 * k))
+    res = {}
+    for prec in ("f64", "f32"):
+        plan = EB.BatchPlan(n, k, prec)
+        plan.set_refinement(4)
+        A0, B0 = plan.empty_a(), plan.empty_b()
+        t0 = time.time()
+        EB.fill_gram(plan, A0, B0, args.seed)
+        torch.cuda.synchronize()
+        gen = time.time() - t0
+        A, B = A0.clone(), B0.clone()
+        times = []
+        for it in range(args.warmup + args.steps):
+            A.copy_(A0)
+            B.copy_(B0)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            plan.solve(A, B, use_graph=True)
+            e1.record()
+            torch.cuda.synchronize()
+            if it >= args.warmup:
+                times.append(e0.elapsed_time(e1))
+        info, eta_r, sweeps = plan.results()
+        eta = plan.backward_error(A0, B, B0)
+        ms = float(np.mean(times))
+        res[prec] = {"solves_per_s": round(len(n) / (ms * 1e-3), 1), "ms": round(ms, 3),
+                     "p50_ms": round(pct(times, 50), 3), "p99_ms": round(pct(times, 99), 3),
+                     "gflops": round(flops / (ms * 1e-3) / 1e9, 1), "max_backward_error": float(eta.max()),
+                     "all_le_1e-14": bool((eta <= 1e-14).all()), "failed": int((info != 0).sum()),
+                     "buckets": plan.buckets, "input_generation_s": round(gen, 2)}
+        if prec != "f64":
+            res[prec]["max_sweeps"] = int(sweeps.max())
+            res[prec]["mean_sweeps"] = round(float(sweeps.mean()), 2)
+        plan.close()
+        del A, B, A0, B0
+        torch.cuda.empty_cache()
+    best = max(res, key=lambda p: res[p]["solves_per_s"] if res[p]["all_le_1e-14"] else -1)
+    out = _common("batched HPD solves/s at per-matrix FP64 backward err <=1e-14", "solves/s",
+                  res[best]["solves_per_s"], args, res[best]["ms"], "f64",
+                  {"workload": f"{len(n)} matrices n~logU[16,1024] A=GG^T/n+0.1I k~U[1,8]",
+                   "count": len(n), "sum_n": int(n.sum()), "best": best, "parallelism": "single GPU"})
+    out.update(res)
+    return out
+
+
+def bench_sweep(args):
+    """Precision sweep at n = 32768: one orthogonal factor Q, A = Q diag(s) Q^T
+    for each kappa; plain FP64 vs FP32 vs split-BF16x3 trailing updates."""
+    import torch
+
+    import elemental_amd as E
+    from elemental_amd import inputs
+    from elemental_amd.mixed import workspace_bytes
+
+    n, k = (args.n if args.n != 16384 else 32768), 8
+    G = inputs.normal_device(n, n, args.seed).tensor().clone()
+    Q, R = torch.linalg.qr(G)
+    Q.mul_(torch.sign(torch.diagonal(R)).unsqueeze(0))
+    del R, G
+    torch.cuda.empty_cache()
+    B = inputs.normal_device(n, k, args.seed + 1)
+    A = E.Matrix(n, n)
+    Aw = E.Matrix(n, n)
+    X = E.Matrix(n, k)
+    rows = []
+    for kappa in (1e2, 1e4, 1e6, 1e8):
+        s_ = torch.logspace(0.0, -float(np.log10(kappa)), n, dtype=torch.float64, device="cuda")
+        At = A.tensor()
+        At.copy_((Q * s_.unsqueeze(0)) @ Q.t())
+        At.copy_(0.5 * (At + At.t()))
+        anorm = sym_inf_norm(A)
+        # plain FP64
+        Aw.tensor().copy_(At)
+        X.tensor().copy_(B.tensor())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        try:
+            E.HPDSolve(E.LOWER, E.NORMAL, Aw, X)
+            torch.cuda.synchronize()
+            t64 = time.perf_counter() - t0
+            eta64 = eta_inf(A, X, B, anorm)
+        except E.LogicError:
+            t64, eta64 = float("nan"), float("nan")
+        Xref = X.tensor().clone()
+        row = {"kappa": kappa, "f64": {"ms": round(t64 * 1e3, 2), "backward_error": eta64,
+                                       "tflops": round(n ** 3 / 3 / t64 / 1e12, 2),
+                                       "frac_fp64_peak": round(n ** 3 / 3 / t64 / PEAK["f64"], 3)}}
+        for trailing in ("f32", "bf16x3"):
+            opts = E.IROptions(trailing=trailing)
+            ws = torch.empty(workspace_bytes(n, k, opts), dtype=torch.uint8, device="cuda")
+            X.tensor().copy_(B.tensor())
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rep = E.HPDSolveMixed(A, X, opts, workspace=ws)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            eta = eta_inf(A, X, B, anorm)
+            fwd = float(torch.linalg.norm(X.tensor() - Xref) / torch.linalg.norm(Xref))
+            peak = PEAK["f32"] if trailing == "f32" else PEAK["bf16"]
+            row[trailing] = {"ms": round(t * 1e3, 2), "backward_error": eta, "forward_error_vs_fp64": fwd,
+                             "ir_iterations": rep.iterations, "gmres_iterations": rep.gmres_iterations,
+                             "used_gmres": rep.used_gmres, "shifted": rep.shifted, "converged": rep.converged,
+                             "speedup_vs_fp64": round(t64 / t, 3) if t64 == t64 else None,
+                             "factor_frac_of_peak": round(n ** 3 / 3 / t / peak, 4)}
+            del ws
+        rows.append(row)
+        print(json.dumps(row), file=sys.stderr, flush=True)
+    best = rows[2]["f32"]["ms"]
+    out = _common("precision sweep: mixed+IR vs plain FP64 at n=32768 (ms per solve, kappa=1e6 headline)", "ms",
+                  best, args, best, "f64", {"workload": f"HPD factor+solve n={n} k={k} kappa in 1e2..1e8",
+                                             "n": n, "k": k, "parallelism": "single GPU"}, higher=False)
+    out["sweep"] = rows
+    return out
+
+
+def bench_serve(args):
+    """Serving loop under Poisson arrivals at three rates, plain FP64 and
+    FP32 + FP64 refinement."""
+    import threading
+
+    import torch
+
+    from elemental_amd import inputs
+    from elemental_amd.serve import Server
+
+    rng = np.random.default_rng(args.seed)
+    P = 24
+    sizes = np.floor(np.exp(rng.uniform(np.log(64), np.log(8192), P))).astype(int)
+    pool = [inputs.spectrum_device(int(nn), 1e4, args.seed + i).tensor() for i, nn in enumerate(sizes)]
+    k = 4
+    torch.cuda.synchronize()
+
+    def schedule(count, rate):
+        arr = np.cumsum(rng.exponential(1.0 / rate, count))
+        reqs, seen = [], []
+        for j in range(count):
+            if seen and rng.random() < 0.3:
+                key, i = seen[int(rng.integers(len(seen)))]
+                reqs.append((arr[j], key, i, True))
+            else:
+                i = int(rng.integers(P))
+                key = 1000 * j + i
+                seen.append((key, i))
+                reqs.append((arr[j], key, i, False))
+        return reqs
+
+    def run(prec, reqs, slots=4):
+        Bs = [inputs.normal_device(pool[i].shape[0], k, 7 + j).tensor() for j, (_, _, i, _) in enumerate(reqs)]
+        Xs = [b.clone() for b in Bs]
+        torch.cuda.synchronize()
+        with Server(slots=slots, precision=prec) as S:
+            tickets = [None] * len(reqs)
+            t0 = time.perf_counter()
+
+            def submitter():
+                for j, (ta, key, i, reuse) in enumerate(reqs):
+                    d = ta - (time.perf_counter() - t0)
+                    if d > 0:
+                        time.sleep(d)
+                    tickets[j] = S.submit(key, pool[i], Xs[j], reuse=reuse)
+
+            th = threading.Thread(target=submitter)
+            th.start()
+            th.join()
+            res = [S.wait(t) for t in tickets]
+            st = S.stats()
+        lat = np.array([(r.t_done - r.t_submit) * 1e3 for r in res])
+        span = max(r.t_done for r in res) - min(r.t_submit for r in res)
+        ok = all(r.status == 0 for r in res)
+        # FP64 backward error of a sample of results
+        worst = 0.0
+        for j in range(0, len(reqs), max(1, len(reqs) // 12)):
+            A = pool[reqs[j][2]]
+            Rr = Bs[j] - A @ Xs[j]
+            an = A.abs().sum(dim=1).max()
+            e = (Rr.abs().amax(dim=0) / (an * Xs[j].abs().amax(dim=0) + Bs[j].abs().amax(dim=0))).max()
+            worst = max(worst, float(e))
+        return {"solves_per_s": round(len(reqs) / span, 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3), "all_ok": ok,
+                "max_backward_error_sampled": worst, "reuse_hits": int(st.reuse_hits),
+                "evictions": int(st.evictions), "peak_reserved_GB": round(st.peak_reserved / 2**30, 2)}
+
+    # capacity estimate: a closed loop (all requests queued at once) with the same mix
+    cal = schedule(args.requests, 1e9)
+    cap = {p: run(p, cal)["solves_per_s"] for p in ("f64", "f32")}
+    out_rates = []
+    for frac in (0.5, 0.8, 1.0):
+        rate = frac * max(cap.values())
+        reqs = schedule(args.requests, rate)
+        row = {"arrival_rate_per_s": round(rate, 2)}
+        for p in ("f64", "f32"):
+            row[p] = run(p, reqs)
+        out_rates.append(row)
+        print(json.dumps(row), file=sys.stderr, flush=True)
+    mid = out_rates[1]
+    best = "f32" if mid["f32"]["all_ok"] and mid["f32"]["solves_per_s"] >= mid["f64"]["solves_per_s"] else "f64"
+    out = _common("served HPD solves/s and p99 latency at FP64 backward err <=1e-14", "solves/s",
+                  mid[best]["solves_per_s"], args, mid[best]["p50_ms"], "f64",
+                  {"workload": f"Poisson arrivals, n~logU[64,8192] from a pool of {P} kappa=1e4 matrices, k={k}, "
+                               f"30% reuse of a resident factor, 4 slots", "requests_per_rate": args.requests,
+                   "headline_rate": mid["arrival_rate_per_s"], "best": best, "parallelism": "single GPU"})
+    out["closed_loop_capacity"] = cap
+    out["rates"] = out_rates
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; "
               "the multi-GPU layout is described in DESIGN.md (sharding section)", file=sys.stderr)
         sys.exit(2)
+    if args.mode != "single":
+        out = {"batch": bench_batch, "sweep": bench_sweep, "serve": bench_serve}[args.mode](args)
+        print(json.dumps(out))
+        return
     import torch
 
     import elemental_amd as E

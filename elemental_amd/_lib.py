@@ -54,6 +54,24 @@ class IRReportC(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):  # elemamd_server_config
+    _fields_ = [("slots", ctypes.c_int), ("precision", ctypes.c_int), ("nb", ctypes.c_int), ("tol", ctypes.c_double),
+                ("memory_budget", ctypes.c_size_t)]
+
+
+class ServeResult(ctypes.Structure):  # elemamd_serve_result
+    _fields_ = [("ticket", ctypes.c_int64), ("status", ctypes.c_int), ("info", ctypes.c_int),
+                ("factored", ctypes.c_int), ("iterations", ctypes.c_int), ("gmres_iterations", ctypes.c_int),
+                ("slot", ctypes.c_int), ("backward_error", ctypes.c_double), ("t_submit", ctypes.c_double),
+                ("t_start", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):  # elemamd_server_statistics
+    _fields_ = [("completed", ctypes.c_int64), ("reuse_hits", ctypes.c_int64), ("evictions", ctypes.c_int64),
+                ("peak_reserved", ctypes.c_size_t), ("reserved", ctypes.c_size_t), ("budget", ctypes.c_size_t),
+                ("resident_factors", ctypes.c_int)]
+
+
 # name -> (restype, argtypes); the symbol list the header declares
 _i, _d, _c, _vp, _i64, _u64, _sz = (ctypes.c_int, ctypes.c_double, ctypes.c_char, ctypes.c_void_p,
                                     ctypes.c_int64, ctypes.c_uint64, ctypes.c_size_t)
@@ -82,6 +100,12 @@ SIGNATURES = {
     "elemamd_batch_hpd_solve": (_i, [_vp, _vp, _vp, _vp, _sz, _i, _vp]),
     "elemamd_batch_results": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "elemamd_batch_backward_error": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "elemamd_server_create": (_i, [ctypes.POINTER(ServerConfig), _vp]),
+    "elemamd_server_submit": (_i, [_vp, _i, _i, _i, _i, _vp, _i64, _vp, _i64, _vp]),
+    "elemamd_server_wait": (_i, [_vp, _i64, ctypes.POINTER(ServeResult)]),
+    "elemamd_server_evict": (_i, [_vp, _i]),
+    "elemamd_server_stats": (_i, [_vp, ctypes.POINTER(ServerStats)]),
+    "elemamd_server_destroy": (_i, [_vp]),
 }
 
 

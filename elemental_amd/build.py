@@ -30,6 +30,7 @@ SOURCES = [
     "batch.hip",
     "batch_plan.cpp",
     "runtime.cpp",
+    "server.cpp",
     "capi.cpp",
 ]
 

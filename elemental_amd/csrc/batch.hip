@@ -13,6 +13,8 @@
 //
 // Layout: matrix i is column-major with ld = n_i at element offset a_off[i] of
 // one buffer; its right-hand sides (n_i x k_i, ld = n_i) at b_off[i].
+#include <algorithm>
+
 #include "batch.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
@@ -218,72 +220,126 @@ __global__ void batch_to_float_kernel(BatchArgs a, const double* A, float* S) {
     }
 }
 
-// R_i := B_i - A_i X_i (A_i symmetric, lower triangle read), FP64; then
-// eta_i = max_c ||r_c||_inf / (||A_i||_inf ||x_c||_inf + ||b_c||_inf), and
-// rscale_i(c) = ||r_c||_inf.  One workgroup per matrix; thread = row.
-__global__ __launch_bounds__(256) void batch_residual_kernel(BatchArgs a, const double* A, const double* X,
-                                                             const double* Bin, double* R, double* eta,
-                                                             double* rscale, int kmax) {
-    __shared__ double red[3][256];
-    __shared__ double s_anorm;
-    const int p = blockIdx.x;
+// R_i := B_i - A_i X_i (A_i symmetric, lower triangle read), FP64.  Work
+// items are 64-row blocks of every matrix (prefix sums over the bucket); a
+// workgroup streams the 64x64 tiles of its block row of the symmetric matrix
+// (tiles above the diagonal are read transposed from the stored lower
+// triangle) through LDS, with the right-hand sides in chunks of 8 columns.
+// Thread (row r, wave w) accumulates columns w and w + 4 of row r.  Per-matrix
+// maxima -- ||A||_inf (row sums of |A|), ||r_c||_inf, ||x_c||_inf,
+// ||b_c||_inf -- are reduced per wave and merged with integer atomic max on
+// the bit patterns of the non-negative doubles, which is order independent.
+// red[p] layout: [0] ||A||_inf, [1 + c] r, [1 + kmax + c] x, [1 + 2 kmax + c] b.
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double u = __shfl_xor(v, o);
+        v = (u > v || u != u) ? u : v;
+    }
+    return v;
+}
+
+__device__ inline void atomic_max_nonneg(double* p, double v) {
+    atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ __launch_bounds__(256) void batch_symv_residual_kernel(BatchArgs a, const double* A, const double* X,
+                                                                  const double* Bin, double* R, const int* prefix,
+                                                                  int cnt, double* red, int kmax) {
+    __shared__ double sM[64][65];  // sM[q][p] = Asym(r0 + p, q0 + q)
+    __shared__ double sX[64][9];
+    const int p = find_item(prefix, cnt, blockIdx.x);
+    const int I = blockIdx.x - prefix[p];
     if (a.skip != nullptr && a.skip[p]) return;
     const int n = a.n[p], k = a.k[p];
     const double* Ap = A + a.a_off[p];
     const double* Xp = X + a.b_off[p];
     const double* Bp = Bin + a.b_off[p];
     double* Rp = R + a.b_off[p];
-    const int t = threadIdx.x;
-    // ||A||_inf (row sums of |Asym|)
-    double amax = 0.0;
-    for (int i = t; i < n; i += 256) {
-        double s = 0.0;
-        for (int j = 0; j < n; ++j) s += fabs(j <= i ? Ap[i + (i64)j * n] : Ap[j + (i64)i * n]);
-        amax = fmax(amax, s);
-    }
-    red[0][t] = amax;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (t < h) red[0][t] = fmax(red[0][t], red[0][t + h]);
-        __syncthreads();
-    }
-    if (t == 0) s_anorm = red[0][0];
-    __syncthreads();
-    const double anorm = s_anorm;
-    double eta_max = 0.0;
-    for (int c = 0; c < k; ++c) {
-        double rmax = 0.0, xmax = 0.0, bmax = 0.0;
-        for (int i = t; i < n; i += 256) {
-            double s = Bp[i + (i64)c * n];
-            for (int j = 0; j < n; ++j) {
-                const double aij = j <= i ? Ap[i + (i64)j * n] : Ap[j + (i64)i * n];
-                s -= aij * Xp[j + (i64)c * n];
+    double* rd = red + (i64)p * (1 + 3 * kmax);
+    const int t = threadIdx.x, r = t & 63, w = t >> 6;
+    const int r0 = I * 64;
+    const int nT = (n + 63) / 64;
+    const int row = r0 + r;
+    const bool in = row < n;
+    for (int c0 = 0; c0 < k; c0 += 8) {
+        const int kc = min(8, k - c0);
+        double acc0 = 0.0, acc1 = 0.0, asum = 0.0;
+        for (int J = 0; J < nT; ++J) {
+            const int q0 = J * 64;
+            __syncthreads();
+            for (int idx = t; idx < 64 * 64; idx += 256) {
+                const int x = idx & 63, y = idx >> 6;
+                if (J < I) {  // stored tile A(r0 + x, q0 + y): lanes along rows
+                    sM[y][x] = (r0 + x < n && q0 + y < n) ? Ap[(r0 + x) + (i64)(q0 + y) * n] : 0.0;
+                } else if (J > I) {  // Asym(r0 + y, q0 + x) = A(q0 + x, r0 + y): lanes along rows of A
+                    sM[x][y] = (q0 + x < n && r0 + y < n) ? Ap[(q0 + x) + (i64)(r0 + y) * n] : 0.0;
+                } else if (x >= y) {  // diagonal tile, lower part mirrored
+                    const double v = (r0 + x < n) ? Ap[(r0 + x) + (i64)(r0 + y) * n] : 0.0;
+                    sM[y][x] = v;
+                    sM[x][y] = v;
+                }
             }
-            Rp[i + (i64)c * n] = s;
-            rmax = fmax(rmax, fabs(s));
-            xmax = fmax(xmax, fabs(Xp[i + (i64)c * n]));
-            bmax = fmax(bmax, fabs(Bp[i + (i64)c * n]));
-        }
-        __syncthreads();
-        red[0][t] = rmax;
-        red[1][t] = xmax;
-        red[2][t] = bmax;
-        __syncthreads();
-        for (int h = 128; h > 0; h >>= 1) {
-            if (t < h) {
-                red[0][t] = fmax(red[0][t], red[0][t + h]);
-                red[1][t] = fmax(red[1][t], red[1][t + h]);
-                red[2][t] = fmax(red[2][t], red[2][t + h]);
+            for (int idx = t; idx < 64 * 8; idx += 256) {
+                const int q = idx & 63, c = idx >> 6;
+                sX[q][c] = (q0 + q < n && c < kc) ? Xp[(q0 + q) + (i64)(c0 + c) * n] : 0.0;
             }
             __syncthreads();
+#pragma unroll 8
+            for (int q = 0; q < 64; ++q) {
+                const double m = sM[q][r];
+                acc0 += m * sX[q][w];
+                acc1 += m * sX[q][w + 4];
+                asum += fabs(m);
+            }
         }
-        const double den = anorm * red[1][0] + red[2][0];
-        const double e = den > 0.0 ? red[0][0] / den : (red[0][0] > 0.0 ? INFINITY : 0.0);
-        eta_max = fmax(eta_max, e != e ? INFINITY : e);
-        if (t == 0 && rscale != nullptr) rscale[(i64)p * kmax + c] = red[0][0];
-        __syncthreads();
+        if (c0 == 0 && w == 0) {
+            const double am = wave_max(in ? asum : 0.0);
+            if (r == 0) atomic_max_nonneg(&rd[0], am);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int c = w + 4 * u;
+            double rv = 0.0, xv = 0.0, bv = 0.0;
+            if (in && c < kc) {
+                const i64 e = row + (i64)(c0 + c) * n;
+                bv = Bp[e];
+                xv = Xp[e];
+                rv = bv - (u == 0 ? acc0 : acc1);
+                Rp[e] = rv;
+            }
+            const double rm = wave_max(fabs(rv)), xm = wave_max(fabs(xv)), bm = wave_max(fabs(bv));
+            if (r == 0 && c < kc) {
+                atomic_max_nonneg(&rd[1 + c0 + c], rm);
+                atomic_max_nonneg(&rd[1 + kmax + c0 + c], xm);
+                atomic_max_nonneg(&rd[1 + 2 * kmax + c0 + c], bm);
+            }
+        }
     }
-    if (t == 0) eta[p] = eta_max;
+}
+
+// eta_i = max_c ||r_c||_inf / (||A_i||_inf ||x_c||_inf + ||b_c||_inf) (a NaN
+// or an infinite ratio gives +inf), rscale_i(c) = ||r_c||_inf.
+__global__ void batch_eta_kernel(BatchArgs a, const double* red, double* eta, double* rscale, int kmax) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.count) return;
+    if (a.skip != nullptr && a.skip[p]) return;
+    const double* rd = red + (i64)p * (1 + 3 * kmax);
+    const double anorm = rd[0];
+    double e = 0.0;
+    for (int c = 0; c < a.k[p]; ++c) {
+        const double rm = rd[1 + c], xm = rd[1 + kmax + c], bm = rd[1 + 2 * kmax + c];
+        const double den = anorm * xm + bm;
+        double ec = den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
+        if (ec != ec) ec = INFINITY;
+        e = fmax(e, ec);
+        if (rscale != nullptr) rscale[(i64)p * kmax + c] = rm;
+    }
+    eta[p] = e;
+}
+
+__global__ void batch_fill_kernel(double* p, double v, i64 count) {
+    for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (i64)gridDim.x * blockDim.x) p[i] = v;
 }
 
 // refinement bookkeeping: mark converged matrices, scale residuals to float
@@ -393,9 +449,16 @@ hipError_t batch_to_float(const BatchArgs& a, const double* A, float* S, hipStre
 }
 
 hipError_t batch_residual(const BatchArgs& a, const double* A, const double* X, const double* B, double* R,
-                          double* eta, double* rscale, int kmax, hipStream_t s) {
-    if (a.count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(batch_residual_kernel, dim3(a.count), dim3(256), 0, s, a, A, X, B, R, eta, rscale, kmax);
+                          const int* prefix, int wgs, double* red, double* eta, double* rscale, int kmax,
+                          hipStream_t s) {
+    if (a.count <= 0 || wgs <= 0) return hipSuccess;
+    const i64 nred = (i64)a.count * (1 + 3 * kmax);
+    hipLaunchKernelGGL(batch_fill_kernel, dim3((unsigned)std::min<i64>(ceil_div(nred, 256), 1024)), dim3(256), 0, s,
+                       red, 0.0, nred);
+    hipLaunchKernelGGL(batch_symv_residual_kernel, dim3((unsigned)wgs), dim3(256), 0, s, a, A, X, B, R, prefix,
+                       a.count, red, kmax);
+    hipLaunchKernelGGL(batch_eta_kernel, dim3((unsigned)ceil_div(a.count, 256)), dim3(256), 0, s, a, red, eta, rscale,
+                       kmax);
     return hipGetLastError();
 }
 

@@ -28,8 +28,12 @@ hipError_t batch_syrk(const BatchArgs& a, void* A, int mode, int step, int nb, c
                       hipStream_t s);
 hipError_t batch_solve(const BatchArgs& a, const void* L, void* B, bool f32, hipStream_t s);
 hipError_t batch_to_float(const BatchArgs& a, const double* A, float* S, hipStream_t s);
+// R := B - A X per matrix; eta (and rscale = ||r_c||_inf when non-null) per
+// matrix; prefix/wgs: 64-row blocks of the matrices; red: count*(1+3*kmax)
+// doubles of reduction scratch.
 hipError_t batch_residual(const BatchArgs& a, const double* A, const double* X, const double* B, double* R,
-                          double* eta, double* rscale, int kmax, hipStream_t s);
+                          const int* prefix, int wgs, double* red, double* eta, double* rscale, int kmax,
+                          hipStream_t s);
 hipError_t batch_ir_prepare(const BatchArgs& a, const double* eta, double tol, const double* R, const double* rscale,
                             float* Rf, int kmax, int* iters, hipStream_t s);
 hipError_t batch_ir_update(const BatchArgs& a, const float* D, const double* rscale, double* X, int kmax,
@@ -41,6 +45,7 @@ struct BatchBucket {
     int nb = 128;
     int first = 0, count = 0;  // processing positions [first, first + count)
     int steps = 0;
+    int res_prefix_off = 0, res_wgs = 0;  // 64-row blocks of the residual
     struct Step {
         int active, trsm_wgs, syrk_wgs, trsm_prefix_off, syrk_prefix_off;
     };
@@ -62,7 +67,7 @@ struct BatchPlan {
     int *d_n = nullptr, *d_k = nullptr, *d_prefix = nullptr;
     i64 *d_aoff = nullptr, *d_boff = nullptr;
     int *d_info = nullptr, *d_skip = nullptr, *d_iters = nullptr;
-    double *d_eta = nullptr, *d_rscale = nullptr;
+    double *d_eta = nullptr, *d_rscale = nullptr, *d_red = nullptr;
     std::vector<void*> dev_allocs;
     // captured graph and the arguments it was captured with
     hipGraphExec_t graph_exec = nullptr;

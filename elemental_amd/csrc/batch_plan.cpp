@@ -124,6 +124,11 @@ int batch_plan_create(int count, const int* n, const int* k, int precision, int 
             prefix_all.insert(prefix_all.end(), ps.begin(), ps.end());
             B.step.push_back(st);
         }
+        std::vector<int> pr(B.count + 1, 0);
+        for (int q = 0; q < B.count; ++q) pr[q + 1] = pr[q] + (ns[B.first + q] + 63) / 64;
+        B.res_wgs = pr[B.count];
+        B.res_prefix_off = (int)prefix_all.size();
+        prefix_all.insert(prefix_all.end(), pr.begin(), pr.end());
         P->buckets.push_back(B);
     }
     int *d_n = nullptr, *d_k = nullptr, *d_pref = nullptr, *d_ints = nullptr;
@@ -141,7 +146,8 @@ int batch_plan_create(int count, const int* n, const int* k, int precision, int 
     if (d_pref) P->dev_allocs.push_back(d_pref);
     ok = ok && hipMalloc(&d_ints, sizeof(int) * (size_t)std::max(3 * count, 1)) == hipSuccess;
     if (d_ints) P->dev_allocs.push_back(d_ints);
-    ok = ok && hipMalloc(&d_dbl, sizeof(double) * (size_t)std::max(count * (1 + P->kmax), 1)) == hipSuccess;
+    // eta, rscale (kmax per matrix), reduction scratch (1 + 3 kmax per matrix)
+    ok = ok && hipMalloc(&d_dbl, sizeof(double) * (size_t)std::max(count * (2 + 4 * P->kmax), 1)) == hipSuccess;
     if (d_dbl) P->dev_allocs.push_back(d_dbl);
     if (!ok) {
         delete P;
@@ -157,6 +163,7 @@ int batch_plan_create(int count, const int* n, const int* k, int precision, int 
     P->d_iters = d_ints + 2 * count;
     P->d_eta = d_dbl;
     P->d_rscale = d_dbl + count;
+    P->d_red = d_dbl + (size_t)count * (1 + P->kmax);
     *out = P;
     return kOk;
 }
@@ -210,15 +217,18 @@ static int enqueue(BatchPlan* P, double* A, double* B, void* work, hipStream_t s
         double* eta = P->d_eta + bk.first;
         double* rscale = P->d_rscale + (i64)bk.first * P->kmax;
         int* iters = P->d_iters + bk.first;
+        double* red = P->d_red + (i64)bk.first * (1 + 3 * P->kmax);
         ELEMAMD_HIP_TRY(batch_copy_rhs(a, nullptr, X, nullptr, s));  // X := 0
         for (int it = 0; it < P->ir_sweeps; ++it) {
-            ELEMAMD_HIP_TRY(batch_residual(ai, A, X, B, R, eta, rscale, P->kmax, s));
+            ELEMAMD_HIP_TRY(batch_residual(ai, A, X, B, R, P->d_prefix + bk.res_prefix_off, bk.res_wgs, red, eta,
+                                           rscale, P->kmax, s));
             ELEMAMD_HIP_TRY(batch_ir_prepare(ai, eta, P->tol, R, rscale, Rf, P->kmax, iters, s));
             ELEMAMD_HIP_TRY(batch_solve(ai, S, Rf, true, s));
             ELEMAMD_HIP_TRY(batch_ir_update(ai, Rf, rscale, X, P->kmax, s));
         }
         // backward error of every matrix of the bucket, then X -> B
-        ELEMAMD_HIP_TRY(batch_residual(a, A, X, B, R, eta, nullptr, P->kmax, s));
+        ELEMAMD_HIP_TRY(batch_residual(a, A, X, B, R, P->d_prefix + bk.res_prefix_off, bk.res_wgs, red, eta, nullptr,
+                                       P->kmax, s));
         ELEMAMD_HIP_TRY(batch_copy_rhs(a, X, B, nullptr, s));
     }
     return kOk;
@@ -281,7 +291,9 @@ int batch_backward_error(const BatchPlan* P, const double* A, const double* X, c
     if (P->count == 0) return kOk;
     for (const BatchBucket& bk : P->buckets) {
         const BatchArgs a = bucket_args(P, bk, false);
-        ELEMAMD_HIP_TRY(batch_residual(a, A, X, B, R, P->d_eta + bk.first, nullptr, P->kmax, s));
+        ELEMAMD_HIP_TRY(batch_residual(a, A, X, B, R, P->d_prefix + bk.res_prefix_off, bk.res_wgs,
+                                       P->d_red + (i64)bk.first * (1 + 3 * P->kmax), P->d_eta + bk.first, nullptr,
+                                       P->kmax, s));
     }
     std::vector<double> he(P->count);
     ELEMAMD_HIP_TRY(hipMemcpyAsync(he.data(), P->d_eta, sizeof(double) * he.size(), hipMemcpyDeviceToHost, s));

@@ -7,11 +7,15 @@
 #include "kernels.hpp"
 #include "lapack.hpp"
 #include "runtime.hpp"
+#include "server.hpp"
 
 using namespace elemamd;
 
 struct elemamd_batch_plan {
     BatchPlan* p;
+};
+struct elemamd_server {
+    Server* s;
 };
 
 namespace {
@@ -74,7 +78,7 @@ int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const do
     if (up(uplo) != 'L' || up(trans) != 'N') return ELEMAMD_ERR_ARG;
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
-    return launch_syrk_ln_f64(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr, kNoLimit, S(stream)) == hipSuccess
+    return launch_syrk_ln_f64(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr, 0, 1, S(stream)) == hipSuccess
                ? ELEMAMD_OK
                                                                                                   : ELEMAMD_ERR_HIP;
 }
@@ -85,7 +89,7 @@ int elemamd_herk_f32(int mode, int n, int k, float alpha, const float* A, int64_
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
     return launch_syrk_ln_f32(mode == ELEMAMD_TRAILING_BF16X3, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr,
-                              kNoLimit, S(stream)) == hipSuccess
+                              0, 1, S(stream)) == hipSuccess
                ? ELEMAMD_OK
                : ELEMAMD_ERR_HIP;
 }
@@ -253,6 +257,78 @@ int elemamd_batch_backward_error(const elemamd_batch_plan* plan, const double* A
                                  double* R, double* eta, void* stream) {
     if (plan == nullptr || A == nullptr || X == nullptr || B == nullptr || R == nullptr) return ELEMAMD_ERR_ARG;
     return batch_backward_error(plan->p, A, X, B, R, eta, S(stream));
+}
+
+int elemamd_server_create(const elemamd_server_config* cfg, elemamd_server** server) {
+    if (cfg == nullptr || server == nullptr) return ELEMAMD_ERR_ARG;
+    *server = nullptr;
+    ServerConfig c;
+    c.slots = cfg->slots;
+    c.precision = cfg->precision;
+    c.nb = cfg->nb > 0 ? cfg->nb : 128;
+    c.tol = cfg->tol > 0.0 ? cfg->tol : 1e-14;
+    c.memory_budget = cfg->memory_budget;
+    auto* S = new Server(c);
+    const int rc = S->start();
+    if (rc != kOk) {
+        delete S;
+        return rc;
+    }
+    *server = new elemamd_server{S};
+    return ELEMAMD_OK;
+}
+
+int elemamd_server_submit(elemamd_server* server, int key, int reuse, int n, int k, const double* A, int64_t lda,
+                          double* B, int64_t ldb, int64_t* ticket) {
+    if (server == nullptr) return ELEMAMD_ERR_ARG;
+    ServeRequest q{key, reuse, n, k, A, lda, B, ldb};
+    return server->s->submit(q, ticket);
+}
+
+int elemamd_server_wait(elemamd_server* server, int64_t ticket, elemamd_serve_result* result) {
+    if (server == nullptr) return ELEMAMD_ERR_ARG;
+    ServeResult r{};
+    const int rc = server->s->wait(ticket, &r);
+    if (rc != kOk) return rc;
+    if (result) {
+        result->ticket = r.ticket;
+        result->status = r.status;
+        result->info = r.info;
+        result->factored = r.factored;
+        result->iterations = r.iterations;
+        result->gmres_iterations = r.gmres_iterations;
+        result->slot = r.slot;
+        result->backward_error = r.backward_error;
+        result->t_submit = r.t_submit;
+        result->t_start = r.t_start;
+        result->t_done = r.t_done;
+    }
+    return ELEMAMD_OK;
+}
+
+int elemamd_server_evict(elemamd_server* server, int key) {
+    if (server == nullptr) return ELEMAMD_ERR_ARG;
+    return server->s->evict(key);
+}
+
+int elemamd_server_stats(elemamd_server* server, elemamd_server_statistics* stats) {
+    if (server == nullptr || stats == nullptr) return ELEMAMD_ERR_ARG;
+    const ServerStats t = server->s->stats();
+    stats->completed = t.completed;
+    stats->reuse_hits = t.reuse_hits;
+    stats->evictions = t.evictions;
+    stats->peak_reserved = t.peak_used;
+    stats->reserved = t.used;
+    stats->budget = t.budget;
+    stats->resident_factors = t.resident;
+    return ELEMAMD_OK;
+}
+
+int elemamd_server_destroy(elemamd_server* server) {
+    if (server == nullptr) return ELEMAMD_ERR_ARG;
+    delete server->s;
+    delete server;
+    return ELEMAMD_OK;
 }
 
 }  // extern "C"

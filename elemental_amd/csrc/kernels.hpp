@@ -22,11 +22,14 @@ hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, co
                            hipStream_t s);
 
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
+// With info != null the k columns are factor columns [kcol0, kcol0 + k) and a
+// recorded failure drops the panels of its nbp-block and later ones.
 hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
-                              double beta, double* C, i64 ldc, const int* info, int limit, hipStream_t s);
+                              double beta, double* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
 // float storage; bf16x3 = false: FP32 MFMA, true: split-BF16x3 MFMA with FP32 accumulation.
 hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, const float* A, i64 lda, const float* B,
-                              i64 ldb, float beta, float* C, i64 ldc, const int* info, int limit, hipStream_t s);
+                              i64 ldb, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
+                              hipStream_t s);
 
 // gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,

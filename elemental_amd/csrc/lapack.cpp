@@ -20,22 +20,38 @@ namespace elemamd {
 // solve the panel below it, and apply the symmetric rank-nb update to the
 // trailing matrix.
 //
-// Scheduled with a look-ahead of one block: after panel j is solved, the
-// caller's stream updates only the next block column with panel j and goes
-// straight on to that block's potrf and panel solve, while an auxiliary
-// stream applies panel j to the rest of the trailing matrix.  The next
-// block-column update waits for the previous rest-update (they write the same
-// elements); everything joins back onto the caller's stream at the end.
-// Every element still receives the updates of panels 0, 1, 2, ... in order.
+// Scheduling on the GPU (same operations, other timing):
+//  * Deferred trailing updates.  Block columns are taken in groups of G
+//    (G*nb = the update depth).  Inside a group each block column is first
+//    brought up to date with the group's earlier panels (a rank-(j*nb)
+//    update of that block column only), then factored and its panel solved.
+//    The trailing matrix right of the group receives the whole group's panels
+//    in one rank-(G*nb) pass, so the dominant kernel reads and writes the
+//    trailing matrix once per G panels instead of once per panel.
+//  * Look-ahead.  The trailing pass is split: the caller's stream updates the
+//    next group's block columns and goes straight on to factor them, while an
+//    auxiliary stream applies the group to the rest of the trailing matrix.
+//    The next group's update waits for the previous rest-update (they write
+//    the same elements); everything joins the caller's stream at the end.
+//  * Failure.  Every launch carries the device status word: a potrf or panel
+//    solve after a failed pivot does nothing, and a trailing pass applies
+//    only the panels of blocks before the failing one -- the reference's
+//    state when it throws, up to the rounding order of the updates.
 template <typename T>
-static hipError_t syrk_launch(int trailing, int m, int n, int k, const T* A, i64 lda, const T* B, T* C, i64 ldc,
-                              const int* info, int limit, hipStream_t s) {
+static hipError_t syrk_launch(int trailing, int m, int n, int k, const T* A, i64 lda, T* C, i64 ldc, const int* info,
+                              int kcol0, int nbp, hipStream_t s) {
     if constexpr (sizeof(T) == 8)
-        return launch_syrk_ln_f64(m, n, k, -1.0, (const double*)A, lda, (const double*)B, lda, 1.0, (double*)C, ldc,
-                                  info, limit, s);
+        return launch_syrk_ln_f64(m, n, k, -1.0, (const double*)A, lda, (const double*)A, lda, 1.0, (double*)C, ldc,
+                                  info, kcol0, nbp, s);
     else
-        return launch_syrk_ln_f32(trailing == kTrailingBf16x3, m, n, k, -1.0f, (const float*)A, lda, (const float*)B,
-                                  lda, 1.0f, (float*)C, ldc, info, limit, s);
+        return launch_syrk_ln_f32(trailing == kTrailingBf16x3, m, n, k, -1.0f, (const float*)A, lda, (const float*)A,
+                                  lda, 1.0f, (float*)C, ldc, info, kcol0, nbp, s);
+}
+
+// update depth (columns per trailing pass)
+template <typename T>
+static int update_depth(int trailing) {
+    return sizeof(T) == 8 ? kUpdateDepthF64 : (trailing == kTrailingBf16x3 ? kUpdateDepthBf16x3 : kUpdateDepthF32);
 }
 
 template <typename T>
@@ -45,38 +61,40 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     if (n == 0) return kOk;
     StreamPair* sp = stream_pair(s);
     if (sp == nullptr) return kErrHip;
-    // step 0 panel
-    {
-        const int b = std::min(nb, n);
-        ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, A, lda, dinfo, 0, s));
-        if (n > b) ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(n - b, b, A, lda, A + b, lda, dinfo, b, s));
-    }
+    const int G = std::max(1, update_depth<T>(trailing) / nb);
+    const i64 gw64 = (i64)G * nb;
+    const int gw = (int)std::min<i64>(gw64, n);
     bool rest_pending = false;
-    for (int k = 0; k < n; k += nb) {
-        const int b = std::min(nb, n - k);
-        const int m2 = n - k - b;
+    for (int c0 = 0; c0 < n; c0 += gw) {
+        const int c1 = (int)std::min<i64>(n, (i64)c0 + gw);
+        // the group's block columns, in order
+        for (int k = c0; k < c1; k += nb) {
+            const int b = std::min(nb, n - k);
+            T* Akk = A + k + (i64)k * lda;
+            if (k > c0)  // panels c0 .. k-1 of this group into block column k
+                ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, n - k, b, k - c0, A + k + (i64)c0 * lda, lda, Akk, lda, dinfo,
+                                               c0, nb, s));
+            ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
+            if (n - k - b > 0) ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
+        }
+        const int m2 = n - c1;
         if (m2 <= 0) break;
-        const int limit = k + b;  // this step's work is cancelled only by failures at columns <= k+b-1
-        const T* P = A + (k + b) + (i64)k * lda;  // panel j, m2 x b
-        T* C = A + (k + b) + (i64)(k + b) * lda;  // trailing matrix, m2 x m2
-        const int bn = std::min(nb, m2);          // next block column
-        // the rest-update of the previous step writes the next block column too
+        const int kg = c1 - c0;
+        const T* P = A + c1 + (i64)c0 * lda;  // the group's panels below it, m2 x kg
+        T* C = A + c1 + (i64)c1 * lda;        // trailing matrix, m2 x m2
+        const int bn = std::min(gw, m2);      // next group's block columns
         if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
-        // fork: the rest of the trailing matrix is updated with panel j on aux
         const bool rest = m2 > bn;
-        if (rest) {
+        if (rest) {  // fork: the rest of the trailing matrix on aux
             ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
             ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
-            ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2 - bn, m2 - bn, b, P + bn, lda, P + bn, C + bn + (i64)bn * lda,
-                                           lda, dinfo, limit, sp->aux));
+            ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2 - bn, m2 - bn, kg, P + bn, lda, C + bn + (i64)bn * lda, lda,
+                                           dinfo, c0, nb, sp->aux));
             ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_b, sp->aux));
         }
         rest_pending = rest;
-        // critical chain on the caller's stream: next block column, its potrf and panel
-        ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2, bn, b, P, lda, P, C, lda, dinfo, limit, s));
-        ELEMAMD_HIP_TRY(launch_potrf_unb<T>(bn, C, lda, dinfo, k + b, s));
-        if (m2 > bn)
-            ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(m2 - bn, bn, C, lda, C + bn, lda, dinfo, k + b + bn, s));
+        // critical chain on the caller's stream: the next group's block columns
+        ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2, bn, kg, P, lda, C, lda, dinfo, c0, nb, s));
     }
     if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
     return kOk;
@@ -157,7 +175,6 @@ struct Carve {
 };
 
 struct MixedWork {
-    float* S;      // n x n low-precision factor (ld n)
     double* X;     // n x k iterate
     double* R;     // n x k residual
     float* Rf;     // n x k float correction
@@ -166,17 +183,14 @@ struct MixedWork {
     double* V;     // (m+1) blocks of n x k: Krylov basis
     double* nrm;   // 8k + n + 8 doubles: norms, row sums, scratch
     double* h;     // (m+1) x k Gram-Schmidt coefficients
-    float* Dinv;   // inverses of the 64x64 diagonal blocks of S
     int* ctrl;     // persistent-solve tickets and flags
-    int* dinfo;
     size_t bytes;
 };
 
-MixedWork carve_mixed(void* work, size_t cap, int n, int k, int m) {
+MixedWork carve_refine(void* work, size_t cap, int n, int k, int m) {
     Carve c(work, cap);
     MixedWork w;
-    const size_t nn = (size_t)n * n, nk = (size_t)n * k;
-    w.S = c.take<float>(nn);
+    const size_t nk = (size_t)n * k;
     w.X = c.take<double>(nk);
     w.R = c.take<double>(nk);
     w.Rf = c.take<float>(nk);
@@ -185,9 +199,7 @@ MixedWork carve_mixed(void* work, size_t cap, int n, int k, int m) {
     w.V = c.take<double>(m > 0 ? nk * (size_t)(m + 1) : 0);
     w.nrm = c.take<double>((size_t)8 * k + n + 8);
     w.h = c.take<double>((size_t)(m + 2) * k);
-    w.Dinv = c.take<float>(solve_dinv_count(n));
     w.ctrl = c.take<int>(solve_ctrl_count(n, k));
-    w.dinfo = c.take<int>(4);
     w.bytes = c.off + 256;
     return w;
 }
@@ -199,6 +211,7 @@ struct Ctx {
     const double* Bin;
     i64 ldb;
     MixedWork w;
+    const MixedFactor* f;
     hipStream_t s;
     double anorm;
 };
@@ -208,7 +221,7 @@ struct Ctx {
 int precond_apply(Ctx& c, const double* y, double* z, const double* colscale) {
     const int n = c.n, k = c.k;
     ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, y, n, colscale, c.w.Rf, n, c.s));
-    int rc = solve_apply<float>(true, true, n, k, c.w.S, n, c.w.Rf, n, c.w.Dinv, c.w.ctrl, nullptr, c.s);
+    int rc = solve_apply<float>(true, true, n, k, c.f->S, n, c.w.Rf, n, c.f->Dinv, c.w.ctrl, nullptr, c.s);
     if (rc) return rc;
     ELEMAMD_HIP_TRY(launch_copy_f32_to_f64_scaled(n, k, c.w.Rf, n, colscale, z, n, c.s));
     return kOk;
@@ -332,65 +345,114 @@ int gmres_cycle(Ctx& c, int m, double inner_tol, int* iters_out) {
 
 }  // namespace
 
+MixedFactor carve_factor(void* mem, int n) {
+    Carve c(mem, 0);
+    MixedFactor f{};
+    f.S = c.take<float>((size_t)n * n);
+    f.Dinv = c.take<float>(solve_dinv_count(n));
+    f.dinfo = c.take<int>(4);
+    f.scratch = c.take<double>((size_t)n + 8);
+    return f;
+}
+
+size_t mixed_factor_bytes(int n) {
+    Carve c(nullptr, 0);
+    c.take<float>((size_t)n * n);
+    c.take<float>(solve_dinv_count(n));
+    c.take<int>(4);
+    c.take<double>((size_t)n + 8);
+    return c.off + 256;
+}
+
+size_t mixed_refine_bytes(int n, int k, const MixedOpts& o) { return carve_refine(nullptr, 0, n, k, o.gmres_restart).bytes; }
+
 size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o) {
-    const int m = o.gmres_restart;
-    return carve_mixed(nullptr, 0, n, k, m).bytes;
+    return mixed_factor_bytes(n) + mixed_refine_bytes(n, k, o);
+}
+
+// Low-precision factor of A: S = float(A) (lower), FP64 diagonal potrf,
+// float panels and trailing updates; diagonal-block inverses for the solves;
+// ||A||_inf.  On breakdown the factor of A + sigma*diag(A) is retried.
+int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFactor& f, hipStream_t s) {
+    if (n < 0 || lda < std::max(1, n) || o.nb < 1 || o.nb > 128) return kErrArg;
+    f.lowprec_info = 0;
+    f.shifted = 0;
+    f.anorm = 0.0;
+    if (n == 0) return kOk;
+    double* rowsum = f.scratch;
+    double* dtmp = f.scratch + n;
+    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, s));
+    ELEMAMD_HIP_TRY(launch_vec_max(n, rowsum, dtmp, s));
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(&f.anorm, dtmp, sizeof(double), hipMemcpyDeviceToHost, s));
+    double sigma = 0.0;
+    int info = 0;
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32(n, A, lda, f.S, n, s));
+        if (sigma > 0.0) ELEMAMD_HIP_TRY(launch_shift_diag(n, A, lda, f.S, n, sigma, s));
+        int rc = cholesky_lower<float>(n, f.S, n, o.nb, o.trailing, f.dinfo, s);
+        if (rc) return rc;
+        ELEMAMD_HIP_TRY(hipMemcpyAsync(&info, f.dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+        ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
+        f.lowprec_info = info;
+        if (info == 0) break;
+        if (!o.allow_shift) break;
+        sigma = (sigma == 0.0) ? 1e-5 : sigma * 10.0;
+        f.shifted = 1;
+    }
+    if (info != 0) return kErrNotConverged;  // broke down even after shifting
+    return solve_prepare<float>(n, f.S, n, f.Dinv, s);
 }
 
 int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o, void* work,
                     size_t work_bytes, MixedReport* rep, hipStream_t s) {
-    MixedReport r{};
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
     if (o.nb < 1 || o.nb > 128) return kErrArg;
+    if (n == 0 || k == 0) {
+        if (rep) {
+            *rep = MixedReport{};
+            rep->converged = 1;
+        }
+        return kOk;
+    }
+    const size_t fb = mixed_factor_bytes(n);
+    if (work == nullptr || work_bytes < mixed_workspace_bytes(n, k, o)) return kErrWorkspace;
+    MixedFactor f = carve_factor(work, n);
+    int rc = mixed_factor(n, A, lda, o, f, s);
+    if (rc == kErrNotConverged) {
+        if (rep) {
+            *rep = MixedReport{};
+            rep->lowprec_info = f.lowprec_info;
+            rep->shifted = f.shifted;
+            rep->anorm_inf = f.anorm;
+        }
+        return rc;
+    }
+    if (rc) return rc;
+    return mixed_refine(n, k, A, lda, B, ldb, o, f, (char*)work + fb, work_bytes - fb, rep, s);
+}
+
+// FP64-residual refinement (classic, then GMRES-IR on stagnation) of A X = B
+// with a low-precision factor; B is overwritten with X.  Synchronous.
+int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o,
+                 const MixedFactor& f, void* work, size_t work_bytes, MixedReport* rep, hipStream_t s) {
+    MixedReport r{};
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
+    r.lowprec_info = f.lowprec_info;
+    r.shifted = f.shifted;
+    r.anorm_inf = f.anorm;
     if (n == 0 || k == 0) {
         r.converged = 1;
         if (rep) *rep = r;
         return kOk;
     }
     const int m = o.gmres_restart;
-    MixedWork w = carve_mixed(work, work_bytes, n, k, m);
+    MixedWork w = carve_refine(work, work_bytes, n, k, m);
     if (work == nullptr || work_bytes < w.bytes) return kErrWorkspace;
-    Ctx c{n, k, A, lda, B, ldb, w, s, 0.0};
-
-    // ||A||_inf from the symmetric row sums, ||b||_inf per column
-    double* rowsum = w.nrm + 8 * k;
-    double* dtmp = w.nrm + 8 * k + n;
-    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, s));
-    ELEMAMD_HIP_TRY(launch_vec_max(n, rowsum, dtmp, s));
+    Ctx c{n, k, A, lda, B, ldb, w, &f, s, f.anorm};
     std::vector<double> bn, xn, rn;
-    ELEMAMD_HIP_TRY(hipMemcpyAsync(&c.anorm, dtmp, sizeof(double), hipMemcpyDeviceToHost, s));
     ELEMAMD_HIP_TRY(launch_col_absmax(n, k, B, ldb, w.nrm, s));
     bn.resize(k);
     ELEMAMD_HIP_TRY(hipMemcpyAsync(bn.data(), w.nrm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-
-    // Low-precision factor: S = float(A), FP64 diagonal potrf, float panels.
-    double sigma = 0.0;
-    int info = 0;
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32(n, A, lda, w.S, n, s));
-        if (sigma > 0.0) ELEMAMD_HIP_TRY(launch_shift_diag(n, A, lda, w.S, n, sigma, s));
-        int rc = cholesky_lower<float>(n, w.S, n, o.nb, o.trailing, w.dinfo, s);
-        if (rc) return rc;
-        ELEMAMD_HIP_TRY(hipMemcpyAsync(&info, w.dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
-        ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
-        r.lowprec_info = info;
-        if (info == 0) break;
-        if (!o.allow_shift) break;
-        sigma = (sigma == 0.0) ? 1e-5 : sigma * 10.0;
-        r.shifted = 1;
-    }
-    r.anorm_inf = c.anorm;
-    if (info == 0) {
-        int rc = solve_prepare<float>(n, w.S, n, w.Dinv, s);
-        if (rc) return rc;
-    }
-
-    if (info != 0) {
-        // The low-precision factor broke down even after shifting: report it
-        // (the caller can fall back to the FP64 factorization).
-        if (rep) *rep = r;
-        return kErrNotConverged;
-    }
 
     // X = 0, R = B
     ELEMAMD_HIP_TRY(launch_copy_f64(n, k, B, ldb, w.R, n, s));
@@ -426,7 +488,7 @@ int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, 
         if (!gmres) {
             // d = M^{-1} r in float, x += d
             ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, w.R, n, w.nrm + k, w.Rf, n, s));
-            rc = solve_apply<float>(true, true, n, k, w.S, n, w.Rf, n, w.Dinv, w.ctrl, nullptr, s);
+            rc = solve_apply<float>(true, true, n, k, f.S, n, w.Rf, n, f.Dinv, w.ctrl, nullptr, s);
             if (rc) return rc;
             ELEMAMD_HIP_TRY(launch_axpy_f32_to_f64_scaled(n, k, w.Rf, n, w.nrm + k, w.X, n, s));
         } else {

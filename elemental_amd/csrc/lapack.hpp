@@ -7,6 +7,12 @@ namespace elemamd {
 
 enum : int { kTrailingF64 = 0, kTrailingF32 = 1, kTrailingBf16x3 = 2 };
 
+// Columns of panels applied per pass over the trailing matrix (a multiple of
+// the block size nb is used: max(1, depth / nb) panels).
+constexpr int kUpdateDepthF64 = 256;
+constexpr int kUpdateDepthF32 = 512;
+constexpr int kUpdateDepthBf16x3 = 512;
+
 // cholesky::LVar3 on device memory; T = double (FP64 path) or float (the
 // low-precision factor, diagonal blocks still factored in double).
 template <typename T>
@@ -47,6 +53,24 @@ struct MixedReport {
     double backward_error;
     double anorm_inf;
 };
+
+// Low-precision factor kept for repeated solves (device memory of
+// mixed_factor_bytes(n), laid out by carve_factor).
+struct MixedFactor {
+    float* S;         // n x n float factor (ld n), lower triangle
+    float* Dinv;      // inverses of its 64x64 diagonal blocks
+    int* dinfo;       // device status word
+    double* scratch;  // n + 8 doubles
+    double anorm;     // ||A||_inf
+    int lowprec_info;
+    int shifted;
+};
+size_t mixed_factor_bytes(int n);
+MixedFactor carve_factor(void* mem, int n);
+int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFactor& f, hipStream_t s);
+size_t mixed_refine_bytes(int n, int k, const MixedOpts& o);
+int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o,
+                 const MixedFactor& f, void* work, size_t work_bytes, MixedReport* rep, hipStream_t s);
 
 size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o);
 int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o, void* work,

@@ -29,31 +29,47 @@
 
 namespace elemamd {
 
+// Number of leading panel columns still to apply after a factorization
+// failure: the panels are global columns [kcol0, kcol0 + k) of the factor,
+// and a failing pivot at column c (info = c + 1) voids the nbp-block holding
+// c and every later one, exactly the panels the reference never computes
+// before it throws.  Without info (or info == 0) all k columns apply.
+__device__ inline int syrk_valid_k(const int* info, int k, int kcol0, int nbp) {
+    if (info == nullptr) return k;
+    const int v = *info;
+    if (v == 0) return k;
+    const int lim = ((v - 1) / nbp) * nbp - kcol0;
+    return lim < 0 ? 0 : (lim < k ? lim : k);
+}
+
 __global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
-        double* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
-    if (info != nullptr && *info != 0 && *info <= limit) return;
+        double* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
+    const int ke = syrk_valid_k(info, k, kcol0, nbp);
+    if (info != nullptr && ke == 0) return;
     int ti, tj;
     if (!syrk_tile(m, n, ti, tj)) return;
-    syrk_tile_f64(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
+    syrk_tile_f64(m, n, ke, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
 }
 
 __global__ __launch_bounds__(256, 2) void syrk_ln_f32_kernel(int m, int n, int k, float alpha,
         const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
-        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
-    if (info != nullptr && *info != 0 && *info <= limit) return;
+        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
+    const int ke = syrk_valid_k(info, k, kcol0, nbp);
+    if (info != nullptr && ke == 0) return;
     int ti, tj;
     if (!syrk_tile(m, n, ti, tj)) return;
-    syrk_tile_f32(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
+    syrk_tile_f32(m, n, ke, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
 }
 
 __global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3_kernel(int m, int n, int k, float alpha,
         const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
-        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int limit) {
-    if (info != nullptr && *info != 0 && *info <= limit) return;
+        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
+    const int ke = syrk_valid_k(info, k, kcol0, nbp);
+    if (info != nullptr && ke == 0) return;
     int ti, tj;
     if (!syrk_tile(m, n, ti, tj)) return;
-    syrk_tile_bf16x3(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
+    syrk_tile_bf16x3(m, n, ke, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj);
 }
 
 static dim3 syrk_grid(int m, int n) {
@@ -63,24 +79,22 @@ static dim3 syrk_grid(int m, int n) {
 }
 
 hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
-                              double beta, double* C, i64 ldc, const int* info, int limit, hipStream_t s) {
+                              double beta, double* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
     hipLaunchKernelGGL(syrk_ln_f64_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                       ldc, info, limit);
+                       ldc, info, kcol0, nbp);
     return hipGetLastError();
 }
 
-
-
 hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, const float* A, i64 lda, const float* B,
-                              i64 ldb, float beta, float* C, i64 ldc, const int* info, int limit, hipStream_t s) {
+                              i64 ldb, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
     if (bf16x3)
         hipLaunchKernelGGL(syrk_ln_bf16x3_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta,
-                           C, ldc, info, limit);
+                           C, ldc, info, kcol0, nbp);
     else
         hipLaunchKernelGGL(syrk_ln_f32_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                           ldc, info, limit);
+                           ldc, info, kcol0, nbp);
     return hipGetLastError();
 }
 

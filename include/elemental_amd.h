@@ -215,6 +215,57 @@ int elemamd_batch_backward_error(const elemamd_batch_plan *plan, const double *A
                                  double *R, double *eta, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Serving loop (no reference counterpart): continuous batching of      */
+/* factor-and-solve requests of mixed size on one GPU.                  */
+
+#define ELEMAMD_ERR_NOT_HPD (-5)   /* a served FP64 factorization met a non-positive pivot */
+
+typedef struct elemamd_server elemamd_server;
+
+typedef struct {
+    int slots;              /* requests running at once, one HIP stream each (1..64) */
+    int precision;          /* ELEMAMD_BATCH_F64, _F32_IR or _BF16X3_IR */
+    int nb;                 /* block size (0 -> 128) */
+    double tol;             /* refinement target (0 -> 1e-14) */
+    size_t memory_budget;   /* bytes of HBM the server may reserve (0 -> free HBM at creation minus a reserve) */
+} elemamd_server_config;
+
+typedef struct {
+    int64_t ticket;
+    int status;             /* ELEMAMD_OK or a negative error (ELEMAMD_ERR_NOT_HPD: info = failing column) */
+    int info;
+    int factored;           /* 1: this request computed its factor; 0: it reused a resident one */
+    int iterations;         /* refinement steps (low-precision modes) */
+    int gmres_iterations;
+    int slot;
+    double backward_error;  /* low-precision modes: final normwise backward error */
+    double t_submit, t_start, t_done;  /* seconds since the server was created */
+} elemamd_serve_result;
+
+typedef struct {
+    int64_t completed, reuse_hits, evictions;
+    size_t peak_reserved, reserved, budget;
+    int resident_factors;
+} elemamd_server_statistics;
+
+/* Start `slots` worker threads on the current device. */
+int elemamd_server_create(const elemamd_server_config *cfg, elemamd_server **server);
+/* Queue a request: solve A X = B (A n x n device, lower triangle read and
+ * never modified; B n x k device, overwritten with X).  The factor is kept
+ * resident under `key`; with reuse = 1 a resident factor of `key` is used
+ * instead of factoring again.  A and B must stay valid until the request is
+ * waited for.  Thread-safe; returns at once with a ticket. */
+int elemamd_server_submit(elemamd_server *server, int key, int reuse, int n, int k, const double *A, int64_t lda,
+                          double *B, int64_t ldb, int64_t *ticket);
+/* Block until the ticket's request completes; fills *result (may be NULL). */
+int elemamd_server_wait(elemamd_server *server, int64_t ticket, elemamd_serve_result *result);
+/* Release the resident factor of `key` (an error if a request is using it). */
+int elemamd_server_evict(elemamd_server *server, int key);
+int elemamd_server_stats(elemamd_server *server, elemamd_server_statistics *stats);
+/* Drain the queue, stop the workers, free every resident factor. */
+int elemamd_server_destroy(elemamd_server *server);
+
+/* ------------------------------------------------------------------ */
 /* Utilities.                                                          */
 
 /* Fill a rows x cols column-major block with N(0,1) numbers: entry t in
