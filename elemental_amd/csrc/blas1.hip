@@ -1,3 +1,4 @@
+#include <algorithm>
 // Bandwidth-bound helpers: fills, precision conversions, the symmetric
 // (lower-stored) product used by the FP64 iterative-refinement residual
 // r = b - A*x, and small reductions for the convergence tests.
@@ -145,7 +146,7 @@ template <bool ABS>
 __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double alpha, const double* __restrict__ A,
                                                          i64 lda, const double* __restrict__ X, i64 ldx, double beta,
                                                          const double* __restrict__ B, i64 ldb, double* __restrict__ R,
-                                                         i64 ldr) {
+                                                         i64 ldr, int nsplit, double* __restrict__ part) {
     __shared__ double sM[2][kSymT][kSymLd];     // sM[q][p] = Asym(p0+p, q0+q)
     __shared__ double sX[2][kSymT][kSymCols + 1];  // sX[q][c] = X(q0+q, c_base+c)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -154,6 +155,8 @@ __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double al
     const int c_base = blockIdx.y * kSymCols;
     const int kcl = ABS ? 1 : min(kSymCols, k - c_base);
     const int nT = (int)ceil_div(n, kSymT);
+    // this workgroup's share of the tiles of the block row (split over blockIdx.z)
+    const int Jb = (int)((i64)nT * blockIdx.z / nsplit), Je = (int)((i64)nT * (blockIdx.z + 1) / nsplit);
 
     double gm[16], gx[8];
     // stage tile J into registers
@@ -216,14 +219,14 @@ __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double al
     d4_t acc[2];
     acc[0] = (d4_t){0.0, 0.0, 0.0, 0.0};
     acc[1] = (d4_t){0.0, 0.0, 0.0, 0.0};
-    gload(0);
-    sstore(0, 0);
+    gload(Jb);
+    sstore(Jb, 0);
     __syncthreads();
-    for (int J = 0; J < nT; ++J) {
-        const int buf = J & 1;
+    for (int J = Jb; J < Je; ++J) {
+        const int buf = (J - Jb) & 1;
         // branch-free body: the last iteration restages its own tile into the
         // spare buffer, which is never read again
-        gload((J + 1) < nT ? J + 1 : J);
+        gload((J + 1) < Je ? J + 1 : J);
         // wave w: output rows p = 16w .. 16w+15, both 16-column blocks of c
 #pragma unroll
         for (int ks = 0; ks < kSymT / 4; ++ks) {
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double al
             acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag, acc[0], 0, 0, 0);
             acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag, acc[1], 0, 0, 0);
         }
-        sstore((J + 1) < nT ? J + 1 : J, buf ^ 1);
+        sstore((J + 1) < Je ? J + 1 : J, buf ^ 1);
         __syncthreads();
     }
     // dependent zero-operand MFMAs right before the accumulators are read, so
@@ -244,6 +247,9 @@ __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double al
     acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, acc[1], 0, 0, 0);
     const i64 p = p0 + 16 * w + (lane & 15);
     if (p < n) {
+        // one split: the result; several: this split's partial sum, combined
+        // by symm_combine_kernel in a fixed order
+        double* P = nsplit > 1 ? part + (i64)blockIdx.z * n * (ABS ? 1 : k) : nullptr;
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
@@ -251,37 +257,72 @@ __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double al
                 const int c = 16 * cb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
                 if (c < kcl) {
                     const i64 cc = c_base + c;
-                    const double bv = (beta == 0.0) ? 0.0 : beta * B[p + cc * ldb];
-                    R[p + cc * ldr] = bv + alpha * acc[cb][v];
+                    if (nsplit > 1) {
+                        P[p + cc * n] = acc[cb][v];
+                    } else {
+                        const double bv = (beta == 0.0) ? 0.0 : beta * B[p + cc * ldb];
+                        R[p + cc * ldr] = bv + alpha * acc[cb][v];
+                    }
                 }
             }
     }
 }
 
-hipError_t launch_symm_lower_residual(int n, int k, const double* A, i64 lda, const double* X, i64 ldx,
-                                      const double* B, i64 ldb, double* R, i64 ldr, hipStream_t s) {
+// R = beta*B + alpha * sum_s part[s], s in increasing order
+__global__ void symm_combine_kernel(int n, int k, int nsplit, const double* __restrict__ part, double alpha,
+                                    double beta, const double* __restrict__ B, i64 ldb, double* __restrict__ R,
+                                    i64 ldr) {
+    const int c = blockIdx.y;
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int s = 0; s < nsplit; ++s) acc += part[(i64)s * n * k + i + (i64)c * n];
+    const double bv = (beta == 0.0) ? 0.0 : beta * B[i + (i64)c * ldb];
+    R[i + (i64)c * ldr] = bv + alpha * acc;
+}
+
+// Splits of the tile sweep: enough workgroups to keep several tiles in
+// flight per CU (the kernel is bound by HBM latency otherwise).
+int symm_splits(int n, int k) {
+    const i64 rows = ceil_div(n, kSymT) * ceil_div(k, kSymCols);
+    i64 s = ceil_div(1024, rows);
+    s = std::min<i64>(std::max<i64>(s, 1), 8);
+    s = std::min<i64>(s, ceil_div(n, kSymT));
+    return (int)s;
+}
+size_t symm_part_count(int n, int k) {
+    const int s = symm_splits(n, k);
+    return s > 1 ? (size_t)s * n * k : 0;
+}
+
+template <bool ABS>
+static hipError_t symm_launch(int n, int k, double alpha, const double* A, i64 lda, const double* X, i64 ldx,
+                              double beta, const double* B, i64 ldb, double* R, i64 ldr, double* part, hipStream_t s) {
     if (n <= 0 || k <= 0) return hipSuccess;
-    dim3 grid((unsigned)ceil_div(n, kSymT), (unsigned)ceil_div(k, kSymCols));
-    hipLaunchKernelGGL(symm_lower_kernel<false>, grid, dim3(256), 0, s, n, k, -1.0, A, lda, X, ldx, 1.0, B, ldb, R,
-                       ldr);
+    const int ns = part != nullptr ? symm_splits(n, k) : 1;
+    dim3 grid((unsigned)ceil_div(n, kSymT), ABS ? 1u : (unsigned)ceil_div(k, kSymCols), (unsigned)ns);
+    hipLaunchKernelGGL(symm_lower_kernel<ABS>, grid, dim3(256), 0, s, n, k, alpha, A, lda, X, ldx, beta, B, ldb, R,
+                       ldr, ns, part);
+    if (ns > 1)
+        hipLaunchKernelGGL(symm_combine_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)k), dim3(256), 0, s, n, k,
+                           ns, part, alpha, beta, B, ldb, R, ldr);
     return hipGetLastError();
+}
+
+hipError_t launch_symm_lower_residual(int n, int k, const double* A, i64 lda, const double* X, i64 ldx,
+                                      const double* B, i64 ldb, double* R, i64 ldr, double* part, hipStream_t s) {
+    return symm_launch<false>(n, k, -1.0, A, lda, X, ldx, 1.0, B, ldb, R, ldr, part, s);
 }
 
 hipError_t launch_symm_lower(int n, int k, const double* A, i64 lda, const double* X, i64 ldx, double* Y, i64 ldy,
-                             hipStream_t s) {
-    if (n <= 0 || k <= 0) return hipSuccess;
-    dim3 grid((unsigned)ceil_div(n, kSymT), (unsigned)ceil_div(k, kSymCols));
-    hipLaunchKernelGGL(symm_lower_kernel<false>, grid, dim3(256), 0, s, n, k, 1.0, A, lda, X, ldx, 0.0, Y, ldy, Y,
-                       ldy);
-    return hipGetLastError();
+                             double* part, hipStream_t s) {
+    return symm_launch<false>(n, k, 1.0, A, lda, X, ldx, 0.0, Y, ldy, Y, ldy, part, s);
 }
 
-hipError_t launch_sym_lower_abs_rowsum(int n, const double* A, i64 lda, double* rowsum, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    dim3 grid((unsigned)ceil_div(n, kSymT), 1);
-    hipLaunchKernelGGL(symm_lower_kernel<true>, grid, dim3(256), 0, s, n, 1, 1.0, A, lda, (const double*)nullptr,
-                       (i64)1, 0.0, (const double*)nullptr, (i64)1, rowsum, (i64)n);
-    return hipGetLastError();
+hipError_t launch_sym_lower_abs_rowsum(int n, const double* A, i64 lda, double* rowsum, double* part,
+                                       hipStream_t s) {
+    return symm_launch<true>(n, 1, 1.0, A, lda, (const double*)nullptr, (i64)1, 0.0, (const double*)nullptr,
+                             (i64)1, rowsum, (i64)n, part, s);
 }
 
 // ------------------------------------------------------------ reductions

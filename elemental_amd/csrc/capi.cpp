@@ -190,7 +190,10 @@ int elemamd_sym_residual_f64(int n, int k, const double* A, int64_t lda, const d
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldx < std::max(1, n) || ldb < std::max(1, n) ||
         ldr < std::max(1, n))
         return ELEMAMD_ERR_ARG;
-    return launch_symm_lower_residual(n, k, A, lda, X, ldx, B, ldb, R, ldr, S(stream)) == hipSuccess
+    double* part = nullptr;
+    const size_t pc = symm_part_count(n, k);
+    if (pc > 0 && (part = (double*)stream_scratch(pc * sizeof(double), S(stream))) == nullptr) return ELEMAMD_ERR_HIP;
+    return launch_symm_lower_residual(n, k, A, lda, X, ldx, B, ldb, R, ldr, part, S(stream)) == hipSuccess
                ? ELEMAMD_OK
                : ELEMAMD_ERR_HIP;
 }

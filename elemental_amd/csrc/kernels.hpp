@@ -21,6 +21,12 @@ template <typename T>
 hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
                            hipStream_t s);
 
+// The same solve in 32-column sub-blocks (MFMA products between them): the
+// panel solve of the blocked factorization.
+template <typename T>
+hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
+                                   hipStream_t s);
+
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
 // With info != null the k columns are factor columns [kcol0, kcol0 + k) and a
 // recorded failure drops the panels of its nbp-block and later ones.
@@ -58,15 +64,21 @@ hipError_t launch_axpy_f32_to_f64_scaled(int m, int n, const float* D, i64 ldd, 
                                          i64 ldx, hipStream_t s);
 hipError_t launch_copy_f64(int m, int n, const double* X, i64 ldx, double* Y, i64 ldy, hipStream_t s);
 // R := B - A*X with A symmetric (lower triangle read), n x n; X, B, R n x k.
+// `part` (symm_part_count(n, k) doubles, or null for one unsplit sweep)
+// holds the partial sums of the split tile sweep; they are added in a fixed
+// order, so the result does not depend on scheduling.
+int symm_splits(int n, int k);
+size_t symm_part_count(int n, int k);
 hipError_t launch_symm_lower_residual(int n, int k, const double* A, i64 lda, const double* X, i64 ldx,
-                                      const double* B, i64 ldb, double* R, i64 ldr, hipStream_t s);
+                                      const double* B, i64 ldb, double* R, i64 ldr, double* part, hipStream_t s);
 // Y := A*X with A symmetric (lower triangle read).
 hipError_t launch_symm_lower(int n, int k, const double* A, i64 lda, const double* X, i64 ldx, double* Y, i64 ldy,
-                             hipStream_t s);
+                             double* part, hipStream_t s);
 // colmax[c] = max_i |X(i,c)| for c < k.
 hipError_t launch_col_absmax(int m, int k, const double* X, i64 ldx, double* colmax, hipStream_t s);
-// rowsum[i] = sum_j |A(i,j)| of the symmetric matrix stored in the lower triangle.
-hipError_t launch_sym_lower_abs_rowsum(int n, const double* A, i64 lda, double* rowsum, hipStream_t s);
+// rowsum[i] = sum_j |A(i,j)| of the symmetric matrix stored in the lower
+// triangle; part: symm_part_count(n, 1) doubles or null.
+hipError_t launch_sym_lower_abs_rowsum(int n, const double* A, i64 lda, double* rowsum, double* part, hipStream_t s);
 // out[0] = max_i v[i]
 hipError_t launch_vec_max(int n, const double* v, double* out, hipStream_t s);
 

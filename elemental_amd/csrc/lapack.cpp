@@ -75,7 +75,8 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
                 ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, n - k, b, k - c0, A + k + (i64)c0 * lda, lda, Akk, lda, dinfo,
                                                c0, nb, s));
             ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
-            if (n - k - b > 0) ELEMAMD_HIP_TRY(launch_trsm_rlt<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
+            if (n - k - b > 0)
+                ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
         }
         const int m2 = n - c1;
         if (m2 <= 0) break;
@@ -184,6 +185,7 @@ struct MixedWork {
     double* nrm;   // 8k + n + 8 doubles: norms, row sums, scratch
     double* h;     // (m+1) x k Gram-Schmidt coefficients
     int* ctrl;     // persistent-solve tickets and flags
+    double* part;  // partial sums of the split symmetric product
     size_t bytes;
 };
 
@@ -200,6 +202,7 @@ MixedWork carve_refine(void* work, size_t cap, int n, int k, int m) {
     w.nrm = c.take<double>((size_t)8 * k + n + 8);
     w.h = c.take<double>((size_t)(m + 2) * k);
     w.ctrl = c.take<int>(solve_ctrl_count(n, k));
+    w.part = c.take<double>(symm_part_count(n, k));
     w.bytes = c.off + 256;
     return w;
 }
@@ -272,7 +275,7 @@ int gmres_cycle(Ctx& c, int m, double inner_tol, int* iters_out) {
         double* Vj = c.w.V + (size_t)j * nk;
         double* Vj1 = c.w.V + (size_t)(j + 1) * nk;
         // W = M^{-1} A V_j
-        ELEMAMD_HIP_TRY(launch_symm_lower(n, k, c.A, c.lda, Vj, n, c.w.D, n, c.s));
+        ELEMAMD_HIP_TRY(launch_symm_lower(n, k, c.A, c.lda, Vj, n, c.w.D, n, c.w.part, c.s));
         rc = host_colmax(c, c.w.D, dscale, hs);
         if (rc) return rc;
         rc = precond_apply(c, c.w.D, c.w.W, dscale);
@@ -351,7 +354,7 @@ MixedFactor carve_factor(void* mem, int n) {
     f.S = c.take<float>((size_t)n * n);
     f.Dinv = c.take<float>(solve_dinv_count(n));
     f.dinfo = c.take<int>(4);
-    f.scratch = c.take<double>((size_t)n + 8);
+    f.scratch = c.take<double>((size_t)n + 8 + symm_part_count(n, 1));
     return f;
 }
 
@@ -360,7 +363,7 @@ size_t mixed_factor_bytes(int n) {
     c.take<float>((size_t)n * n);
     c.take<float>(solve_dinv_count(n));
     c.take<int>(4);
-    c.take<double>((size_t)n + 8);
+    c.take<double>((size_t)n + 8 + symm_part_count(n, 1));
     return c.off + 256;
 }
 
@@ -381,7 +384,7 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     if (n == 0) return kOk;
     double* rowsum = f.scratch;
     double* dtmp = f.scratch + n;
-    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, s));
+    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, symm_part_count(n, 1) ? f.scratch + n + 8 : nullptr, s));
     ELEMAMD_HIP_TRY(launch_vec_max(n, rowsum, dtmp, s));
     ELEMAMD_HIP_TRY(hipMemcpyAsync(&f.anorm, dtmp, sizeof(double), hipMemcpyDeviceToHost, s));
     double sigma = 0.0;
@@ -463,7 +466,7 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
     double eta = INFINITY;
     for (int it = 0; it <= o.max_iter; ++it) {
         if (it > 0) {
-            ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, A, lda, w.X, n, B, ldb, w.R, n, s));
+            ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, A, lda, w.X, n, B, ldb, w.R, n, w.part, s));
             r.iterations = it;
         }
         // norms

@@ -60,7 +60,7 @@ struct MixedFactor {
     float* S;         // n x n float factor (ld n), lower triangle
     float* Dinv;      // inverses of its 64x64 diagonal blocks
     int* dinfo;       // device status word
-    double* scratch;  // n + 8 doubles
+    double* scratch;  // n + 8 doubles, then the split row-sum partials
     double anorm;     // ||A||_inf
     int lowprec_info;
     int shifted;
