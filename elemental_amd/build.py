@@ -25,6 +25,7 @@ SOURCES = [
     "syrk.hip",
     "gemm.hip",
     "solve.hip",
+    "solve_chain.hip",
     "blas1.hip",
     "lapack.cpp",
     "batch.hip",

@@ -46,6 +46,11 @@ hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A
 // solve_ctrl_count(n, kc) ints (ctrl[1] != 0 after a solve means a bounded wait
 // expired).
 size_t solve_ctrl_count(int n, int kc);
+// solve_chain.hip -- chain + helpers triangular solve (forward, or backward with L^T)
+size_t chain_ctrl_count(int n, int kc);
+template <typename T>
+hipError_t launch_trsm_chain(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv, int* ctrl,
+                             T* part, const int* info, hipStream_t s);
 size_t solve_dinv_count(int n);
 template <typename T>
 hipError_t launch_trtri_diag64(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s);

@@ -129,8 +129,10 @@ int solve_prepare(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
 template <typename T>
 int solve_apply(bool forward, bool backward, int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
                 int* ctrl, const int* dinfo, hipStream_t s) {
-    if (forward) ELEMAMD_HIP_TRY(launch_trsm_persist<T>(false, n, k, L, ldl, B, ldb, Dinv, ctrl, dinfo, s));
-    if (backward) ELEMAMD_HIP_TRY(launch_trsm_persist<T>(true, n, k, L, ldl, B, ldb, Dinv, ctrl, dinfo, s));
+    // ctrl: solve_ctrl_count(n, k) ints = flags, then the helpers' partial sums
+    T* part = (T*)((char*)ctrl + ((chain_ctrl_count(n, k) * sizeof(int) + 255) & ~(size_t)255));
+    if (forward) ELEMAMD_HIP_TRY(launch_trsm_chain<T>(false, n, k, L, ldl, B, ldb, Dinv, ctrl, part, dinfo, s));
+    if (backward) ELEMAMD_HIP_TRY(launch_trsm_chain<T>(true, n, k, L, ldl, B, ldb, Dinv, ctrl, part, dinfo, s));
     return kOk;
 }
 

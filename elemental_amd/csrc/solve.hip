@@ -28,40 +28,9 @@
 // code, never a hang.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "solve_dev.hpp"
 
 namespace elemamd {
-
-constexpr int kSB = 64;        // block rows
-constexpr int kSC = 16;        // right-hand-side columns per workgroup
-constexpr int kSLd = kSB + 1;
-constexpr long kSpinLimit = 1L << 26;
-
-// MFMA 16x16x4 of the storage type; output row index of accumulator v.
-__device__ inline d4_t mfma16(double a, double b, d4_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-__device__ inline f4_t mfma16(float a, float b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-template <typename T>
-__device__ inline int mfma_row(int lane, int v) {
-    return sizeof(T) == 8 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
-}
-__device__ inline double wave_bcast(double v, int lane) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-__device__ inline float wave_bcast(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-template <typename T>
-struct Acc4;
-template <>
-struct Acc4<double> {
-    typedef d4_t type;
-};
-template <>
-struct Acc4<float> {
-    typedef f4_t type;
-};
 
 // ------------------------------------------------ inverse of diagonal blocks
 // Dinv + I*64*64 (column-major, ld 64) := inv(L(I*64.., I*64..)), upper part 0.
@@ -365,7 +334,12 @@ hipError_t launch_trsm_persist(bool trans, int n, int kc, const T* L, i64 ldl, T
     return hipGetLastError();
 }
 
-size_t solve_ctrl_count(int n, int kc) { return 2 + (size_t)ceil_div(n, kSB) * ceil_div(kc, kSC); }
+// Solve scratch in ints: the chain solve's tickets and flags, then (256-byte
+// aligned) its n x kc partial sums (room for doubles).
+size_t solve_ctrl_count(int n, int kc) {
+    const size_t flags = ((chain_ctrl_count(n, kc) * sizeof(int) + 255) & ~(size_t)255) / sizeof(int);
+    return flags + (size_t)n * kc * 2 + 64;
+}
 size_t solve_dinv_count(int n) { return (size_t)ceil_div(n, kSB) * kSB * kSB; }
 
 template hipError_t launch_trtri_diag64<double>(int, const double*, i64, double*, hipStream_t);

@@ -1,0 +1,60 @@
+// Device helpers shared by the persistent triangular-solve kernels.
+#pragma once
+
+#include "common.hpp"
+
+namespace elemamd {
+
+constexpr int kSB = 64;        // block rows
+constexpr int kSC = 16;        // right-hand-side columns per workgroup
+constexpr int kSLd = kSB + 1;
+constexpr long kSpinLimit = 1L << 26;
+
+// MFMA 16x16x4 of the storage type; output row index of accumulator v.
+__device__ inline d4_t mfma16(double a, double b, d4_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline f4_t mfma16(float a, float b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+template <typename T>
+__device__ inline int mfma_row(int lane, int v) {
+    return sizeof(T) == 8 ? (lane >> 4) + 4 * v : 4 * (lane >> 4) + v;
+}
+__device__ inline double wave_bcast(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+__device__ inline float wave_bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<double> {
+    typedef d4_t type;
+};
+template <>
+struct Acc4<float> {
+    typedef f4_t type;
+};
+
+// Store of a value another workgroup will read (device-coherent, bypassing
+// the per-XCD caches that are not coherent across XCDs) and the matching load.
+template <typename T>
+__device__ inline void coherent_store(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ inline T coherent_load(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every wave waits until its coherent stores are acknowledged, then one
+// thread raises the flag (coherent store): a reader that sees the flag and
+// then loads coherently sees the data.
+__device__ inline void publish_flag(int* flag, int t) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (t == 0) coherent_store(flag, 1);
+}
+
+}  // namespace elemamd
