@@ -137,18 +137,34 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(BatchArgs a, const T* 
                     const int q0 = J * kBS;
                     const int qs = min(kBS, n - q0);
                     __syncthreads();
-                    // tile M[q][p]: forward L(r0+p, q0+q); backward L(q0+q, r0+p)
-                    for (int idx = t; idx < kBS * kBS; idx += 256) {
-                        const int x = idx & 63, y = idx >> 6;
-                        if (!back) {  // lanes along p (rows of L)
-                            sL[y][x] = (x < bs && y < qs) ? L[(r0 + x) + (i64)(q0 + y) * n] : T(0);
-                        } else {  // lanes along q
-                            sL[x][y] = (x < qs && y < bs) ? L[(q0 + x) + (i64)(r0 + y) * n] : T(0);
+                    // tile M[q][p]: forward L(r0+p, q0+q); backward L(q0+q, r0+p).
+                    // All loads of a thread first (clamped addresses), then LDS.
+                    {
+                        const int x = t & 63, y0 = t >> 6;
+                        T g[16], gx[4];
+#pragma unroll
+                        for (int s2 = 0; s2 < 16; ++s2) {
+                            const int y = y0 + 4 * s2;
+                            const bool ok = !back ? (x < bs && y < qs) : (x < qs && y < bs);
+                            const i64 off = !back ? (r0 + x) + (i64)(q0 + y) * n : (q0 + x) + (i64)(r0 + y) * n;
+                            const T v = L[ok ? off : 0];
+                            g[s2] = ok ? v : T(0);
                         }
-                    }
-                    for (int idx = t; idx < kBS * kBC; idx += 256) {
-                        const int q = idx & 63, c = idx >> 6;
-                        sX[q][c] = (q < qs && c < kc) ? B[(q0 + q) + (i64)(cs + c) * n] : T(0);
+#pragma unroll
+                        for (int s2 = 0; s2 < 4; ++s2) {
+                            const int c = y0 + 4 * s2;
+                            const bool ok = x < qs && c < kc;
+                            const T v = B[ok ? (q0 + x) + (i64)(cs + c) * n : 0];
+                            gx[s2] = ok ? v : T(0);
+                        }
+#pragma unroll
+                        for (int s2 = 0; s2 < 16; ++s2) {
+                            const int y = y0 + 4 * s2;
+                            if (!back) sL[y][x] = g[s2];
+                            else sL[x][y] = g[s2];
+                        }
+#pragma unroll
+                        for (int s2 = 0; s2 < 4; ++s2) sX[x][y0 + 4 * s2] = gx[s2];
                     }
                     __syncthreads();
                     for (int q = 0; q < qs; ++q) {
@@ -165,17 +181,20 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(BatchArgs a, const T* 
                     sR[pl][c] = (pl < bs && c < kc) ? B[(r0 + pl) + (i64)(cs + c) * n] - acc[u] : T(0);
                 }
                 // diagonal block in lower-triangular form on (possibly reversed) indices
-                for (int idx = t; idx < kBS * kBS; idx += 256) {
-                    const int x = idx & 63, y = idx >> 6;  // M(x, y) into sL[y][x]
-                    const int gi = back ? (kBS - 1 - y) : x;
-                    const int gk = back ? (kBS - 1 - x) : y;
-                    T v = T(0);
-                    if (gi < bs && gk < bs) {
-                        if (x >= y) v = L[(r0 + gi) + (i64)(r0 + gk) * n];
-                    } else if (x == y) {
-                        v = T(1);
+                {
+                    const int x = t & 63, y0 = t >> 6;  // M(x, y) into sL[y][x]
+                    T g[16];
+#pragma unroll
+                    for (int s2 = 0; s2 < 16; ++s2) {
+                        const int y = y0 + 4 * s2;
+                        const int gi = back ? (kBS - 1 - y) : x;
+                        const int gk = back ? (kBS - 1 - x) : y;
+                        const bool ok = gi < bs && gk < bs && x >= y;
+                        const T v = L[ok ? (r0 + gi) + (i64)(r0 + gk) * n : 0];
+                        g[s2] = ok ? v : ((x == y) ? T(1) : T(0));
                     }
-                    sL[y][x] = v;
+#pragma unroll
+                    for (int s2 = 0; s2 < 16; ++s2) sL[y0 + 4 * s2][x] = g[s2];
                 }
                 __syncthreads();
                 if (t < kBS) sRinv[t] = T(1) / sL[t][t];
@@ -268,21 +287,48 @@ __global__ __launch_bounds__(256) void batch_symv_residual_kernel(BatchArgs a, c
         for (int J = 0; J < nT; ++J) {
             const int q0 = J * 64;
             __syncthreads();
-            for (int idx = t; idx < 64 * 64; idx += 256) {
-                const int x = idx & 63, y = idx >> 6;
-                if (J < I) {  // stored tile A(r0 + x, q0 + y): lanes along rows
-                    sM[y][x] = (r0 + x < n && q0 + y < n) ? Ap[(r0 + x) + (i64)(q0 + y) * n] : 0.0;
-                } else if (J > I) {  // Asym(r0 + y, q0 + x) = A(q0 + x, r0 + y): lanes along rows of A
-                    sM[x][y] = (q0 + x < n && r0 + y < n) ? Ap[(q0 + x) + (i64)(r0 + y) * n] : 0.0;
-                } else if (x >= y) {  // diagonal tile, lower part mirrored
-                    const double v = (r0 + x < n) ? Ap[(r0 + x) + (i64)(r0 + y) * n] : 0.0;
-                    sM[y][x] = v;
-                    sM[x][y] = v;
+            {  // all 16 + 2 loads of a thread in flight (clamped addresses), then LDS
+                const int x = t & 63, y0 = t >> 6;
+                double g[16], gx[2];
+#pragma unroll
+                for (int s2 = 0; s2 < 16; ++s2) {
+                    const int y = y0 + 4 * s2;
+                    bool ok;
+                    i64 off;
+                    if (J < I) {  // stored tile A(r0 + x, q0 + y): lanes along rows
+                        ok = r0 + x < n && q0 + y < n;
+                        off = (r0 + x) + (i64)(q0 + y) * n;
+                    } else if (J > I) {  // Asym(r0 + y, q0 + x) = A(q0 + x, r0 + y)
+                        ok = q0 + x < n && r0 + y < n;
+                        off = (q0 + x) + (i64)(r0 + y) * n;
+                    } else {  // diagonal tile, lower part (mirrored below)
+                        ok = r0 + x < n && x >= y;
+                        off = (r0 + x) + (i64)(r0 + y) * n;
+                    }
+                    const double v = Ap[ok ? off : 0];
+                    g[s2] = ok ? v : 0.0;
                 }
-            }
-            for (int idx = t; idx < 64 * 8; idx += 256) {
-                const int q = idx & 63, c = idx >> 6;
-                sX[q][c] = (q0 + q < n && c < kc) ? Xp[(q0 + q) + (i64)(c0 + c) * n] : 0.0;
+#pragma unroll
+                for (int s2 = 0; s2 < 2; ++s2) {
+                    const int c = y0 + 4 * s2;
+                    const bool ok = q0 + x < n && c < kc;
+                    const double v = Xp[ok ? (q0 + x) + (i64)(c0 + c) * n : 0];
+                    gx[s2] = ok ? v : 0.0;
+                }
+#pragma unroll
+                for (int s2 = 0; s2 < 16; ++s2) {
+                    const int y = y0 + 4 * s2;
+                    if (J < I) {
+                        sM[y][x] = g[s2];
+                    } else if (J > I) {
+                        sM[x][y] = g[s2];
+                    } else if (x >= y) {
+                        sM[y][x] = g[s2];
+                        sM[x][y] = g[s2];
+                    }
+                }
+#pragma unroll
+                for (int s2 = 0; s2 < 2; ++s2) sX[x][y0 + 4 * s2] = gx[s2];
             }
             __syncthreads();
 #pragma unroll 8

@@ -58,11 +58,27 @@ __device__ __forceinline__ void trsm_rlt_rows(int m, int nb, const T* __restrict
     if (info != nullptr && *info != 0 && *info <= limit) return;
     const int t = threadIdx.x;
     const int n = nb;
-    for (int idx = t; idx < n * n; idx += 256) {
-        const int i = idx % n, j = idx / n;
-        sL[i + j * kTrsmMaxN] = (i >= j) ? (double)L[i + (i64)j * ldl] : 0.0;
+    // stage L11 (lower, zero above) with all loads of a thread in flight at once:
+    // thread t covers rows i = t % 128 of columns j = t / 128 + 2s
+    {
+        const int i = t & (kTrsmMaxN - 1), j0 = t >> 7;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {  // four batches of 16 loads in flight
+            T g[16];
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int j = j0 + 2 * (16 * h + s);
+                const bool ok = i < n && j < n && i >= j;
+                g[s] = L[ok ? i + (i64)j * ldl : 0];
+            }
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                const int j = j0 + 2 * (16 * h + s);
+                sL[i + j * kTrsmMaxN] = (i < n && j < n && i >= j) ? (double)g[s] : 0.0;
+            }
+        }
     }
-    for (int k = t; k < n; k += 256) sRinv[k] = 1.0 / (double)L[k + (i64)k * ldl];
+    if (t < n) sRinv[t] = 1.0 / (double)L[t + (i64)t * ldl];
     __syncthreads();
 
     const int q = t & 3;
@@ -112,21 +128,45 @@ __device__ __forceinline__ void trsm_rlt_blocked_rows(int m, int nb, const T* __
     const int nc = (nb + kPanelSub - 1) / kPanelSub;
     const int ncol = nc * kPanelSub;  // padded width
     // slab of B (rows past m and padded columns are zero)
-    for (int idx = t; idx < kTrsmRowsPerBlock * ncol; idx += 256) {
-        const int r = idx % kTrsmRowsPerBlock, j = idx / kTrsmRowsPerBlock;
-        sX[r][j] = (r0 + r < m && j < nb) ? (double)B[r0 + r + (i64)j * ldb] : 0.0;
+    {  // thread: row r = t % 64 of columns t / 64 + 4s.  Loads from clamped
+       // addresses without branches, all in flight before the first LDS store.
+        const int r = t & 63, j0 = t >> 6;
+        const bool rin = r0 + r < m;
+        T g[kTrsmMaxN / 4];
+#pragma unroll
+        for (int s = 0; s < kTrsmMaxN / 4; ++s) {
+            const int j = j0 + 4 * s;
+            const bool ok = rin && j < nb;
+            g[s] = B[ok ? r0 + r + (i64)j * ldb : 0];
+        }
+#pragma unroll
+        for (int s = 0; s < kTrsmMaxN / 4; ++s) {
+            const int j = j0 + 4 * s;
+            if (j < ncol) sX[r][j] = (rin && j < nb) ? (double)g[s] : 0.0;
+        }
     }
     for (int c = 0; c < nc; ++c) {
         const int c0 = c * kPanelSub;
         const int kw = c0 + kPanelSub;  // columns of this block row of L that are used
         __syncthreads();                 // previous substitution done with sLr / sX
-        for (int idx = t; idx < kPanelSub * kw; idx += 256) {
-            const int i = idx % kPanelSub, j = idx / kPanelSub;
+        {  // thread: row i = t % 32 of columns t / 32 + 8s; loads first, then LDS
+            const int i = t & 31, jj = t >> 5;
             const int gi = c0 + i;
-            double v;
-            if (gi < nb && j < nb) v = (j <= gi) ? (double)L[gi + (i64)j * ldl] : 0.0;
-            else v = (gi == j) ? 1.0 : 0.0;  // identity on the padding
-            sLr[i][j] = v;
+            T g[kTrsmMaxN / 8];
+#pragma unroll
+            for (int s = 0; s < kTrsmMaxN / 8; ++s) {
+                const int j = jj + 8 * s;
+                const bool ok = gi < nb && j <= gi;
+                g[s] = L[ok ? gi + (i64)j * ldl : 0];
+            }
+#pragma unroll
+            for (int s = 0; s < kTrsmMaxN / 8; ++s) {
+                const int j = jj + 8 * s;
+                double v;
+                if (gi < nb && j < nb) v = (j <= gi) ? (double)g[s] : 0.0;
+                else v = (gi == j) ? 1.0 : 0.0;  // identity on the padding
+                if (j < kw) sLr[i][j] = v;
+            }
         }
         if (t < kPanelSub) {
             const int gi = c0 + t;
@@ -177,9 +217,15 @@ __device__ __forceinline__ void trsm_rlt_blocked_rows(int m, int nb, const T* __
         }
     }
     __syncthreads();
-    for (int idx = t; idx < kTrsmRowsPerBlock * nb; idx += 256) {
-        const int r = idx % kTrsmRowsPerBlock, j = idx / kTrsmRowsPerBlock;
-        if (r0 + r < m) B[r0 + r + (i64)j * ldb] = (T)sX[r][j];
+    {
+        const int r = t & 63, j0 = t >> 6;
+        if (r0 + r < m) {
+#pragma unroll
+            for (int s = 0; s < kTrsmMaxN / 4; ++s) {
+                const int j = j0 + 4 * s;
+                if (j < nb) B[r0 + r + (i64)j * ldb] = (T)sX[r][j];
+            }
+        }
     }
 }
 

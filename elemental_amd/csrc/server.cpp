@@ -96,8 +96,10 @@ Server::~Server() {
     }
     cv_.notify_all();
     for (auto& t : workers_) t.join();
+    // factor memory comes from the stream-ordered pool: return it the same way
     for (auto& kv : cache_)
-        if (kv.second->mem) (void)hipFree(kv.second->mem);
+        if (kv.second->mem) (void)hipFreeAsync(kv.second->mem, nullptr);
+    (void)hipStreamSynchronize(nullptr);
 }
 
 size_t Server::factor_bytes(int n) const {
@@ -146,7 +148,10 @@ int Server::evict(int key) {
     auto it = cache_.find(key);
     if (it == cache_.end()) return kOk;
     if (it->second->pins > 0 || !it->second->ready) return kErrArg;
-    if (it->second->mem) (void)hipFree(it->second->mem);
+    if (it->second->mem) {
+        (void)hipFreeAsync(it->second->mem, nullptr);
+        (void)hipStreamSynchronize(nullptr);
+    }
     used_ -= it->second->bytes;
     cache_.erase(it);
     return kOk;

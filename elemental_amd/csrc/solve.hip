@@ -270,6 +270,7 @@ __global__ __launch_bounds__(256) void trsm_persist_kernel(int n, int kc, const 
         T v[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) v[u] = sY[0][myrow][4 * w + u];
+#pragma unroll 16
         for (int kk = 0; kk < kSB; ++kk) {
             const T rk = sY[1][kk][0];
             const T lik = sM[0][kk][lane];

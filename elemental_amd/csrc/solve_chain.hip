@@ -289,6 +289,9 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
             T v[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) v[u] = sR[myrow][4 * w + u];
+            // unrolled so the LDS reads of the coming columns are issued early;
+            // the sweep's dependence runs only through v (lane kk's value)
+#pragma unroll 16
             for (int kk = 0; kk < kSB; ++kk) {
                 const T rk = sRinv[kk];
                 const T lik = sM[buf][kk][lane];

@@ -39,6 +39,13 @@ def make(n, k, precision="f64", buckets=None, seed=SEED):
     return plan, A, B
 
 
+def colmajor_copy(t):
+    """A new column-major (unit row stride) copy of a 2-D tensor."""
+    out = torch.empty(t.shape[1], t.shape[0], dtype=t.dtype, device=t.device).t()
+    out.copy_(t)
+    return out
+
+
 def rel(a, b):
     d = (a - b).abs().max().item()
     s = b.abs().max().item()
@@ -87,8 +94,8 @@ def test_f64_batch_matches_single_path_and_oracle(oracle):
     assert eta.max() <= 1e-14, eta.max()
     for i in range(0, len(n), 7):
         ni, ki = int(n[i]), int(k[i])
-        Ai = plan.a_view(A0, i).t().contiguous().t()  # column-major copy, ld = n
-        Bi = plan.b_view(B0, i).t().contiguous().t()
+        Ai = colmajor_copy(plan.a_view(A0, i))  # ld = n
+        Bi = colmajor_copy(plan.b_view(B0, i))
         d = torch.zeros(1, dtype=torch.int32, device="cuda")
         rc = lib().elemamd_hpd_solve_f64(b"L", b"N", ni, ki, ctypes.c_void_p(Ai.data_ptr()), ni,
                                          ctypes.c_void_p(Bi.data_ptr()), ni, nb, ctypes.c_void_p(d.data_ptr()), S())
@@ -97,9 +104,12 @@ def test_f64_batch_matches_single_path_and_oracle(oracle):
         assert torch.equal(torch.tril(Ai), Lb), f"factor of matrix {i} differs from the single path"
         assert rel(plan.b_view(B, i), Bi) <= 1e-12
         # oracle: the reference's HPDSolve on the host
-        Ah = np.asfortranarray(plan.a_view(A0, i).cpu().numpy())
-        Bh = np.asfortranarray(plan.b_view(B0, i).cpu().numpy())
-        assert oracle.hpd_solve(Ah, Bh, nb=nb) == 0
+        Ah = np.array(plan.a_view(A0, i).cpu().numpy(), order="F", copy=True)
+        Bh = np.array(plan.b_view(B0, i).cpu().numpy(), order="F", copy=True)
+        assert np.abs(Ah - Ah.T).max() <= 1e-14 * np.abs(Ah).max(), f"matrix {i} not symmetric on the host"
+        assert np.linalg.eigvalsh(Ah).min() > 0, f"matrix {i} not HPD on the host"
+        info_h = oracle.hpd_solve(Ah, Bh, nb=nb)
+        assert info_h == 0, (i, ni, nb, info_h)
         assert np.abs(plan.b_view(B, i).cpu().numpy() - Bh).max() <= 1e-12 * np.abs(Bh).max()
 
 

@@ -18,7 +18,7 @@ if not torch.cuda.is_available():  # pragma: no cover - CPU container
     pytest.skip("no GPU", allow_module_level=True)
 
 from elemental_amd import inputs  # noqa: E402
-from elemental_amd.serve import Server  # noqa: E402
+from elemental_amd.serve import Server, result_dict  # noqa: E402
 
 SIZES = [64, 300, 1000, 129, 700, 1500, 256, 90]
 
@@ -87,7 +87,8 @@ def test_reuse_skips_factorization():
         X2 = B2.clone()
         r2 = S.wait(S.submit(5, A, X2, reuse=True))
         st = S.stats()
-    assert r1.factored == 1 and r2.factored == 0
+    assert r1.status == 0 and r2.status == 0, (result_dict(r1), result_dict(r2))
+    assert r1.factored == 1 and r2.factored == 0, (result_dict(r1), result_dict(r2))
     assert st.reuse_hits == 1 and st.resident_factors == 1
     assert eta(A, X2, B2) <= 1e-14
     assert r2.t_done - r2.t_start < r1.t_done - r1.t_start
