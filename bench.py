@@ -88,44 +88,71 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
 
+UPDATE_DEPTH = {"f64": 256, "f32": 512, "bf16x3": 512}  # columns per trailing pass (lapack.hpp)
+
+
+def trailing_launches(n, nb, trailing):
+    """(m, n_cols, k) of every trailing-update launch the blocked factorization
+    makes: per group of G = depth/nb block columns, the in-group updates of
+    each block column, the next group's block columns and the rest."""
+    G = max(1, UPDATE_DEPTH[trailing] // nb)
+    gw = min(G * nb, n)
+    out = []
+    for c0 in range(0, n, gw):
+        c1 = min(n, c0 + gw)
+        for k in range(c0, c1, nb):
+            b = min(nb, n - k)
+            if k > c0:
+                out.append((n - k, b, k - c0))
+        m2 = n - c1
+        if m2 <= 0:
+            break
+        bn = min(gw, m2)
+        if m2 > bn:
+            out.append((m2 - bn, m2 - bn, c1 - c0))
+        out.append((m2, bn, c1 - c0))
+    return out
+
+
 def syrk_roofline(n, nb, trailing, stream):
-    """Time the trailing-update kernel at every step size of the blocked
-    factorization (the same launches the factorization makes), with HIP
-    events on the launching stream.  Returns (avg_ms, flops_per_launch)."""
+    """Replay the trailing-update launches of the blocked factorization that
+    update the rest of the trailing matrix (same shapes, same kernel; they
+    carry all but a few percent of the factorization's flops) on `stream`
+    between HIP events.  Returns (average ms per launch, algorithmic flops
+    per launch, launches)."""
     import torch
 
     from elemental_amd import lib
 
-    steps = [(n - (j + 1) * nb, min(nb, n - j * nb)) for j in range((n + nb - 1) // nb)]
-    steps = [(m, b) for m, b in steps if m > 0]
-    if trailing == "f64":
-        P = torch.randn(nb, n, dtype=torch.float64, device="cuda")
-        C = torch.zeros(n, n, dtype=torch.float64, device="cuda")
-    else:
-        P = torch.randn(nb, n, dtype=torch.float32, device="cuda")
-        C = torch.zeros(n, n, dtype=torch.float32, device="cuda")
+    shapes = trailing_launches(n, nb, trailing)
+    kmax = max(k for _, _, k in shapes)
+    dt = torch.float64 if trailing == "f64" else torch.float32
+    P = torch.randn(kmax, n, dtype=dt, device="cuda")  # column-major n x kmax panel, ld n
+    C = torch.zeros(n, n, dtype=dt, device="cuda")
     mode = {"f32": 1, "bf16x3": 2}.get(trailing)
     s = ctypes.c_void_p(stream.cuda_stream)
 
-    def launch(m, b):
+    def launch(m, k):
         if trailing == "f64":
-            return lib().elemamd_herk_f64(b"L", b"N", m, b, -1.0, ctypes.c_void_p(P.data_ptr()), n, 1.0,
+            return lib().elemamd_herk_f64(b"L", b"N", m, k, -1.0, ctypes.c_void_p(P.data_ptr()), n, 1.0,
                                           ctypes.c_void_p(C.data_ptr()), n, s)
-        return lib().elemamd_herk_f32(mode, m, b, -1.0, ctypes.c_void_p(P.data_ptr()), n, 1.0,
+        return lib().elemamd_herk_f32(mode, m, k, -1.0, ctypes.c_void_p(P.data_ptr()), n, 1.0,
                                       ctypes.c_void_p(C.data_ptr()), n, s)
 
-    for m, b in steps[:3]:
-        assert launch(m, b) == 0
+    # the square launches (rest of the trailing matrix) carry ~all the flops
+    rest = [(m, nc, k) for m, nc, k in shapes if m == nc]
+    for m, _, k in rest[:2]:
+        assert launch(m, k) == 0
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
-    for m, b in steps:
-        assert launch(m, b) == 0
+    for m, _, k in rest:
+        assert launch(m, k) == 0
     ev1.record(stream)
     torch.cuda.synchronize()
     total_ms = ev0.elapsed_time(ev1)
-    flops = sum(m * (m + 1) * b for m, b in steps)  # lower triangle: m(m+1)/2 entries x 2b
-    return total_ms / len(steps), flops / len(steps), len(steps)
+    flops = sum(m * (m + 1) * k for m, _, k in rest)  # lower triangle: m(m+1)/2 entries x 2k
+    return total_ms / len(rest), flops / len(rest), len(rest)
 
 
 def cpu_baseline(A_host, B_host, runs):

@@ -101,7 +101,7 @@ SIGNATURES = {
     "elemamd_batch_results": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "elemamd_batch_backward_error": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "elemamd_server_create": (_i, [ctypes.POINTER(ServerConfig), _vp]),
-    "elemamd_server_submit": (_i, [_vp, _i, _i, _i, _i, _vp, _i64, _vp, _i64, _vp]),
+    "elemamd_server_submit": (_i, [_vp, _i, _i, _i, _i, _vp, _i64, _vp, _i64, _vp, _vp]),
     "elemamd_server_wait": (_i, [_vp, _i64, ctypes.POINTER(ServeResult)]),
     "elemamd_server_evict": (_i, [_vp, _i]),
     "elemamd_server_stats": (_i, [_vp, ctypes.POINTER(ServerStats)]),

@@ -282,9 +282,9 @@ int elemamd_server_create(const elemamd_server_config* cfg, elemamd_server** ser
 }
 
 int elemamd_server_submit(elemamd_server* server, int key, int reuse, int n, int k, const double* A, int64_t lda,
-                          double* B, int64_t ldb, int64_t* ticket) {
+                          double* B, int64_t ldb, void* ready_stream, int64_t* ticket) {
     if (server == nullptr) return ELEMAMD_ERR_ARG;
-    ServeRequest q{key, reuse, n, k, A, lda, B, ldb};
+    ServeRequest q{key, reuse, n, k, A, lda, B, ldb, S(ready_stream)};
     return server->s->submit(q, ticket);
 }
 

@@ -61,6 +61,7 @@ struct Server::Factor {
 struct Server::Req {
     ServeRequest q;
     ServeResult res{};
+    hipEvent_t ready = nullptr;  // recorded on the submitter's stream
     bool done = false;
 };
 
@@ -121,6 +122,13 @@ int Server::submit(const ServeRequest& q, int64_t* ticket) {
     auto r = std::make_shared<Req>();
     r->q = q;
     r->res.t_submit = now_s(t0_);
+    {  // order the request after the producer of A and B (null: the default stream)
+        if (hipEventCreateWithFlags(&r->ready, hipEventDisableTiming) != hipSuccess) return kErrHip;
+        if (hipEventRecord(r->ready, q.ready) != hipSuccess) {
+            (void)hipEventDestroy(r->ready);
+            return kErrHip;
+        }
+    }
     {
         std::lock_guard<std::mutex> g(mu_);
         r->res.ticket = next_ticket_++;
@@ -260,6 +268,7 @@ void Server::worker(int slot) {
         res.slot = slot;
         res.factored = build ? 1 : 0;
         int rc = s ? kOk : kErrHip;
+        if (rc == kOk && r->ready != nullptr && hipStreamWaitEvent(s, r->ready, 0) != hipSuccess) rc = kErrHip;
         const size_t wbytes = work_bytes(q.n, q.k);
         if (rc == kOk && wbytes > work_cap) {  // grow-only per-slot workspace, stream ordered
             if (work) (void)hipFreeAsync(work, s);
@@ -303,6 +312,7 @@ void Server::worker(int slot) {
             res.backward_error = rep.backward_error;
         }
         (void)hipStreamSynchronize(s);
+        if (r->ready != nullptr) (void)hipEventDestroy(r->ready);
         res.status = rc;
         res.t_done = now_s(t0_);
         {

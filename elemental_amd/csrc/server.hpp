@@ -31,6 +31,7 @@ struct ServeRequest {
     i64 lda;
     double* B;        // device, n x k, overwritten with X
     i64 ldb;
+    hipStream_t ready;  // the inputs are ready once this stream's work so far completes (null: default stream)
 };
 
 struct ServeResult {

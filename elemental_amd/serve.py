@@ -36,9 +36,13 @@ class Server:
         n, k = B.shape
         if A.stride(0) != 1 or B.stride(0) != 1:
             raise ValueError("A and B must be column-major (unit row stride)")
+        import torch
+
         t = ctypes.c_int64()
+        ready = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)  # A and B were produced on it
         rc = lib().elemamd_server_submit(self._h, key, 1 if reuse else 0, n, k, ctypes.c_void_p(A.data_ptr()),
-                                         A.stride(1), ctypes.c_void_p(B.data_ptr()), B.stride(1), ctypes.byref(t))
+                                         A.stride(1), ctypes.c_void_p(B.data_ptr()), B.stride(1), ready,
+                                         ctypes.byref(t))
         if rc != 0:
             raise RuntimeError(f"elemamd_server_submit failed ({rc})")
         self._keep[t.value] = (A, B)

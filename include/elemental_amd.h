@@ -253,10 +253,12 @@ int elemamd_server_create(const elemamd_server_config *cfg, elemamd_server **ser
 /* Queue a request: solve A X = B (A n x n device, lower triangle read and
  * never modified; B n x k device, overwritten with X).  The factor is kept
  * resident under `key`; with reuse = 1 a resident factor of `key` is used
- * instead of factoring again.  A and B must stay valid until the request is
- * waited for.  Thread-safe; returns at once with a ticket. */
+ * instead of factoring again.  The request starts after the work enqueued so
+ * far on `ready_stream` (the producer of A and B; NULL = the default
+ * stream); the server runs on streams of its own.  A and B must stay valid until the
+ * request is waited for.  Thread-safe; returns at once with a ticket. */
 int elemamd_server_submit(elemamd_server *server, int key, int reuse, int n, int k, const double *A, int64_t lda,
-                          double *B, int64_t ldb, int64_t *ticket);
+                          double *B, int64_t ldb, void *ready_stream, int64_t *ticket);
 /* Block until the ticket's request completes; fills *result (may be NULL). */
 int elemamd_server_wait(elemamd_server *server, int64_t ticket, elemamd_serve_result *result);
 /* Release the resident factor of `key` (an error if a request is using it). */
