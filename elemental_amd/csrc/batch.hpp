@@ -72,7 +72,7 @@ struct BatchPlan {
     // captured graph and the arguments it was captured with
     hipGraphExec_t graph_exec = nullptr;
     const void *g_A = nullptr, *g_B = nullptr, *g_work = nullptr;
-    hipStream_t g_stream = nullptr;
+    hipStream_t capture_stream = nullptr;
     ~BatchPlan();
 };
 

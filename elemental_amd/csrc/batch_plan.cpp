@@ -39,6 +39,7 @@ BatchArgs bucket_args(const BatchPlan* P, const BatchBucket& b, bool with_skip) 
 
 BatchPlan::~BatchPlan() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
     for (void* p : dev_allocs) (void)hipFree(p);
 }
 
@@ -239,15 +240,20 @@ int batch_hpd_solve(BatchPlan* P, double* A, double* B, void* work, size_t work_
     if (P == nullptr || (P->count > 0 && (A == nullptr || B == nullptr))) return kErrArg;
     if (work_bytes < batch_workspace_bytes(P) || (P->precision != 0 && work == nullptr)) return kErrWorkspace;
     if (!use_graph) return enqueue(P, A, B, work, s);
-    if (P->graph_exec == nullptr || P->g_A != A || P->g_B != B || P->g_work != work || P->g_stream != s) {
+    if (P->graph_exec == nullptr || P->g_A != A || P->g_B != B || P->g_work != work) {
         if (P->graph_exec) {
             ELEMAMD_HIP_TRY(hipGraphExecDestroy(P->graph_exec));
             P->graph_exec = nullptr;
         }
+        // capture on a private stream (the caller's may be the default
+        // stream, which cannot be captured); the graph is launched on `s`
+        if (P->capture_stream == nullptr)
+            ELEMAMD_HIP_TRY(hipStreamCreateWithFlags(&P->capture_stream, hipStreamNonBlocking));
+        hipStream_t cs = P->capture_stream;
         hipGraph_t g = nullptr;
-        ELEMAMD_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue(P, A, B, work, s);
-        const hipError_t e = hipStreamEndCapture(s, &g);
+        ELEMAMD_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue(P, A, B, work, cs);
+        const hipError_t e = hipStreamEndCapture(cs, &g);
         if (rc != kOk) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
@@ -262,7 +268,6 @@ int batch_hpd_solve(BatchPlan* P, double* A, double* B, void* work, size_t work_
         P->g_A = A;
         P->g_B = B;
         P->g_work = work;
-        P->g_stream = s;
     }
     ELEMAMD_HIP_TRY(hipGraphLaunch(P->graph_exec, s));
     return kOk;

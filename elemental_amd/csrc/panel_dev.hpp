@@ -122,6 +122,7 @@ __device__ __forceinline__ void trsm_rlt_blocked_rows(int m, int nb, const T* __
     __shared__ double sX[kTrsmRowsPerBlock][kPanelLd];  // B, then X, of the 64-row slab
     __shared__ double sLr[kPanelSub][kPanelLd];         // block row c of L (columns 0 .. 32c+31)
     __shared__ double sRinv[kPanelSub];
+    __shared__ double sLd[kPanelSub][kPanelSub];  // diagonal block, transposed
     if (info != nullptr && *info != 0 && *info <= limit) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const i64 r0 = (i64)row_block * kTrsmRowsPerBlock;
@@ -203,14 +204,30 @@ __device__ __forceinline__ void trsm_rlt_blocked_rows(int m, int nb, const T* __
             __syncthreads();
         }
         if (w == 0) {
+            // diagonal block transposed, sLd[k][j] = L(c0+j, c0+k): the column a
+            // step needs is contiguous, read as broadcast 16-byte loads
+            if (lane < kPanelSub) {
+#pragma unroll
+                for (int k = 0; k < kPanelSub; ++k) sLd[k][lane] = sLr[lane][c0 + k];
+            }
             double b[kPanelSub];
 #pragma unroll
             for (int j = 0; j < kPanelSub; ++j) b[j] = sX[lane][c0 + j];
+            // right-looking sweep; column k+1 is fetched while column k is applied
+            double lk[kPanelSub], ln[kPanelSub];
+#pragma unroll
+            for (int j = 0; j < kPanelSub; ++j) lk[j] = sLd[0][j];
 #pragma unroll
             for (int k = 0; k < kPanelSub; ++k) {
+                if (k + 1 < kPanelSub) {
+#pragma unroll
+                    for (int j = k + 2; j < kPanelSub; ++j) ln[j] = sLd[k + 1][j];
+                }
                 b[k] = b[k] * sRinv[k];
 #pragma unroll
-                for (int j = k + 1; j < kPanelSub; ++j) b[j] = b[j] - sLr[j][c0 + k] * b[k];
+                for (int j = k + 1; j < kPanelSub; ++j) b[j] = b[j] - lk[j] * b[k];
+#pragma unroll
+                for (int j = k + 2; j < kPanelSub; ++j) lk[j] = ln[j];
             }
 #pragma unroll
             for (int j = 0; j < kPanelSub; ++j) sX[lane][c0 + j] = b[j];

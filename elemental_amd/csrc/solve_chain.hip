@@ -33,6 +33,7 @@
 
 namespace elemamd {
 
+// Band width D of the chain: its LDS holds D tiles of L plus D solved blocks.
 template <typename T>
 struct ChainDepth {
     static constexpr int value = sizeof(T) == 8 ? 3 : 4;
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
                                                          const int* __restrict__ info) {
     constexpr int D = ChainDepth<T>::value;
     typedef typename Acc4<T>::type acc_t;
-    __shared__ T sM[2][kSB][kSLd];      // sM[q][p]: contraction index q, output row p
+    __shared__ T sM[D][kSB][kSLd];      // [q][p] tiles: chain band (then diagonal in sM[0]); helpers: double buffer
     __shared__ T sRing[D][kSB][kSC + 1];  // chain: last D solved blocks [q][c]; helpers: Y double buffer
     __shared__ T sR[kSB][kSC + 1];
     __shared__ T sRinv[kSB];
@@ -182,16 +183,27 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
     }
 
     // ----------------------------------------------------------------- chain
+    // Every operand of step pos that does not depend on the chain itself (the
+    // D band tiles of L, the diagonal operand, the own right-hand sides) is
+    // loaded into registers during step pos-1, so the chain never waits for
+    // L from memory; only the helper's partial sum is fetched in the step.
     const int cy = ord;
     const int c_base = cy * kSC, kcl = min(kSC, kc - c_base);
-    int buf = 0;
-    if (nT > 1) gload(blk(1), blk(0));  // first band tile of position 1
-    for (int pos = 0; pos < nT; ++pos) {
+    T gt[D][16], gd[16], gb[4];
+    auto prefetch = [&](int pos) {
         const int I = blk(pos);
         const i64 r0 = (i64)I * kSB;
         const int bs = (int)min<i64>(kSB, n - r0);
-        // own right-hand sides (MFMA output order) and diagonal operand: no dependences
-        T gb[4], gd[16];
+        const int qlo = max(0, pos - D);
+#pragma unroll
+        for (int t2 = 0; t2 < D; ++t2) {
+            const int q = qlo + t2;
+            if (q < pos) {  // uniform
+                gload(I, blk(q));
+#pragma unroll
+                for (int s2 = 0; s2 < 16; ++s2) gt[t2][s2] = gm[s2];
+            }
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = mfma_row<T>(lane, v);
@@ -201,16 +213,16 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
         if constexpr (USE_INV) {
             const T* Di = Dinv + (i64)I * kSB * kSB;
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int qq = w + 4 * s;
-                gd[s] = !TRANS ? Di[lane + qq * kSB] : Di[qq + lane * kSB];  // M[q][p] = Dinv(p,q) / Dinv(q,p)
+            for (int s2 = 0; s2 < 16; ++s2) {
+                const int qq = w + 4 * s2;
+                gd[s2] = !TRANS ? Di[lane + qq * kSB] : Di[qq + lane * kSB];  // M[q][p] = Dinv(p,q) / Dinv(q,p)
             }
         } else {
             // lower-triangular M(i, k) at [k][i], identity on padded rows; the
             // backward solve is the same sweep on reversed indices
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const int kk = w + 4 * s;
+            for (int s2 = 0; s2 < 16; ++s2) {
+                const int kk = w + 4 * s2;
                 const int gi = TRANS ? (kSB - 1 - kk) : lane;
                 const int gk = TRANS ? (kSB - 1 - lane) : kk;
                 T v = T(0);
@@ -219,36 +231,40 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
                 } else if (lane == kk) {
                     v = T(1);
                 }
-                gd[s] = v;
+                gd[s2] = v;
             }
         }
-        // band: the last D solved blocks, from LDS
-        acc_t acc = {};
-        const int qlo = max(0, pos - D);
-        for (int q = qlo; q < pos; ++q) {
-            sstore(buf);  // tile (pos, q): its loads were issued after the stores of Y_{pos-1}
-            if (q == qlo) publish_flag(&flagsX[(pos - 1) * CY + cy], t);  // pos >= 1 here
-            // next tile of the stream (clamped reload at the very end keeps the body branch-free)
-            int np = pos, nq = q + 1;
-            if (nq >= pos) {
-                np = pos + 1;
-                nq = max(0, np - D);
-            }
-            if (np >= nT) {
-                np = pos;
-                nq = q;
-            }
-            gload(blk(np), blk(nq));
-            __syncthreads();
+    };
+    auto stage = [&](int slot, const T (&g)[16]) {  // a band tile into sM[slot] ([q][p] layout)
+        if (!TRANS) {
 #pragma unroll
-            for (int ks = 0; ks < kSB / 4; ++ks) {
-                const int qq = 4 * ks + (lane >> 4);
-                acc = mfma16(sRing[q % D][qq][lane & 15], sM[buf][qq][16 * w + (lane & 15)], acc);
-            }
-            buf ^= 1;
+            for (int s2 = 0; s2 < 16; ++s2) sM[slot][w + 4 * s2][lane] = g[s2];
+        } else {
+#pragma unroll
+            for (int s2 = 0; s2 < 16; ++s2) sM[slot][lane][w + 4 * s2] = g[s2];
         }
-        acc = mfma16(T(0), T(0), acc);
-        // partial sum of the older blocks, from the helper
+    };
+
+    prefetch(0);
+    for (int pos = 0; pos < nT; ++pos) {
+        const int I = blk(pos);
+        const i64 r0 = (i64)I * kSB;
+        const int qlo = max(0, pos - D);
+        const int nbt = pos - qlo;  // band tiles of this step
+        // Y_{pos-1} is stored (coherently) and no other load is outstanding
+        // except the prefetch: publish it, which also orders the LDS reuse below
+        if (pos > 0) publish_flag(&flagsX[(pos - 1) * CY + cy], t);
+#pragma unroll
+        for (int t2 = 0; t2 < D; ++t2)
+            if (t2 < nbt) stage(t2, gt[t2]);
+        T gdc[16], gbc[4];
+#pragma unroll
+        for (int s2 = 0; s2 < 16; ++s2) gdc[s2] = gd[s2];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) gbc[v] = gb[v];
+        __syncthreads();
+        if (pos + 1 < nT) prefetch(pos + 1);
+        // partial sum of the older blocks (helper), fetched while the band runs
         T sp[4] = {T(0), T(0), T(0), T(0)};
         if (pos >= D + 1) {
             wait_one(&flagsP[pos * CY + cy]);
@@ -259,11 +275,24 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
                 sp[v] = (row < n && c < kcl) ? coherent_load(&part[row + (i64)(c_base + c) * n]) : T(0);
             }
         }
+        // band: the last D solved blocks, all operands in LDS
+        acc_t acc = {};
+        for (int t2 = 0; t2 < nbt; ++t2) {
+            const int q = qlo + t2;
+#pragma unroll
+            for (int ks = 0; ks < kSB / 4; ++ks) {
+                const int qq = 4 * ks + (lane >> 4);
+                acc = mfma16(sRing[q % D][qq][lane & 15], sM[t2][qq][16 * w + (lane & 15)], acc);
+            }
+        }
+        // dependent zero-operand MFMA before the accumulator is read (the
+        // compiler's wait states after a loop back edge are short for FP64 MFMA)
+        acc = mfma16(T(0), T(0), acc);
         __syncthreads();  // band products are done with sM and sRing
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sR[16 * w + (lane & 15)][mfma_row<T>(lane, v)] = gb[v] - acc[v] - sp[v];
+        for (int v = 0; v < 4; ++v) sR[16 * w + (lane & 15)][mfma_row<T>(lane, v)] = gbc[v] - acc[v] - sp[v];
 #pragma unroll
-        for (int s = 0; s < 16; ++s) sM[buf][w + 4 * s][lane] = gd[s];
+        for (int s2 = 0; s2 < 16; ++s2) sM[0][w + 4 * s2][lane] = gdc[s2];
         __syncthreads();
         const int slot = pos % D;
         if constexpr (USE_INV) {
@@ -271,7 +300,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
 #pragma unroll
             for (int ks = 0; ks < kSB / 4; ++ks) {
                 const int qq = 4 * ks + (lane >> 4);
-                y = mfma16(sR[qq][lane & 15], sM[buf][qq][16 * w + (lane & 15)], y);
+                y = mfma16(sR[qq][lane & 15], sM[0][qq][16 * w + (lane & 15)], y);
             }
             y = mfma16(T(0), T(0), y);
 #pragma unroll
@@ -283,7 +312,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
                 if (row < n && c < kcl) coherent_store(&B[row + (i64)(c_base + c) * ldb], y[v]);
             }
         } else {
-            if (t < kSB) sRinv[t] = T(1) / sM[buf][t][t];
+            if (t < kSB) sRinv[t] = T(1) / sM[0][t][t];
             __syncthreads();
             const int myrow = TRANS ? (kSB - 1 - lane) : lane;
             T v[4];
@@ -294,7 +323,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
 #pragma unroll 16
             for (int kk = 0; kk < kSB; ++kk) {
                 const T rk = sRinv[kk];
-                const T lik = sM[buf][kk][lane];
+                const T lik = sM[0][kk][lane];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const T y = wave_bcast(v[u], kk) * rk;
@@ -310,7 +339,6 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
                 if (row < n && c < kcl) coherent_store(&B[row + (i64)(c_base + c) * ldb], v[u]);
             }
         }
-        buf ^= 1;  // the next band tile goes to the buffer the diagonal did not use
     }
     publish_flag(&flagsX[(nT - 1) * CY + cy], t);
 }

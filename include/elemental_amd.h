@@ -198,8 +198,8 @@ int elemamd_batch_set_refinement(elemamd_batch_plan *plan, int sweeps, double to
 size_t elemamd_batch_workspace(const elemamd_batch_plan *plan);
 /* Factor and solve every matrix; B_i is overwritten with X_i.  With
  * use_graph = 1 the launch sequence is captured in a hipGraph on the first
- * call (and again whenever A, B, work or stream change) and replayed as one
- * submission.  Asynchronous on `stream`. */
+ * call (on a private stream; again whenever A, B or work change) and
+ * launched on `stream` as one submission.  Asynchronous on `stream`. */
 int elemamd_batch_hpd_solve(elemamd_batch_plan *plan, double *A, double *B, void *work, size_t work_bytes,
                             int use_graph, void *stream);
 /* Synchronize and fetch per-matrix results in the caller's order: factor
