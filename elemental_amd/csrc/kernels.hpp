@@ -37,6 +37,15 @@ hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, con
                               i64 ldb, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
                               hipStream_t s);
 
+// Split-BF16x3 from pre-split panels: Hi/Lo bf16 planes of the panel
+// (row-major, row stride ldk, a multiple of 32), made by launch_split_bf16
+// from a float m x k panel; rows of the planes = rows (and columns) of C.
+hipError_t launch_split_bf16(int m, int k, const float* P, i64 ldp, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+                             hipStream_t s);
+hipError_t launch_syrk_ln_bf16x3_planes(int m, int n, int k, float alpha, const unsigned short* Hi,
+                                        const unsigned short* Lo, i64 ldk, float beta, float* C, i64 ldc,
+                                        const int* info, int kcol0, int nbp, hipStream_t s);
+
 // gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
                               double beta, double* C, i64 ldc, bool lower_only, hipStream_t s);

@@ -64,8 +64,20 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     const int G = std::max(1, update_depth<T>(trailing) / nb);
     const i64 gw64 = (i64)G * nb;
     const int gw = (int)std::min<i64>(gw64, n);
+    // split-BF16x3: each pass's panel is split into bf16 hi/lo planes once
+    // (two buffers, alternating by group: the rest-update of a group still
+    // reads its planes while the next group's are written)
+    const bool planes = sizeof(T) == 4 && trailing == kTrailingBf16x3;
+    const i64 ldk = ((i64)gw + 31) / 32 * 32;
+    const size_t plane_elems = (size_t)n * ldk;
+    unsigned short* pbuf = nullptr;
+    if (planes) {
+        pbuf = (unsigned short*)stream_scratch(4 * plane_elems * sizeof(unsigned short), s);
+        if (pbuf == nullptr) return kErrHip;
+    }
+    int group = 0;
     bool rest_pending = false;
-    for (int c0 = 0; c0 < n; c0 += gw) {
+    for (int c0 = 0; c0 < n; c0 += gw, ++group) {
         const int c1 = (int)std::min<i64>(n, (i64)c0 + gw);
         // the group's block columns, in order
         for (int k = c0; k < c1; k += nb) {
@@ -85,17 +97,33 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         T* C = A + c1 + (i64)c1 * lda;        // trailing matrix, m2 x m2
         const int bn = std::min(gw, m2);      // next group's block columns
         if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
+        unsigned short *Hi = nullptr, *Lo = nullptr;
+        if (planes) {
+            Hi = pbuf + (size_t)(group & 1) * 2 * plane_elems;
+            Lo = Hi + plane_elems;
+            ELEMAMD_HIP_TRY(launch_split_bf16(m2, kg, (const float*)P, lda, Hi, Lo, ldk, s));
+        }
         const bool rest = m2 > bn;
         if (rest) {  // fork: the rest of the trailing matrix on aux
             ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
             ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
-            ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2 - bn, m2 - bn, kg, P + bn, lda, C + bn + (i64)bn * lda, lda,
-                                           dinfo, c0, nb, sp->aux));
+            if (planes)
+                ELEMAMD_HIP_TRY(launch_syrk_ln_bf16x3_planes(m2 - bn, m2 - bn, kg, -1.0f, Hi + (i64)bn * ldk,
+                                                             Lo + (i64)bn * ldk, ldk, 1.0f,
+                                                             (float*)(C + bn + (i64)bn * lda), lda, dinfo, c0, nb,
+                                                             sp->aux));
+            else
+                ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2 - bn, m2 - bn, kg, P + bn, lda, C + bn + (i64)bn * lda,
+                                               lda, dinfo, c0, nb, sp->aux));
             ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_b, sp->aux));
         }
         rest_pending = rest;
         // critical chain on the caller's stream: the next group's block columns
-        ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2, bn, kg, P, lda, C, lda, dinfo, c0, nb, s));
+        if (planes)
+            ELEMAMD_HIP_TRY(launch_syrk_ln_bf16x3_planes(m2, bn, kg, -1.0f, Hi, Lo, ldk, 1.0f, (float*)C, lda, dinfo,
+                                                         c0, nb, s));
+        else
+            ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2, bn, kg, P, lda, C, lda, dinfo, c0, nb, s));
     }
     if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
     return kOk;

@@ -42,7 +42,7 @@ __device__ inline int syrk_valid_k(const int* info, int k, int kcol0, int nbp) {
     return lim < 0 ? 0 : (lim < k ? lim : k);
 }
 
-__global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
+__global__ __launch_bounds__(256, 1) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
         double* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
@@ -95,6 +95,55 @@ hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, con
     else
         hipLaunchKernelGGL(syrk_ln_f32_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
                            ldc, info, kcol0, nbp);
+    return hipGetLastError();
+}
+
+
+// Hi/Lo bf16 planes (row-major, k contiguous, ldk a multiple of 32, zero
+// padded) of a float panel P (m x k, column-major): thread = (row, 8 k).
+__global__ __launch_bounds__(256) void split_bf16_kernel(int m, int k, const float* __restrict__ P, i64 ldp,
+                                                         unsigned short* __restrict__ Hi,
+                                                         unsigned short* __restrict__ Lo, i64 ldk) {
+    const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int k0 = blockIdx.y * 8;
+    if (r >= m) return;
+    s8_t h, l;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int kk = k0 + q;
+        const float x = kk < k ? P[r + (i64)kk * ldp] : 0.0f;
+        const unsigned short hb = f32_to_bf16_rne(x);
+        h[q] = (short)hb;
+        l[q] = (short)f32_to_bf16_rne(x - bf16_to_f32(hb));
+    }
+    *reinterpret_cast<s8_t*>(Hi + r * ldk + k0) = h;
+    *reinterpret_cast<s8_t*>(Lo + r * ldk + k0) = l;
+}
+
+__global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3p_kernel(int m, int n, int k, float alpha,
+        const unsigned short* __restrict__ Hi, const unsigned short* __restrict__ Lo, i64 ldk, float beta,
+        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
+    const int ke = syrk_valid_k(info, k, kcol0, nbp);
+    if (info != nullptr && ke == 0) return;
+    int ti, tj;
+    if (!syrk_tile(m, n, ti, tj)) return;
+    syrk_tile_bf16x3_planes(m, n, ke, alpha, Hi, Lo, ldk, Hi, Lo, beta, C, ldc, ti, tj);
+}
+
+hipError_t launch_split_bf16(int m, int k, const float* P, i64 ldp, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+                             hipStream_t s) {
+    if (m <= 0 || k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)(ldk / 8)), dim3(256), 0, s, m,
+                       k, P, ldp, Hi, Lo, ldk);
+    return hipGetLastError();
+}
+
+hipError_t launch_syrk_ln_bf16x3_planes(int m, int n, int k, float alpha, const unsigned short* Hi,
+                                        const unsigned short* Lo, i64 ldk, float beta, float* C, i64 ldc,
+                                        const int* info, int kcol0, int nbp, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(syrk_ln_bf16x3p_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, Hi, Lo, ldk, beta, C,
+                       ldc, info, kcol0, nbp);
     return hipGetLastError();
 }
 

@@ -33,16 +33,24 @@ __device__ inline bool syrk_tile(int m, int n, int& ti, int& tj) {
 }
 
 // ---------------------------------------------------------------- FP64
+// K chunks of 32 (one workgroup per CU: 147 KB of LDS), so each chunk is 128
+// MFMAs per wave and the next chunk's loads have that long to arrive.  With
+// beta == 1 (the factorization) the accumulators start from C, read at the
+// start under the first chunk's loads, and the tile is only stored at the end.
+constexpr int kSyrkKC64 = 32;
+
 __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
         double* __restrict__ C, i64 ldc, int ti, int tj) {
-    __shared__ double sI[2][kSyrkKC][kSyrkLdp];
-    __shared__ double sJ[2][kSyrkKC][kSyrkLdp];
+    constexpr int KC = kSyrkKC64;
+    __shared__ double sI[2][KC][kSyrkLdp];
+    __shared__ double sJ[2][KC][kSyrkLdp];
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64;  // MFMA M offset = C column offset
     const int wi = (w & 1) * 64;   // MFMA N offset = C row offset
     const bool skip = (ti == tj) && (wj > wi);
+    const bool acc_c = (beta == 1.0);
 
     // global -> register staging: thread owns tile row lr, k rows lk + 2q
     const int lr = tid & 127;
@@ -50,11 +58,11 @@ __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
     const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
     const double* pI = A + (i64)(i0 + lr);
     const double* pJ = B + (i64)(j0 + lr);
-    double gI[8], gJ[8];
+    double gI[KC / 2], gJ[KC / 2];
 
     auto gload = [&](int kc0) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < KC / 2; ++q) {
             const int kk = kc0 + lk + 2 * q;
             const bool kin = kk < k;
             gI[q] = (rowI && kin) ? pI[(i64)kk * lda] : 0.0;
@@ -63,33 +71,47 @@ __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < KC / 2; ++q) {
             sI[buf][lk + 2 * q][lr] = gI[q];
             sJ[buf][lk + 2 * q][lr] = gJ[q];
         }
     };
 
     d4_t acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
-
     gload(0);
+    if (acc_c && !skip) {
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int i = i0 + wi + 16 * nb + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int j = j0 + wj + 16 * mb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
+                    acc[mb][nb][v] = (i < m && j < n && j <= i) ? C[(i64)i + (i64)j * ldc] : 0.0;
+                }
+            }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
+    }
+    const double ascale = acc_c ? alpha : 1.0;
     sstore(0);
     __syncthreads();
-    const int nchunks = (int)ceil_div(k, kSyrkKC);
+    const int nchunks = (int)ceil_div(k, KC);
     for (int c = 0; c < nchunks; ++c) {
         const int buf = c & 1;
         const bool more = (c + 1) < nchunks;
-        if (more) gload((c + 1) * kSyrkKC);
+        if (more) gload((c + 1) * KC);
         if (!skip) {
 #pragma unroll
-            for (int ks = 0; ks < kSyrkKC / 4; ++ks) {
+            for (int ks = 0; ks < KC / 4; ++ks) {
                 const int kr = ks * 4 + (lane >> 4);
                 double fa[4], fb[4];
 #pragma unroll
-                for (int mb = 0; mb < 4; ++mb) fa[mb] = sJ[buf][kr][wj + 16 * mb + (lane & 15)];
+                for (int mb = 0; mb < 4; ++mb) fa[mb] = ascale * sJ[buf][kr][wj + 16 * mb + (lane & 15)];
 #pragma unroll
                 for (int nb = 0; nb < 4; ++nb) fb[nb] = sI[buf][kr][wi + 16 * nb + (lane & 15)];
 #pragma unroll
@@ -103,6 +125,13 @@ __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
         __syncthreads();
     }
     if (skip) return;
+    // dependent zero-operand MFMAs before the accumulators are read (the
+    // compiler's wait states after a loop back edge are short for FP64 MFMA)
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+            acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, acc[mb][nb], 0, 0, 0);
 
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) {
@@ -114,8 +143,8 @@ __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
                 const int j = j0 + wj + 16 * mb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
                 if (i < m && j < n && j <= i) {
                     double* p = C + (i64)i + (i64)j * ldc;
-                    const double upd = alpha * acc[mb][nb][v];
-                    *p = (beta == 0.0) ? upd : beta * (*p) + upd;
+                    if (acc_c) *p = acc[mb][nb][v];
+                    else *p = (beta == 0.0) ? alpha * acc[mb][nb][v] : beta * (*p) + alpha * acc[mb][nb][v];
                 }
             }
         }
@@ -308,6 +337,135 @@ __device__ __forceinline__ void syrk_tile_bf16x3(int m, int n, int k, float alph
                     float* p = C + (i64)i + (i64)j * ldc;
                     const float upd = alpha * acc[mb][nb][v];
                     *p = (beta == 0.0f) ? upd : beta * (*p) + upd;
+                }
+            }
+        }
+}
+
+
+// Split-BF16x3 tile from pre-split panels: Hi/Lo hold bf16(x) and
+// bf16(x - bf16(x)) of the panel, row-major with ldk >= k (k contiguous, rows
+// of the panel = rows of C).  The split is done once per pass over the
+// trailing matrix instead of once per tile; the tile only moves 16-byte
+// groups of 8 bf16 from memory to LDS.  Same arithmetic as syrk_tile_bf16x3.
+__device__ __forceinline__ void syrk_tile_bf16x3_planes(int m, int n, int k, float alpha,
+        const unsigned short* __restrict__ Hi, const unsigned short* __restrict__ Lo, i64 ldk,
+        const unsigned short* __restrict__ HiJ, const unsigned short* __restrict__ LoJ, float beta,
+        float* __restrict__ C, i64 ldc, int ti, int tj) {
+    __shared__ __attribute__((aligned(16))) unsigned short sIh[kSyrkTile][kBfLd];
+    __shared__ __attribute__((aligned(16))) unsigned short sIl[kSyrkTile][kBfLd];
+    __shared__ __attribute__((aligned(16))) unsigned short sJh[kSyrkTile][kBfLd];
+    __shared__ __attribute__((aligned(16))) unsigned short sJl[kSyrkTile][kBfLd];
+    const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
+    const bool skip = (ti == tj) && (wj > wi);
+    const int lr = tid & 127, kh = tid >> 7;  // row, k half (16 values = 2 x 8)
+    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
+    const unsigned short* hI = Hi + (i64)(rowI ? i0 + lr : 0) * ldk;
+    const unsigned short* lI = Lo + (i64)(rowI ? i0 + lr : 0) * ldk;
+    const unsigned short* hJ = HiJ + (i64)(rowJ ? j0 + lr : 0) * ldk;
+    const unsigned short* lJ = LoJ + (i64)(rowJ ? j0 + lr : 0) * ldk;
+    s8_t g[8];  // I hi, I lo, J hi, J lo; two groups of 8 each
+    const s8_t zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto gload = [&](int kc0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int kk = kc0 + kh * 16 + 8 * h;
+            const bool kin = kk < k;  // ldk pads to a multiple of 32 with zeros
+            g[0 + h] = (rowI && kin) ? *reinterpret_cast<const s8_t*>(hI + kk) : zero;
+            g[2 + h] = (rowI && kin) ? *reinterpret_cast<const s8_t*>(lI + kk) : zero;
+            g[4 + h] = (rowJ && kin) ? *reinterpret_cast<const s8_t*>(hJ + kk) : zero;
+            g[6 + h] = (rowJ && kin) ? *reinterpret_cast<const s8_t*>(lJ + kk) : zero;
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            *reinterpret_cast<s8_t*>(&sIh[lr][kh * 16 + 8 * h]) = g[0 + h];
+            *reinterpret_cast<s8_t*>(&sIl[lr][kh * 16 + 8 * h]) = g[2 + h];
+            *reinterpret_cast<s8_t*>(&sJh[lr][kh * 16 + 8 * h]) = g[4 + h];
+            *reinterpret_cast<s8_t*>(&sJl[lr][kh * 16 + 8 * h]) = g[6 + h];
+        }
+    };
+    // C - P P^T (the factorization): accumulate onto C, read while the first
+    // chunk loads, with the J operands negated (sign bits flipped, exact)
+    const bool acc_c = (beta == 1.0f && alpha == -1.0f);
+    f4_t acc[4][4];
+    if (acc_c && !skip) {
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int i = i0 + wi + 16 * nb + (lane & 15);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int j = j0 + wj + 16 * mb + 4 * (lane >> 4) + v;
+                    acc[mb][nb][v] = (i < m && j < n && j <= i) ? C[(i64)i + (i64)j * ldc] : 0.0f;
+                }
+            }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
+    }
+    const int nchunks = (int)ceil_div(k, kBfKC);
+    gload(0);
+    for (int c = 0; c < nchunks; ++c) {
+        __syncthreads();  // previous chunk's LDS reads are done
+        sstore();
+        __syncthreads();
+        if (c + 1 < nchunks) gload((c + 1) * kBfKC);
+        if (!skip) {
+            const int kofs = 8 * (lane >> 4);
+            s8_t ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const int r = wj + 16 * mb + (lane & 15);
+                ah[mb] = *reinterpret_cast<const s8_t*>(&sJh[r][kofs]);
+                al[mb] = *reinterpret_cast<const s8_t*>(&sJl[r][kofs]);
+                if (acc_c) {  // flip the bf16 sign bits
+                    ah[mb] ^= (s8_t){(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
+                                     (short)0x8000, (short)0x8000, (short)0x8000};
+                    al[mb] ^= (s8_t){(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
+                                     (short)0x8000, (short)0x8000, (short)0x8000};
+                }
+            }
+#pragma unroll
+            for (int nb = 0; nb < 4; ++nb) {
+                const int r = wi + 16 * nb + (lane & 15);
+                bh[nb] = *reinterpret_cast<const s8_t*>(&sIh[r][kofs]);
+                bl[nb] = *reinterpret_cast<const s8_t*>(&sIl[r][kofs]);
+            }
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb) {
+                    f4_t a = acc[mb][nb];
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, al[mb]),
+                                                                __builtin_bit_cast(bf8_t, bh[nb]), a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, ah[mb]),
+                                                                __builtin_bit_cast(bf8_t, bl[nb]), a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, ah[mb]),
+                                                                __builtin_bit_cast(bf8_t, bh[nb]), a, 0, 0, 0);
+                    acc[mb][nb] = a;
+                }
+        }
+    }
+    if (skip) return;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb) {
+            const int i = i0 + wi + 16 * nb + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j = j0 + wj + 16 * mb + 4 * (lane >> 4) + v;
+                if (i < m && j < n && j <= i) {
+                    float* p = C + (i64)i + (i64)j * ldc;
+                    if (acc_c) *p = acc[mb][nb][v];
+                    else *p = (beta == 0.0f) ? alpha * acc[mb][nb][v] : beta * (*p) + alpha * acc[mb][nb][v];
                 }
             }
         }
