@@ -88,8 +88,20 @@ int elemamd_herk_f32(int mode, int n, int k, float alpha, const float* A, int64_
     if (mode != ELEMAMD_TRAILING_F32 && mode != ELEMAMD_TRAILING_BF16X3) return ELEMAMD_ERR_ARG;
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
+    hipStream_t s = S(stream);
+    if (mode == ELEMAMD_TRAILING_BF16X3 && k > 0) {
+        // the factorization's form: split A into bf16 hi/lo planes once, then the planes update
+        const i64 ldk = ((i64)k + 31) / 32 * 32;
+        auto* hi = (unsigned short*)stream_scratch(2 * (size_t)n * ldk * sizeof(unsigned short), s);
+        if (hi == nullptr) return ELEMAMD_ERR_HIP;
+        unsigned short* lo = hi + (size_t)n * ldk;
+        if (launch_split_bf16(n, k, A, lda, hi, lo, ldk, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+        return launch_syrk_ln_bf16x3_planes(n, n, k, alpha, hi, lo, ldk, beta, C, ldc, nullptr, 0, 1, s) == hipSuccess
+                   ? ELEMAMD_OK
+                   : ELEMAMD_ERR_HIP;
+    }
     return launch_syrk_ln_f32(mode == ELEMAMD_TRAILING_BF16X3, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr,
-                              0, 1, S(stream)) == hipSuccess
+                              0, 1, s) == hipSuccess
                ? ELEMAMD_OK
                : ELEMAMD_ERR_HIP;
 }

@@ -42,7 +42,7 @@ __device__ inline int syrk_valid_k(const int* info, int k, int kcol0, int nbp) {
     return lim < 0 ? 0 : (lim < k ? lim : k);
 }
 
-__global__ __launch_bounds__(256, 1) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
+__global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
         double* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
     const int ke = syrk_valid_k(info, k, kcol0, nbp);

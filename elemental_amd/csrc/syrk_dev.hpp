@@ -33,11 +33,11 @@ __device__ inline bool syrk_tile(int m, int n, int& ti, int& tj) {
 }
 
 // ---------------------------------------------------------------- FP64
-// K chunks of 32 (one workgroup per CU: 147 KB of LDS), so each chunk is 128
-// MFMAs per wave and the next chunk's loads have that long to arrive.  With
-// beta == 1 (the factorization) the accumulators start from C, read at the
-// start under the first chunk's loads, and the tile is only stored at the end.
-constexpr int kSyrkKC64 = 32;
+// K chunks of 16 (two workgroups per CU: 2 x 68 KB of LDS; chunks of 32 with
+// one workgroup per CU measured 10-15% slower).  With beta == 1 (the
+// factorization) the accumulators start from C, read at the start under the
+// first chunk's loads, and the tile is only stored at the end.
+constexpr int kSyrkKC64 = 16;
 
 __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,

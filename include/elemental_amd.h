@@ -82,7 +82,9 @@ int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const do
 
 /* Float-storage trailing update of the mixed-precision factor (no reference
  * counterpart): mode ELEMAMD_TRAILING_F32 (FP32 MFMA) or
- * ELEMAMD_TRAILING_BF16X3 (split-BF16x3, FP32 accumulation). */
+ * ELEMAMD_TRAILING_BF16X3 (split-BF16x3, FP32 accumulation: A is split into
+ * bf16 hi/lo planes in a per-stream scratch buffer, then the three products
+ * hi*hi + hi*lo + lo*hi accumulate in FP32). */
 int elemamd_herk_f32(int mode, int n, int k, float alpha, const float *A, int64_t lda, float beta, float *C,
                      int64_t ldc, void *stream);
 
