@@ -25,7 +25,7 @@ SOURCES = [
     "syrk.hip",
     "gemm.hip",
     "solve.hip",
-    "solve_chain.hip",
+    "solve_chain.hip", "solve_sweep.hip",
     "blas1.hip",
     "lapack.cpp",
     "batch.hip",

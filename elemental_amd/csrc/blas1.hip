@@ -58,7 +58,7 @@ hipError_t launch_fill_normal_f64(double* out, i64 rows, i64 cols, i64 ld, uint6
 __global__ void scale_f64_kernel(int m, int n, double alpha, double* A, i64 lda) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
     const int j = blockIdx.y;
-    if (i < m) A[i + j * lda] *= alpha;
+    if (i < m) A[i + j * lda] = (alpha == 0.0) ? 0.0 : A[i + j * lda] * alpha;  // alpha = 0 clears (NaN too)
 }
 hipError_t launch_scale_f64(int m, int n, double alpha, double* A, i64 lda, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;

@@ -67,6 +67,23 @@ template <typename T>
 hipError_t launch_trsm_persist(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
                                int* ctrl, const int* info, hipStream_t s);
 
+// solve_sweep.hip -- block-inverse sweeps for the float correction solves of
+// refinement.  Preparation (once per factor S, lower triangle = L): writes the
+// mirror of L into the strict upper triangle of S, the diagonal-block
+// inverses into Xp (sweep_xinv_count floats, scratch) and the sweep operands
+// into tiles (sweep_tiles_count floats).  Dinv: launch_trtri_diag64 output.
+size_t sweep_tiles_count(int n);
+size_t sweep_xinv_count(int n);
+size_t sweep_work_bytes(int n);
+hipError_t launch_sweep_prepare(int n, float* S, i64 lds, const float* Dinv, float* Xp, float* tiles, hipStream_t s);
+// B := inv(L^T) inv(L) B for B n x k (column-major); work: sweep_work_bytes(n).
+hipError_t launch_sweep_solve(int n, int k, const float* S, i64 lds, const float* tiles, float* B, i64 ldb, void* work,
+                              hipStream_t s);
+// The work's error word: set by a sweep whose bounded wait expired (its
+// result is then wrong), kept until cleared.
+hipError_t launch_sweep_clear_error(int n, void* work, hipStream_t s);
+hipError_t sweep_read_error(int n, void* work, int* out, hipStream_t s);
+
 // blas1.hip -- elementwise helpers, conversions, residual and norms.
 hipError_t launch_fill_int(int* p, int value, int count, hipStream_t s);
 hipError_t launch_fill_normal_f64(double* out, i64 rows, i64 cols, i64 ld, uint64_t seed, i64 offset, hipStream_t s);

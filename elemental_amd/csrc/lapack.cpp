@@ -214,7 +214,7 @@ struct MixedWork {
     double* V;     // (m+1) blocks of n x k: Krylov basis
     double* nrm;   // 8k + n + 8 doubles: norms, row sums, scratch
     double* h;     // (m+1) x k Gram-Schmidt coefficients
-    int* ctrl;     // persistent-solve tickets and flags
+    void* sweep;   // block vectors and flags of the correction sweeps
     double* part;  // partial sums of the split symmetric product
     size_t bytes;
 };
@@ -231,7 +231,7 @@ MixedWork carve_refine(void* work, size_t cap, int n, int k, int m) {
     w.V = c.take<double>(m > 0 ? nk * (size_t)(m + 1) : 0);
     w.nrm = c.take<double>((size_t)8 * k + n + 8);
     w.h = c.take<double>((size_t)(m + 2) * k);
-    w.ctrl = c.take<int>(solve_ctrl_count(n, k));
+    w.sweep = c.take<char>(sweep_work_bytes(n));
     w.part = c.take<double>(symm_part_count(n, k));
     w.bytes = c.off + 256;
     return w;
@@ -254,8 +254,7 @@ struct Ctx {
 int precond_apply(Ctx& c, const double* y, double* z, const double* colscale) {
     const int n = c.n, k = c.k;
     ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, y, n, colscale, c.w.Rf, n, c.s));
-    int rc = solve_apply<float>(true, true, n, k, c.f->S, n, c.w.Rf, n, c.f->Dinv, c.w.ctrl, nullptr, c.s);
-    if (rc) return rc;
+    ELEMAMD_HIP_TRY(launch_sweep_solve(n, k, c.f->S, n, c.f->tiles, c.w.Rf, n, c.w.sweep, c.s));
     ELEMAMD_HIP_TRY(launch_copy_f32_to_f64_scaled(n, k, c.w.Rf, n, colscale, z, n, c.s));
     return kOk;
 }
@@ -383,6 +382,8 @@ MixedFactor carve_factor(void* mem, int n) {
     MixedFactor f{};
     f.S = c.take<float>((size_t)n * n);
     f.Dinv = c.take<float>(solve_dinv_count(n));
+    f.Xp = c.take<float>(sweep_xinv_count(n));
+    f.tiles = c.take<float>(sweep_tiles_count(n));
     f.dinfo = c.take<int>(4);
     f.scratch = c.take<double>((size_t)n + 8 + symm_part_count(n, 1));
     return f;
@@ -392,6 +393,8 @@ size_t mixed_factor_bytes(int n) {
     Carve c(nullptr, 0);
     c.take<float>((size_t)n * n);
     c.take<float>(solve_dinv_count(n));
+    c.take<float>(sweep_xinv_count(n));
+    c.take<float>(sweep_tiles_count(n));
     c.take<int>(4);
     c.take<double>((size_t)n + 8 + symm_part_count(n, 1));
     return c.off + 256;
@@ -404,8 +407,9 @@ size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o) {
 }
 
 // Low-precision factor of A: S = float(A) (lower), FP64 diagonal potrf,
-// float panels and trailing updates; diagonal-block inverses for the solves;
-// ||A||_inf.  On breakdown the factor of A + sigma*diag(A) is retried.
+// float panels and trailing updates; then the operands of the block-inverse
+// correction sweeps (diagonal-block inverses, look-ahead products, the
+// mirror of L in the upper triangle of S); ||A||_inf.  On breakdown the factor of A + sigma*diag(A) is retried.
 int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFactor& f, hipStream_t s) {
     if (n < 0 || lda < std::max(1, n) || o.nb < 1 || o.nb > 128) return kErrArg;
     f.lowprec_info = 0;
@@ -433,7 +437,10 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
         f.shifted = 1;
     }
     if (info != 0) return kErrNotConverged;  // broke down even after shifting
-    return solve_prepare<float>(n, f.S, n, f.Dinv, s);
+    int rc = solve_prepare<float>(n, f.S, n, f.Dinv, s);
+    if (rc) return rc;
+    ELEMAMD_HIP_TRY(launch_sweep_prepare(n, f.S, n, f.Dinv, f.Xp, f.tiles, s));
+    return kOk;
 }
 
 int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o, void* work,
@@ -487,6 +494,7 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
     bn.resize(k);
     ELEMAMD_HIP_TRY(hipMemcpyAsync(bn.data(), w.nrm, sizeof(double) * k, hipMemcpyDeviceToHost, s));
 
+    ELEMAMD_HIP_TRY(launch_sweep_clear_error(n, w.sweep, s));
     // X = 0, R = B
     ELEMAMD_HIP_TRY(launch_copy_f64(n, k, B, ldb, w.R, n, s));
     ELEMAMD_HIP_TRY(launch_scale_f64(n, k, 0.0, w.X, n, s));
@@ -521,8 +529,7 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
         if (!gmres) {
             // d = M^{-1} r in float, x += d
             ELEMAMD_HIP_TRY(launch_copy_f64_to_f32_scaled(n, k, w.R, n, w.nrm + k, w.Rf, n, s));
-            rc = solve_apply<float>(true, true, n, k, f.S, n, w.Rf, n, f.Dinv, w.ctrl, nullptr, s);
-            if (rc) return rc;
+            ELEMAMD_HIP_TRY(launch_sweep_solve(n, k, f.S, n, f.tiles, w.Rf, n, w.sweep, s));
             ELEMAMD_HIP_TRY(launch_axpy_f32_to_f64_scaled(n, k, w.Rf, n, w.nrm + k, w.X, n, s));
         } else {
             r.used_gmres = 1;
@@ -537,8 +544,11 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
     r.backward_error = eta;
     r.converged = (eta <= o.tol) ? 1 : 0;
     ELEMAMD_HIP_TRY(launch_copy_f64(n, k, w.X, n, B, ldb, s));
-    ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
+    int lost = 0;  // a correction sweep whose bounded wait expired
+    ELEMAMD_HIP_TRY(sweep_read_error(n, w.sweep, &lost, s));
+    if (lost) r.converged = 0;
     if (rep) *rep = r;
+    if (lost) return kErrHip;
     return r.converged ? kOk : kErrNotConverged;
 }
 

@@ -57,8 +57,10 @@ struct MixedReport {
 // Low-precision factor kept for repeated solves (device memory of
 // mixed_factor_bytes(n), laid out by carve_factor).
 struct MixedFactor {
-    float* S;         // n x n float factor (ld n), lower triangle
+    float* S;         // n x n float factor (ld n): L below the diagonal, its mirror above
     float* Dinv;      // inverses of its 64x64 diagonal blocks
+    float* Xp;        // inverses of its 256x256 diagonal blocks (sweep preparation scratch)
+    float* tiles;     // operands of the block-inverse sweeps
     int* dinfo;       // device status word
     double* scratch;  // n + 8 doubles, then the split row-sum partials
     double anorm;     // ||A||_inf

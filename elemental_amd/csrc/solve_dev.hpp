@@ -37,6 +37,17 @@ struct Acc4<float> {
     typedef f4_t type;
 };
 
+// Wait states for every MFMA already issued to retire (the longest takes 64
+// cycles).  Called right after an MFMA product whose accumulators are read
+// or rewritten by what follows -- in particular across loop exits, where
+// the reads must not rely on the compiler's hazard bookkeeping.  The
+// scheduling barriers keep the product's MFMAs on their side of the wait.
+__device__ inline void mfma_retire() {
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_nop 15\n\ts_nop 15\n\ts_nop 15\n\ts_nop 15" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 // Store of a value another workgroup will read (device-coherent, bypassing
 // the per-XCD caches that are not coherent across XCDs) and the matching load.
 template <typename T>

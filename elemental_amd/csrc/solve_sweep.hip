@@ -1,0 +1,445 @@
+// Block-inverse sweeps: the float correction solves of iterative refinement,
+//   forward  y := inv(L) r      backward  x := inv(L^T) y
+// with the low-precision Cholesky factor L: the two triangular solves of
+// cholesky::SolveAfter (include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75)
+// arranged so that the sequential dependence between row blocks is short.
+//
+// Rows are cut into NB = 256-row blocks b = 0 .. T-1.  With the inverse
+// X_b = inv(L_bb) of every diagonal block and the look-ahead products
+// P(i)_b = -X_b L_{b,b-i} (i = 1 .. d) computed once per factor, the forward
+// sweep is
+//     y_b = X_b (r_b - sum_{j < b-d} L_bj y_j) + sum_{i=1..d} P(i)_b y_{b-i}.
+// Only the last product waits for the block solved just before, so one sweep
+// step costs one 64 x NB x k product per workgroup plus a flag hand-over;
+// the older blocks are folded in with d steps of slack.  The backward sweep
+// is the same with U = L^T (the strict upper triangle of the factor's
+// storage holds the mirror of L, written by the preparation), X_b^T and
+// Q(i)_b = -X_b^T U_{b,b+i}, over the blocks in reverse order.
+//
+// Workgroup (s, rt) owns rows [64 rt, 64 rt + 64) of the block of step s and
+// all (<= 32) right-hand sides:
+//   1. u = sum over source steps s' < s - d of M(rows, block(s')) y_{s'}
+//      (the A operand straight from the factor into registers, y_{s'}
+//      through LDS);
+//   2. z = r - u, published; then the block's other row tiles are awaited;
+//   3. y = X z + sum_i P(i) y_{s-i}, published.
+// Solved blocks, z and the flags move through device-coherent loads and
+// stores (solve_dev.hpp); workgroups take tickets in launch order, so every
+// wait is on a workgroup that is running or done (the R row tiles of one
+// step need to be resident together), and every wait is bounded.
+#include <algorithm>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "solve_dev.hpp"
+
+namespace elemamd {
+
+constexpr int kSwNB = 256;          // sweep block rows
+constexpr int kSwR = kSwNB / 64;    // row tiles (workgroups) per block
+constexpr int kSwLook = 3;          // look-ahead products d
+constexpr int kSwKP = 32;           // right-hand sides per sweep (row stride of the block vectors)
+constexpr int kSwVP = 48;           // LDS row stride of a block vector: B-operand reads are conflict-free
+constexpr int kSwVecQ = kSwNB * kSwKP / 2 / 256;  // 64-bit loads per thread for one block vector
+constexpr long kSwSpinLimit = 1L << 22;           // polls before a wait is declared lost (seconds)
+
+typedef unsigned long long u64;
+
+// Offset of element (a, c) of an NB x NB tile in the fragment order of the
+// sweep's A operand: row tile, wave, then per lane 4 consecutive K steps as
+// one float4 (MFMA 16x16x4 f32: lane = m + 16 g holds A[m][4 j + g]).
+__host__ __device__ inline i64 sweep_frag(int a, int c) {
+    const int rt = a >> 6, w = (a >> 4) & 3, m = a & 15;
+    const int j = c >> 2, g = c & 3;
+    return ((((i64)(rt * 4 + w) * (kSwNB / 16) + (j >> 2)) * 64 + (m + 16 * g)) * 4 + (j & 3));
+}
+
+__device__ inline f4_t mfma_f32(float a, float b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// ------------------------------------------------------------- preparation
+// Strict upper triangle of S := mirror of the lower (S(i, j) = S(j, i), i < j).
+__global__ __launch_bounds__(256) void sweep_mirror_kernel(int n, float* __restrict__ S, i64 lds) {
+    __shared__ float sT[64][65];
+    const int ti = blockIdx.y, tj = blockIdx.x;
+    if (ti > tj) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {  // sT[c][r] = S(tj*64 + r, ti*64 + c), lanes along r
+        const int c = w + 4 * q;
+        const i64 row = (i64)tj * 64 + lane, col = (i64)ti * 64 + c;
+        sT[c][lane] = (row < n && col < n) ? S[row + col * lds] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {  // S(ti*64 + c, tj*64 + r) = sT[c][r], lanes along c
+        const int r = w + 4 * q;
+        const i64 row = (i64)ti * 64 + lane, col = (i64)tj * 64 + r;
+        if (row < n && col < n && row < col) S[row + col * lds] = sT[lane][r];
+    }
+}
+
+// X_b = inv(L_bb) (column-major NB x NB, zero above the diagonal and on rows
+// or columns past n), column slab c of it per workgroup, from the inverses of
+// the 64-row diagonal blocks:  X_cc = Dinv_c,
+//     X_ic = -Dinv_i sum_{c <= l < i} L_il X_lc   (i > c).
+__global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, const float* __restrict__ S, i64 lds,
+                                                         const float* __restrict__ Dinv, float* __restrict__ Xp) {
+    __shared__ float sA[64][65];  // [row][k]
+    __shared__ float sB[64][65];  // [k][col]
+    const int c = blockIdx.x, b = blockIdx.y;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
+    const int nT64 = (int)ceil_div(n, 64);
+    float* X = Xp + (i64)b * kSwNB * kSwNB;
+    const i64 base = (i64)b * kSwNB;
+    auto dinv = [&](int i, int r, int q) -> float {
+        const int I = b * kSwR + i;
+        return I < nT64 ? Dinv[(i64)I * 4096 + r + q * 64] : 0.0f;
+    };
+    for (int idx = t; idx < 64 * c * 64; idx += 256) {
+        const int r = idx % (64 * c), cc = idx / (64 * c);
+        X[r + (i64)(64 * c + cc) * kSwNB] = 0.0f;
+    }
+    for (int idx = t; idx < 4096; idx += 256) {
+        const int r = idx & 63, q = idx >> 6;
+        X[(64 * c + r) + (i64)(64 * c + q) * kSwNB] = dinv(c, r, q);
+    }
+    __syncthreads();
+    for (int i = c + 1; i < kSwR; ++i) {
+        f4_t acc[4] = {};
+        for (int l = c; l < i; ++l) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int kk = w + 4 * q;
+                const i64 row = base + 64 * i + lane, col = base + 64 * l + kk;
+                sA[lane][kk] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // L(64i + lane, 64l + kk)
+                sB[lane][kk] = X[(64 * l + lane) + (i64)(64 * c + kk) * kSwNB];     // X(64l + lane, 64c + kk)
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[nt] = mfma_f32(sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m], acc[nt]);
+            mfma_retire();
+            __syncthreads();
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sB[16 * w + 4 * g + v][16 * nt + m] = acc[nt][v];
+        for (int idx = t; idx < 4096; idx += 256) {
+            const int r = idx & 63, q = idx >> 6;
+            sA[r][q] = dinv(i, r, q);
+        }
+        __syncthreads();
+        f4_t x[4] = {};
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) x[nt] = mfma_f32(sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m], x[nt]);
+        mfma_retire();
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                X[(64 * i + 16 * w + 4 * g + v) + (i64)(64 * c + 16 * nt + m) * kSwNB] = -x[nt][v];
+        __syncthreads();  // this slab's new tile is read by the next rows (same workgroup)
+    }
+}
+
+// The sweep operands in fragment order: tiles + ((dir*(d+1) + i)*T + s)*NB*NB
+// for direction dir (0 forward, 1 backward), step s, term i (0: the diagonal
+// inverse, i >= 1: the look-ahead product with the block i steps back).
+// One 64 x 64 sub-tile (ra, rc) per workgroup.
+__global__ __launch_bounds__(256) void sweep_tiles_kernel(int n, const float* __restrict__ S, i64 lds,
+                                                          const float* __restrict__ Xp, float* __restrict__ tiles) {
+    __shared__ float sA[64][65];  // [a][q]
+    __shared__ float sB[64][65];  // [q][c]
+    const int T = (int)ceil_div(n, kSwNB);
+    const int ra = blockIdx.x / kSwR, rc = blockIdx.x % kSwR;
+    const int i = blockIdx.y / T, s = blockIdx.y % T, dir = blockIdx.z;
+    if (i > s) return;  // term i of step s exists for s >= i only
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
+    const int b = dir ? T - 1 - s : s;
+    const int bb = dir ? b + i : b - i;
+    const float* X = Xp + (i64)b * kSwNB * kSwNB;
+    float* out = tiles + (((i64)dir * (kSwLook + 1) + i) * T + s) * kSwNB * kSwNB;
+    if (i == 0) {
+        for (int idx = t; idx < 4096; idx += 256) {
+            const int p = idx & 63, q = idx >> 6;  // p runs along the contiguous index of X
+            const int a = 64 * ra + (dir ? q : p), c = 64 * rc + (dir ? p : q);
+            out[sweep_frag(a, c)] = dir ? X[c + (i64)a * kSwNB] : X[a + (i64)c * kSwNB];
+        }
+        return;
+    }
+    // X_b lower triangular: forward X(a, q) needs q < 64(ra+1); backward X(q, a) needs q >= 64 ra
+    const int q_lo = dir ? 64 * ra : 0, q_hi = dir ? kSwNB : 64 * (ra + 1);
+    f4_t acc[4] = {};
+    for (int q0 = q_lo; q0 < q_hi; q0 += 64) {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = w + 4 * u;
+            if (!dir) {
+                sA[lane][e] = X[(64 * ra + lane) + (i64)(q0 + e) * kSwNB];  // X(a, q)
+                const i64 row = (i64)b * kSwNB + q0 + lane, col = (i64)bb * kSwNB + 64 * rc + e;
+                sB[lane][e] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // L(b*NB + q, bb*NB + c)
+            } else {
+                sA[e][lane] = X[(q0 + lane) + (i64)(64 * ra + e) * kSwNB];  // X(q, a)
+                const i64 row = (i64)bb * kSwNB + 64 * rc + lane, col = (i64)b * kSwNB + q0 + e;
+                sB[e][lane] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // U(b*NB + q, bb*NB + c) = L(bb*NB + c, b*NB + q)
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                acc[nt] = mfma_f32(sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m], acc[nt]);
+        mfma_retire();
+        __syncthreads();
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            out[sweep_frag(64 * ra + 16 * w + 4 * g + v, 64 * rc + 16 * nt + m)] = -acc[nt][v];
+}
+
+// ------------------------------------------------------------------- sweep
+// rhs: forward, column-major n x kc (ld ldr); backward, the forward sweep's
+// row-major output.  Y: row-major output (T*NB x KP), Z: row-major scratch.
+// out (backward only): column-major destination of the solution.
+template <int NT, bool BWD>
+__global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* __restrict__ S, i64 lds,
+                                                    const float* __restrict__ tiles, const float* __restrict__ rhs,
+                                                    i64 ldr, float* __restrict__ Y, float* __restrict__ Z,
+                                                    float* __restrict__ out, i64 ldo, int* __restrict__ ctrl,
+                                                    int* __restrict__ err) {
+    __shared__ float sV[2][kSwNB][kSwVP];
+    __shared__ int s_ord;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
+    const int T = (int)ceil_div(n, kSwNB);
+    if (t == 0) s_ord = atomicAdd(&ctrl[0], 1);
+    __syncthreads();
+    const int ord = s_ord;
+    const int s = ord / kSwR, rt = ord - s * kSwR;
+    int* yflag = ctrl + 1;              // [step][row tile]: solved rows published
+    int* zflag = ctrl + 1 + T * kSwR;   // [step][row tile]: z rows published
+    auto blk = [&](int st) { return BWD ? T - 1 - st : st; };
+    const int b = blk(s);
+    const i64 row0 = (i64)b * kSwNB + 64 * rt;
+    const i64 arow = row0 + 16 * w + m < n ? row0 + 16 * w + m : (i64)n - 1;  // A-operand row (clamped in bounds)
+    const i64 orow = row0 + 16 * w + 4 * g;               // output rows orow + v of this lane
+
+    // The R row tiles of one step, then the workgroup.  A wait that expires
+    // records an error in *err (kept until the caller clears it); every later
+    // wait sees it and gives up at once, so a lost flag costs one expiry, not
+    // one per wait.
+    auto wait_all = [&](const int* f) {
+        if (w == 0) {
+            long spins = 0;
+            for (;;) {
+                const int v = (lane < kSwR) ? coherent_load(&f[lane]) : (lane == kSwR ? coherent_load(err) == 0 : 1);
+                if (__ballot(v == 0) == 0ull) break;                      // all flags up
+                if (__ballot(lane == kSwR && v == 0) != 0ull) break;      // an earlier wait expired
+                if (++spins > kSwSpinLimit) {
+                    if (lane == 0) atomicOr(err, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __syncthreads();
+    };
+    u64 gv[kSwVecQ];
+    auto vec_load = [&](const float* src, int bb) {  // a published block vector: coherent loads
+        const u64* p = reinterpret_cast<const u64*>(src + (i64)bb * kSwNB * kSwKP);
+#pragma unroll
+        for (int q = 0; q < kSwVecQ; ++q) gv[q] = coherent_load(&p[t + 256 * q]);
+    };
+    auto vec_store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < kSwVecQ; ++q) {
+            const int e = t + 256 * q, r = e >> 4, cp = e & 15;
+            float2 v;
+            v.x = __uint_as_float((unsigned)gv[q]);
+            v.y = __uint_as_float((unsigned)(gv[q] >> 32));
+            *reinterpret_cast<float2*>(&sV[buf][r][2 * cp]) = v;
+        }
+    };
+    float a[kSwNB / 4];
+    // M(arow, bb*NB + 4j + g): L below the diagonal, its mirror above.  Rows
+    // and columns past n read clamped (finite) entries: they meet zero rows
+    // of the block vectors or produce output rows that are never kept.
+    auto a_from_factor = [&](int bb) {
+        const i64 c0 = (i64)bb * kSwNB + g;
+#pragma unroll
+        for (int j = 0; j < kSwNB / 4; ++j) {
+            const i64 col = c0 + 4 * j < n ? c0 + 4 * j : (i64)n - 1;
+            a[j] = S[arow + col * lds];
+        }
+    };
+    auto a_from_tile = [&](int i) {
+        const float4* p = reinterpret_cast<const float4*>(tiles + ((i64)i * T + s) * kSwNB * kSwNB) +
+                          (i64)(rt * 4 + w) * (kSwNB / 16) * 64 + lane;
+#pragma unroll
+        for (int jq = 0; jq < kSwNB / 16; ++jq) {
+            const float4 v = p[(i64)jq * 64];
+            a[4 * jq] = v.x;
+            a[4 * jq + 1] = v.y;
+            a[4 * jq + 2] = v.z;
+            a[4 * jq + 3] = v.w;
+        }
+    };
+    auto product = [&](int buf, f4_t (&acc)[NT]) {
+#pragma unroll
+        for (int j = 0; j < kSwNB / 4; ++j)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[nt] = mfma_f32(a[j], sV[buf][4 * j + g][16 * nt + m], acc[nt]);
+        mfma_retire();
+    };
+
+    // own right-hand sides, in the accumulator layout (row orow + v, column 16 nt + m)
+    float r[NT][4];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const i64 row = orow + v;
+            const int col = 16 * nt + m;
+            if (BWD) r[nt][v] = rhs[row * kSwKP + col];
+            else r[nt][v] = (row < n && col < kc) ? rhs[row + col * ldr] : 0.0f;
+        }
+
+    // 1. older blocks
+    const int nU = s > kSwLook ? s - kSwLook : 0;
+    f4_t acc[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
+    int buf = 0;
+    if (nU > 0) a_from_factor(blk(0));
+    for (int sp = 0; sp < nU; ++sp) {
+        wait_all(yflag + sp * kSwR);
+        vec_load(Y, blk(sp));
+        vec_store(buf);
+        __syncthreads();
+        product(buf, acc);
+        if (sp + 1 < nU) a_from_factor(blk(sp + 1));
+        buf ^= 1;
+    }
+
+    // 2. z = r - u (rows past n are zero), published
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const i64 row = orow + v;
+            coherent_store(&Z[row * kSwKP + 16 * nt + m], row < n ? r[nt][v] - acc[nt][v] : 0.0f);
+        }
+    publish_flag(&zflag[s * kSwR + rt], t);
+
+    // 3. y = X z + sum_i P(i) y_{s-i}
+    a_from_tile(0);
+    wait_all(zflag + s * kSwR);
+    vec_load(Z, b);
+    vec_store(buf);
+    __syncthreads();
+    f4_t y[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) y[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
+    product(buf, y);
+    buf ^= 1;
+    for (int i = kSwLook; i >= 1; --i) {
+        if (s - i < 0) continue;
+        a_from_tile(i);
+        wait_all(yflag + (s - i) * kSwR);
+        vec_load(Y, blk(s - i));
+        vec_store(buf);
+        __syncthreads();
+        product(buf, y);
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const i64 row = orow + v;
+            const int col = 16 * nt + m;
+            const float yv = row < n ? y[nt][v] : 0.0f;
+            coherent_store(&Y[row * kSwKP + col], yv);
+            if (BWD && row < n && col < kc) out[row + col * ldo] = yv;
+        }
+    publish_flag(&yflag[s * kSwR + rt], t);
+}
+
+// ------------------------------------------------------------------ host
+size_t sweep_tiles_count(int n) {
+    const i64 T = ceil_div(n, kSwNB);
+    return (size_t)2 * (kSwLook + 1) * T * kSwNB * kSwNB;
+}
+size_t sweep_xinv_count(int n) { return (size_t)ceil_div(n, kSwNB) * kSwNB * kSwNB; }
+
+// Work layout: Yf, Yb, Z (rows x KP floats each), then the error word
+// (64 ints reserved), then the forward and backward tickets and flags.
+static size_t sweep_ctrl_ints(int n) { return 1 + 2 * (size_t)ceil_div(n, kSwNB) * kSwR; }
+static size_t sweep_vec_bytes(int n) {
+    const size_t rows = (size_t)ceil_div(n, kSwNB) * kSwNB;
+    return (3 * rows * kSwKP * sizeof(float) + 255) & ~(size_t)255;
+}
+static int* sweep_err(int n, void* work) { return (int*)((char*)work + sweep_vec_bytes(n)); }
+
+size_t sweep_work_bytes(int n) { return sweep_vec_bytes(n) + (64 + 2 * sweep_ctrl_ints(n)) * sizeof(int) + 256; }
+
+hipError_t launch_sweep_clear_error(int n, void* work, hipStream_t s) {
+    return launch_fill_int(sweep_err(n, work), 0, 1, s);
+}
+
+hipError_t sweep_read_error(int n, void* work, int* out, hipStream_t s) {
+    hipError_t e = hipMemcpyAsync(out, sweep_err(n, work), sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t launch_sweep_prepare(int n, float* S, i64 lds, const float* Dinv, float* Xp, float* tiles, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned nt64 = (unsigned)ceil_div(n, 64), T = (unsigned)ceil_div(n, kSwNB);
+    hipLaunchKernelGGL(sweep_mirror_kernel, dim3(nt64, nt64), dim3(256), 0, s, n, S, lds);
+    hipLaunchKernelGGL(sweep_xinv_kernel, dim3(kSwR, T), dim3(256), 0, s, n, S, lds, Dinv, Xp);
+    hipLaunchKernelGGL(sweep_tiles_kernel, dim3(kSwR * kSwR, (kSwLook + 1) * T, 2), dim3(256), 0, s, n, S, lds, Xp,
+                       tiles);
+    return hipGetLastError();
+}
+
+template <int NT>
+static void launch_pair(int n, int kc, const float* S, i64 lds, const float* tiles, float* B, i64 ldb, float* Yf,
+                        float* Yb, float* Z, int* ctrl, int* err, hipStream_t s) {
+    const unsigned T = (unsigned)ceil_div(n, kSwNB);
+    const size_t half = (size_t)(kSwLook + 1) * T * kSwNB * kSwNB;
+    hipLaunchKernelGGL((sweep_kernel<NT, false>), dim3(T * kSwR), dim3(256), 0, s, n, kc, S, lds, tiles, B, ldb, Yf,
+                       Z, nullptr, 0, ctrl, err);
+    hipLaunchKernelGGL((sweep_kernel<NT, true>), dim3(T * kSwR), dim3(256), 0, s, n, kc, S, lds, tiles + half, Yf,
+                       (i64)kSwKP, Yb, Z, B, ldb, ctrl + sweep_ctrl_ints(n), err);
+}
+
+hipError_t launch_sweep_solve(int n, int k, const float* S, i64 lds, const float* tiles, float* B, i64 ldb, void* work,
+                              hipStream_t s) {
+    if (n <= 0 || k <= 0) return hipSuccess;
+    const size_t rows = (size_t)ceil_div(n, kSwNB) * kSwNB;
+    float* Yf = (float*)work;
+    float* Yb = Yf + rows * kSwKP;
+    float* Z = Yb + rows * kSwKP;
+    int* err = sweep_err(n, work);
+    int* ctrl = err + 64;
+    for (int c0 = 0; c0 < k; c0 += kSwKP) {
+        const int kc = std::min(kSwKP, k - c0);
+        hipError_t e = launch_fill_int(ctrl, 0, (int)(2 * sweep_ctrl_ints(n)), s);
+        if (e != hipSuccess) return e;
+        if (kc <= 16)
+            launch_pair<1>(n, kc, S, lds, tiles, B + (i64)c0 * ldb, ldb, Yf, Yb, Z, ctrl, err, s);
+        else
+            launch_pair<2>(n, kc, S, lds, tiles, B + (i64)c0 * ldb, ldb, Yf, Yb, Z, ctrl, err, s);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace elemamd

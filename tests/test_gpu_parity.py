@@ -292,6 +292,22 @@ def test_mixed_ir_reaches_fp64_backward_error(oracle, trailing, kappa, gmres):
     assert fwd <= 100 * kappa * EPS
 
 
+@pytest.mark.parametrize("n,k", [(64, 1), (200, 3), (256, 17), (777, 33), (1300, 40), (2048, 32)])
+def test_mixed_ir_correction_sweeps_shapes(oracle, n, k):
+    """The block-inverse correction sweeps over partial blocks, one or two
+    16-column halves and several 32-column chunks: with kappa = 1e3 and an
+    FP32 factor each refinement step gains ~kappa * 2^-24, so three or four
+    classic steps reach 1e-14 unless a sweep is wrong."""
+    A = inputs.spectrum_host(n, 1e3, seed=31 + n)
+    B = inputs.normal_host(n, k, seed=32 + n)
+    Ad, Bd = dev(A), dev(B)
+    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing="f32"))
+    X = host(Bd)
+    assert rep.converged and not rep.used_gmres, rep
+    assert rep.iterations <= 5, rep
+    assert _eta_inf(A, X, B, oracle) <= 1e-14
+
+
 # ------------------------------------------------------------ full size
 @pytest.mark.slow
 def test_full_size_integer_factor_and_solve_exact():
