@@ -18,8 +18,12 @@ Other workloads (each prints its own single JSON line):
                  FP32 vs split-BF16x3 trailing update with FP64 refinement
   --mode serve : Poisson arrivals of factor+solve requests, n log-uniform in [64, 8192],
                  kappa = 1e4, 30% reuse of a resident factor, at 3 arrival rates
+  --mode roofline : only the trailing-update replays of the single workload (the
+                 dominant kernel, both precisions) and a 1 GiB device copy; the
+                 command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+                 for the HBM traffic figure (the copy calibrates the counters)
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--mode single|batch|sweep|serve]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--mode single|batch|sweep|serve|roofline]
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -52,7 +56,7 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline")
     p.add_argument("--cpu-runs", type=int, default=3)
-    p.add_argument("--mode", default="single", choices=["single", "batch", "sweep", "serve"])
+    p.add_argument("--mode", default="single", choices=["single", "batch", "sweep", "serve", "roofline"])
     p.add_argument("--count", type=int, default=8192, help="batch: number of matrices")
     p.add_argument("--requests", type=int, default=160, help="serve: requests per arrival rate")
     return p.parse_args()
@@ -153,6 +157,48 @@ def syrk_roofline(n, nb, trailing, stream):
     total_ms = ev0.elapsed_time(ev1)
     flops = sum(m * (m + 1) * k for m, _, k in rest)  # lower triangle: m(m+1)/2 entries x 2k
     return total_ms / len(rest), flops / len(rest), len(rest)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "trailing_update_traffic_n16384.md")
+
+
+def profiled_traffic(n, nb, trailing):
+    """HBM bytes per trailing-update launch of the replay, as measured by
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on `bench.py --mode roofline` and
+    recorded in the committed profile summary (lines
+    `traffic_bytes_per_launch[<trailing>] = <bytes>`); None when the summary
+    is absent or was taken on another workload."""
+    import re
+
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    if f"n={n} nb={nb}" not in text:
+        return None
+    m = re.search(r"traffic_bytes_per_launch\[" + re.escape(trailing) + r"\] = ([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def bench_roofline(args):
+    """Trailing-update replays (the selected precision, then FP64) and a 1 GiB
+    device copy, for counter collection."""
+    import torch
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    out = {"mode": "roofline", "n": args.n, "nb": args.nb}
+    for tr in (args.trailing, "f64"):
+        avg_ms, flops, launches = syrk_roofline(args.n, args.nb, tr, stream)
+        out[tr] = {"avg_launch_ms": avg_ms, "flops_per_launch": flops, "launches": launches,
+                   "shapes": [[m, k] for m, nc, k in trailing_launches(args.n, args.nb, tr) if m == nc]}
+    src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    out["copy_bytes"] = src.numel() * 8
+    return out
 
 
 def cpu_baseline(A_host, B_host, runs):
@@ -416,7 +462,8 @@ def main():
               "the multi-GPU layout is described in DESIGN.md (sharding section)", file=sys.stderr)
         sys.exit(2)
     if args.mode != "single":
-        out = {"batch": bench_batch, "sweep": bench_sweep, "serve": bench_serve}[args.mode](args)
+        out = {"batch": bench_batch, "sweep": bench_sweep, "serve": bench_serve,
+               "roofline": bench_roofline}[args.mode](args)
         print(json.dumps(out))
         return
     import torch
@@ -535,7 +582,8 @@ def main():
         "speedup_mixed_vs_fp64": round(mean_plain / mean_mixed, 3),
         "roofline": {
             "bound": "mfma", "kernel": f"syrk_ln_{args.trailing}", "achieved": achieved / 1e12,
-            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+            "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+            "traffic": profiled_traffic(n, args.nb, args.trailing),
             "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_launch,
             "fp64_syrk": {"achieved": fl64 / (avg64 * 1e-3) / 1e12, "peak": PEAK["f64"] / 1e12,
                           "frac": fl64 / (avg64 * 1e-3) / PEAK["f64"], "avg_launch_ms": avg64},

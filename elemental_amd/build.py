@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libelemental_amd.so")
 ARCH = os.environ.get("ELEMAMD_ARCH", "gfx950")
 
 SOURCES = [
-    "potrf.hip",
+    "potrf.hip", "potrf_f32.hip",
     "trsm_panel.hip",
     "syrk.hip",
     "gemm.hip",

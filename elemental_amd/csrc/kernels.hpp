@@ -27,6 +27,13 @@ template <typename T>
 hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
                                    hipStream_t s);
 
+// potrf_f32.hip -- panel step of the float-storage factor: the b x b
+// (b <= 128) diagonal block factored in place (FP64 on 32 x 32 pieces) with
+// its inverse into Linv (128 x 128 floats), and B (m x b) := B inv(L11)^T.
+hipError_t launch_potrf_inv_f32(int b, float* A, i64 lda, float* Linv, int* info, int col_offset, hipStream_t s);
+hipError_t launch_panel_inv_f32(int m, int b, const float* Linv, float* B, i64 ldb, const int* info, int limit,
+                                hipStream_t s);
+
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
 // With info != null the k columns are factor columns [kcol0, kcol0 + k) and a
 // recorded failure drops the panels of its nbp-block and later ones.

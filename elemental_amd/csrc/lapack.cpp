@@ -33,6 +33,9 @@ namespace elemamd {
 //    auxiliary stream applies the group to the rest of the trailing matrix.
 //    The next group's update waits for the previous rest-update (they write
 //    the same elements); everything joins the caller's stream at the end.
+//  * Float storage (the low-precision factor): the diagonal block kernel also
+//    returns inv(L11) and the panel solve is the product A21 inv(L11)^T
+//    (potrf_f32.hip) instead of the substitution the FP64 path keeps.
 //  * Failure.  Every launch carries the device status word: a potrf or panel
 //    solve after a failed pivot does nothing, and a trailing pass applies
 //    only the panels of blocks before the failing one -- the reference's
@@ -70,10 +73,15 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     const bool planes = sizeof(T) == 4 && trailing == kTrailingBf16x3;
     const i64 ldk = ((i64)gw + 31) / 32 * 32;
     const size_t plane_elems = (size_t)n * ldk;
+    // float storage: the diagonal block's inverse (128 x 128) after the planes
+    const size_t plane_bytes = planes ? (4 * plane_elems * sizeof(unsigned short) + 255) & ~(size_t)255 : 0;
     unsigned short* pbuf = nullptr;
-    if (planes) {
-        pbuf = (unsigned short*)stream_scratch(4 * plane_elems * sizeof(unsigned short), s);
-        if (pbuf == nullptr) return kErrHip;
+    float* Linv = nullptr;
+    if (sizeof(T) == 4) {
+        char* scratch = (char*)stream_scratch(plane_bytes + 128 * 128 * sizeof(float), s);
+        if (scratch == nullptr) return kErrHip;
+        pbuf = (unsigned short*)scratch;
+        Linv = (float*)(scratch + plane_bytes);
     }
     int group = 0;
     bool rest_pending = false;
@@ -86,9 +94,15 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
             if (k > c0)  // panels c0 .. k-1 of this group into block column k
                 ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, n - k, b, k - c0, A + k + (i64)c0 * lda, lda, Akk, lda, dinfo,
                                                c0, nb, s));
-            ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
-            if (n - k - b > 0)
-                ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
+            if constexpr (sizeof(T) == 8) {
+                ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
+                if (n - k - b > 0)
+                    ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
+            } else {  // the low-precision factor: diagonal inverse, panel as a product with it
+                ELEMAMD_HIP_TRY(launch_potrf_inv_f32(b, (float*)Akk, lda, Linv, dinfo, k, s));
+                if (n - k - b > 0)
+                    ELEMAMD_HIP_TRY(launch_panel_inv_f32(n - k - b, b, Linv, (float*)Akk + b, lda, dinfo, k + b, s));
+            }
         }
         const int m2 = n - c1;
         if (m2 <= 0) break;

@@ -48,6 +48,18 @@ __device__ inline void mfma_retire() {
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// acc += a * b (one 16x16x4 FP32 MFMA step) with the accumulator tied to
+// the destination.  The compiler left to itself sometimes feeds an MFMA its
+// accumulator from other registers than the ones it writes (renaming inside
+// unrolled loops); such code gave wrong products on gfx950, so the products
+// of the float kernels issue the instruction themselves.  The leading wait
+// states cover a VALU write of an operand just before; the statement is
+// volatile so these steps keep their order with mfma_retire(), after which
+// the accumulator may be read.
+__device__ inline void mfma_acc(f4_t& acc, float a, float b) {
+    asm volatile("s_nop 1\n\tv_mfma_f32_16x16x4_f32 %0, %1, %2, %0" : "+v"(acc) : "v"(a), "v"(b));
+}
+
 // Store of a value another workgroup will read (device-coherent, bypassing
 // the per-XCD caches that are not coherent across XCDs) and the matching load.
 template <typename T>

@@ -54,8 +54,6 @@ __host__ __device__ inline i64 sweep_frag(int a, int c) {
     return ((((i64)(rt * 4 + w) * (kSwNB / 16) + (j >> 2)) * 64 + (m + 16 * g)) * 4 + (j & 3));
 }
 
-__device__ inline f4_t mfma_f32(float a, float b, f4_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-
 // ------------------------------------------------------------- preparation
 // Strict upper triangle of S := mirror of the lower (S(i, j) = S(j, i), i < j).
 __global__ __launch_bounds__(256) void sweep_mirror_kernel(int n, float* __restrict__ S, i64 lds) {
@@ -119,7 +117,7 @@ __global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, const float* __r
             for (int ks = 0; ks < 16; ++ks)
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt)
-                    acc[nt] = mfma_f32(sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m], acc[nt]);
+                    mfma_acc(acc[nt], sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m]);
             mfma_retire();
             __syncthreads();
         }
@@ -136,7 +134,7 @@ __global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, const float* __r
 #pragma unroll
         for (int ks = 0; ks < 16; ++ks)
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) x[nt] = mfma_f32(sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m], x[nt]);
+            for (int nt = 0; nt < 4; ++nt) mfma_acc(x[nt], sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m]);
         mfma_retire();
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
@@ -194,7 +192,7 @@ __global__ __launch_bounds__(256) void sweep_tiles_kernel(int n, const float* __
         for (int ks = 0; ks < 16; ++ks)
 #pragma unroll
             for (int nt = 0; nt < 4; ++nt)
-                acc[nt] = mfma_f32(sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m], acc[nt]);
+                mfma_acc(acc[nt], sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m]);
         mfma_retire();
         __syncthreads();
     }
@@ -295,7 +293,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
 #pragma unroll
         for (int j = 0; j < kSwNB / 4; ++j)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[nt] = mfma_f32(a[j], sV[buf][4 * j + g][16 * nt + m], acc[nt]);
+            for (int nt = 0; nt < NT; ++nt) mfma_acc(acc[nt], a[j], sV[buf][4 * j + g][16 * nt + m]);
         mfma_retire();
     };
 
