@@ -586,7 +586,8 @@ def main():
             "traffic": profiled_traffic(n, args.nb, args.trailing),
             "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_launch,
             "fp64_syrk": {"achieved": fl64 / (avg64 * 1e-3) / 1e12, "peak": PEAK["f64"] / 1e12,
-                          "frac": fl64 / (avg64 * 1e-3) / PEAK["f64"], "avg_launch_ms": avg64},
+                          "frac": fl64 / (avg64 * 1e-3) / PEAK["f64"], "avg_launch_ms": avg64,
+                          "traffic": profiled_traffic(n, args.nb, "f64")},
         },
         "cpu_baseline": cpu,
         "input_generation_s": round(gen_s, 2),
