@@ -31,8 +31,10 @@ hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64
 // (b <= 128) diagonal block factored in place (FP64 on 32 x 32 pieces) with
 // its inverse into Linv (128 x 128 floats), and B (m x b) := B inv(L11)^T.
 hipError_t launch_potrf_inv_f32(int b, float* A, i64 lda, float* Linv, int* info, int col_offset, hipStream_t s);
+// Hi/Lo (may be null): the result's bf16 hi/lo planes are also written, row
+// r of B to Hi + r*ldk (columns 0..b-1).
 hipError_t launch_panel_inv_f32(int m, int b, const float* Linv, float* B, i64 ldb, const int* info, int limit,
-                                hipStream_t s);
+                                unsigned short* Hi, unsigned short* Lo, i64 ldk, hipStream_t s);
 
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
 // With info != null the k columns are factor columns [kcol0, kcol0 + k) and a
@@ -52,6 +54,12 @@ hipError_t launch_split_bf16(int m, int k, const float* P, i64 ldp, unsigned sho
 hipError_t launch_syrk_ln_bf16x3_planes(int m, int n, int k, float alpha, const unsigned short* Hi,
                                         const unsigned short* Lo, i64 ldk, float beta, float* C, i64 ldc,
                                         const int* info, int kcol0, int nbp, hipStream_t s);
+
+// One 128-column block column from the planes: C (m x 128; lower part of its
+// top 128 x 128 block and all rows below) -= P P_top^T, P = rows 0..m-1 of
+// the planes, k columns (the group's earlier panels).
+hipError_t launch_syrk_col_bf16x3_planes(int m, int k, const unsigned short* Hi, const unsigned short* Lo, i64 ldk,
+                                         float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
 
 // gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,

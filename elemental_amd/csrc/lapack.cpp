@@ -67,9 +67,12 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     const int G = std::max(1, update_depth<T>(trailing) / nb);
     const i64 gw64 = (i64)G * nb;
     const int gw = (int)std::min<i64>(gw64, n);
-    // split-BF16x3: each pass's panel is split into bf16 hi/lo planes once
-    // (two buffers, alternating by group: the rest-update of a group still
-    // reads its planes while the next group's are written)
+    // split-BF16x3: the panel solve also writes its result as bf16 hi/lo
+    // planes (row-major, rows from the group's first column down), which
+    // every update of the group reads: the in-group block-column updates, the
+    // next group's block columns and the rest.  Two buffers, alternating by
+    // group: the rest-update of a group still reads its planes while the next
+    // group's are written.
     const bool planes = sizeof(T) == 4 && trailing == kTrailingBf16x3;
     const i64 ldk = ((i64)gw + 31) / 32 * 32;
     const size_t plane_elems = (size_t)n * ldk;
@@ -87,13 +90,24 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     bool rest_pending = false;
     for (int c0 = 0; c0 < n; c0 += gw, ++group) {
         const int c1 = (int)std::min<i64>(n, (i64)c0 + gw);
+        unsigned short *Hi = nullptr, *Lo = nullptr;  // this group's planes, row 0 = matrix row c0
+        if (planes) {
+            Hi = pbuf + (size_t)(group & 1) * 2 * plane_elems;
+            Lo = Hi + plane_elems;
+        }
         // the group's block columns, in order
         for (int k = c0; k < c1; k += nb) {
             const int b = std::min(nb, n - k);
             T* Akk = A + k + (i64)k * lda;
-            if (k > c0)  // panels c0 .. k-1 of this group into block column k
-                ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, n - k, b, k - c0, A + k + (i64)c0 * lda, lda, Akk, lda, dinfo,
-                                               c0, nb, s));
+            if (k > c0) {  // panels c0 .. k-1 of this group into block column k
+                if (planes)
+                    ELEMAMD_HIP_TRY(launch_syrk_col_bf16x3_planes(n - k, k - c0, Hi + (i64)(k - c0) * ldk,
+                                                                  Lo + (i64)(k - c0) * ldk, ldk, (float*)Akk, lda,
+                                                                  dinfo, c0, nb, s));
+                else
+                    ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, n - k, b, k - c0, A + k + (i64)c0 * lda, lda, Akk, lda,
+                                                   dinfo, c0, nb, s));
+            }
             if constexpr (sizeof(T) == 8) {
                 ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
                 if (n - k - b > 0)
@@ -101,7 +115,10 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
             } else {  // the low-precision factor: diagonal inverse, panel as a product with it
                 ELEMAMD_HIP_TRY(launch_potrf_inv_f32(b, (float*)Akk, lda, Linv, dinfo, k, s));
                 if (n - k - b > 0)
-                    ELEMAMD_HIP_TRY(launch_panel_inv_f32(n - k - b, b, Linv, (float*)Akk + b, lda, dinfo, k + b, s));
+                    ELEMAMD_HIP_TRY(launch_panel_inv_f32(
+                        n - k - b, b, Linv, (float*)Akk + b, lda, dinfo, k + b,
+                        planes ? Hi + (i64)(k + b - c0) * ldk + (k - c0) : nullptr,
+                        planes ? Lo + (i64)(k + b - c0) * ldk + (k - c0) : nullptr, ldk, s));
             }
         }
         const int m2 = n - c1;
@@ -111,11 +128,9 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         T* C = A + c1 + (i64)c1 * lda;        // trailing matrix, m2 x m2
         const int bn = std::min(gw, m2);      // next group's block columns
         if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
-        unsigned short *Hi = nullptr, *Lo = nullptr;
-        if (planes) {
-            Hi = pbuf + (size_t)(group & 1) * 2 * plane_elems;
-            Lo = Hi + plane_elems;
-            ELEMAMD_HIP_TRY(launch_split_bf16(m2, kg, (const float*)P, lda, Hi, Lo, ldk, s));
+        if (planes) {  // rows from c1 on
+            Hi += (i64)(c1 - c0) * ldk;
+            Lo += (i64)(c1 - c0) * ldk;
         }
         const bool rest = m2 > bn;
         if (rest) {  // fork: the rest of the trailing matrix on aux

@@ -278,7 +278,8 @@ __global__ __launch_bounds__(256) void trtri_f32_kernel(int b, const float* __re
 // column-major stores are contiguous.
 __global__ __launch_bounds__(256) void panel_inv_f32_kernel(int m, int b, const float* __restrict__ Linv,
                                                             float* __restrict__ B, i64 ldb, const int* __restrict__ info,
-                                                            int limit) {
+                                                            int limit, unsigned short* __restrict__ Hi,
+                                                            unsigned short* __restrict__ Lo, i64 ldk) {
     __shared__ float sX[128][kPiLd];  // sX[c][k] = X(c, k)
     __shared__ float sB[64][kPiLd];
     if (info != nullptr && *info != 0 && *info <= limit) return;
@@ -330,6 +331,22 @@ __global__ __launch_bounds__(256) void panel_inv_f32_kernel(int m, int b, const 
         const int r = idx & 63, c = idx >> 6;
         if (r0 + r < m && c < b) B[r0 + r + (i64)c * ldb] = sB[r][c];
     }
+    if (Hi != nullptr) {  // bf16 hi/lo planes of the result: thread = (row, 8 columns), 16-byte stores
+        for (int e = t; e < 64 * 16; e += 256) {
+            const int r = e >> 4, c = 8 * (e & 15);
+            if (r0 + r >= m || c >= b) continue;
+            s8_t h, l;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float x = (c + q < b) ? sB[r][c + q] : 0.0f;
+                const unsigned short hb = f32_to_bf16_rne(x);
+                h[q] = (short)hb;
+                l[q] = (short)f32_to_bf16_rne(x - bf16_to_f32(hb));
+            }
+            *reinterpret_cast<s8_t*>(Hi + (r0 + r) * ldk + c) = h;
+            *reinterpret_cast<s8_t*>(Lo + (r0 + r) * ldk + c) = l;
+        }
+    }
 }
 
 hipError_t launch_potrf_inv_f32(int b, float* A, i64 lda, float* Linv, int* info, int col_offset, hipStream_t s) {
@@ -341,10 +358,10 @@ hipError_t launch_potrf_inv_f32(int b, float* A, i64 lda, float* Linv, int* info
 }
 
 hipError_t launch_panel_inv_f32(int m, int b, const float* Linv, float* B, i64 ldb, const int* info, int limit,
-                                hipStream_t s) {
+                                unsigned short* Hi, unsigned short* Lo, i64 ldk, hipStream_t s) {
     if (m <= 0 || b <= 0) return hipSuccess;
     hipLaunchKernelGGL(panel_inv_f32_kernel, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, b, Linv, B, ldb,
-                       info, limit);
+                       info, limit, Hi, Lo, ldk);
     return hipGetLastError();
 }
 

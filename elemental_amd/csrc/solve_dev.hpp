@@ -60,6 +60,12 @@ __device__ inline void mfma_acc(f4_t& acc, float a, float b) {
     asm volatile("s_nop 1\n\tv_mfma_f32_16x16x4_f32 %0, %1, %2, %0" : "+v"(acc) : "v"(a), "v"(b));
 }
 
+// acc += a * b for one 16x16x32 bf16 MFMA step (8 bf16 per lane and
+// operand), accumulator tied like mfma_acc.
+__device__ inline void mfma_acc_bf16(f4_t& acc, s8_t a, s8_t b) {
+    asm volatile("s_nop 1\n\tv_mfma_f32_16x16x32_bf16 %0, %1, %2, %0" : "+v"(acc) : "v"(a), "v"(b));
+}
+
 // Store of a value another workgroup will read (device-coherent, bypassing
 // the per-XCD caches that are not coherent across XCDs) and the matching load.
 template <typename T>

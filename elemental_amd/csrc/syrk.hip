@@ -26,6 +26,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "syrk_dev.hpp"
+#include "solve_dev.hpp"
 
 namespace elemamd {
 
@@ -144,6 +145,84 @@ hipError_t launch_syrk_ln_bf16x3_planes(int m, int n, int k, float alpha, const 
     if (m <= 0 || n <= 0) return hipSuccess;
     hipLaunchKernelGGL(syrk_ln_bf16x3p_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, Hi, Lo, ldk, beta, C,
                        ldc, info, kcol0, nbp);
+    return hipGetLastError();
+}
+
+// One block column of the trailing matrix from pre-split planes (the
+// in-group update of the float factorization): C (m x 128, rows r >= c of
+// its top 128 x 128 block and every row below) := C - P P_top^T, P = the m x k
+// panel rows of the planes (hi/lo bf16, row-major, ld ldk; P_top = its first
+// 128 rows).  One 64-row slab per workgroup, wave w owns 16 rows and all 128
+// columns; the fragments are 16-byte loads straight from the planes into
+// registers (the planes' row-major order is the MFMA operand order), the
+// next 32-wide k chunk loading while the current one multiplies.  The
+// accumulators start from C and the slab's operand is negated (sign bits).
+__global__ __launch_bounds__(256) void syrk_col_bf16x3p_kernel(int m, int k, const unsigned short* __restrict__ Hi,
+                                                               const unsigned short* __restrict__ Lo, i64 ldk,
+                                                               float* __restrict__ C, i64 ldc,
+                                                               const int* __restrict__ info, int kcol0, int nbp) {
+    const int ke = syrk_valid_k(info, k, kcol0, nbp);
+    if (info != nullptr && ke == 0) return;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
+    const int r0 = blockIdx.x * 64 + 16 * w;                   // this wave's rows
+    const int ra = r0 + m16 < m ? r0 + m16 : m - 1;            // A-operand row (clamped; its outputs are dropped)
+    const unsigned short* ah_p = Hi + (i64)ra * ldk + 8 * g;
+    const unsigned short* al_p = Lo + (i64)ra * ldk + 8 * g;
+    const s8_t neg = {(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
+                      (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000};
+    f4_t acc[8];
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 4 * g + v, c = 16 * nt + m16;
+            acc[nt][v] = (r < m && (r >= 128 || c <= r)) ? C[r + (i64)c * ldc] : 0.0f;
+        }
+    s8_t fa[2][2], fb[2][8][2];  // [buffer][hi/lo], [buffer][column tile][hi/lo]
+    auto load = [&](int buf, int kc) {
+        fa[buf][0] = *reinterpret_cast<const s8_t*>(ah_p + kc) ^ neg;
+        fa[buf][1] = *reinterpret_cast<const s8_t*>(al_p + kc) ^ neg;
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            const i64 rb = (i64)(16 * nt + m16 < m ? 16 * nt + m16 : m - 1) * ldk + 8 * g + kc;
+            fb[buf][nt][0] = *reinterpret_cast<const s8_t*>(Hi + rb);
+            fb[buf][nt][1] = *reinterpret_cast<const s8_t*>(Lo + rb);
+        }
+    };
+    auto step = [&](int buf) {
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            mfma_acc_bf16(acc[nt], fa[buf][1], fb[buf][nt][0]);
+            mfma_acc_bf16(acc[nt], fa[buf][0], fb[buf][nt][1]);
+            mfma_acc_bf16(acc[nt], fa[buf][0], fb[buf][nt][0]);
+        }
+    };
+    // ldk pads the planes with zeros up to a multiple of 32, so whole chunks are read
+    const int nchunks = (ke + 31) / 32;
+    load(0, 0);
+    int c = 0;
+    for (; c + 2 <= nchunks; c += 2) {
+        if (c + 1 < nchunks) load(1, 32 * (c + 1));
+        step(0);
+        if (c + 2 < nchunks) load(0, 32 * (c + 2));
+        step(1);
+    }
+    if (c < nchunks) step(0);
+    mfma_retire();
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + 4 * g + v, cc = 16 * nt + m16;
+            if (r < m && (r >= 128 || cc <= r)) C[r + (i64)cc * ldc] = acc[nt][v];
+        }
+}
+
+hipError_t launch_syrk_col_bf16x3_planes(int m, int k, const unsigned short* Hi, const unsigned short* Lo, i64 ldk,
+                                         float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
+    if (m <= 0 || k <= 0) return hipSuccess;
+    hipLaunchKernelGGL(syrk_col_bf16x3p_kernel, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, k, Hi, Lo, ldk,
+                       C, ldc, info, kcol0, nbp);
     return hipGetLastError();
 }
 
