@@ -28,13 +28,15 @@ hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64
                                    hipStream_t s);
 
 // potrf_f32.hip -- panel step of the float-storage factor: the b x b
-// (b <= 128) diagonal block factored in place (FP64 on 32 x 32 pieces) with
-// its inverse into Linv (128 x 128 floats), and B (m x b) := B inv(L11)^T.
-hipError_t launch_potrf_inv_f32(int b, float* A, i64 lda, float* Linv, int* info, int col_offset, hipStream_t s);
-// Hi/Lo (may be null): the result's bf16 hi/lo planes are also written, row
-// r of B to Hi + r*ldk (columns 0..b-1).
-hipError_t launch_panel_inv_f32(int m, int b, const float* Linv, float* B, i64 ldb, const int* info, int limit,
-                                unsigned short* Hi, unsigned short* Lo, i64 ldk, hipStream_t s);
+// (b <= 128) diagonal block factored in place (FP64 on 32 x 32 pieces, whose
+// inverses go to Xd, 4 x 32 x 32 floats), and B (m x b) := B L^{-T} with the
+// factored block L and those inverses; Hi/Lo (may be null): the result's
+// bf16 hi/lo planes are also written, row r of B to Hi + r*ldk (columns
+// 0..b-1).
+hipError_t launch_potrf_f32(int b, float* A, i64 lda, float* Xd, int* info, int col_offset, hipStream_t s);
+hipError_t launch_panel_f32(int m, int b, const float* L, i64 ldl, const float* Xd, float* B, i64 ldb,
+                            const int* info, int limit, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+                            hipStream_t s);
 
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
 // With info != null the k columns are factor columns [kcol0, kcol0 + k) and a

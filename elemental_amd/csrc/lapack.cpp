@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -33,9 +34,9 @@ namespace elemamd {
 //    auxiliary stream applies the group to the rest of the trailing matrix.
 //    The next group's update waits for the previous rest-update (they write
 //    the same elements); everything joins the caller's stream at the end.
-//  * Float storage (the low-precision factor): the diagonal block kernel also
-//    returns inv(L11) and the panel solve is the product A21 inv(L11)^T
-//    (potrf_f32.hip) instead of the substitution the FP64 path keeps.
+//  * Float storage (the low-precision factor): the panel solve multiplies by
+//    the inverses of the diagonal block's 32 x 32 pieces (potrf_f32.hip)
+//    instead of the substitution the FP64 path keeps.
 //  * Failure.  Every launch carries the device status word: a potrf or panel
 //    solve after a failed pivot does nothing, and a trailing pass applies
 //    only the panels of blocks before the failing one -- the reference's
@@ -76,15 +77,15 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     const bool planes = sizeof(T) == 4 && trailing == kTrailingBf16x3;
     const i64 ldk = ((i64)gw + 31) / 32 * 32;
     const size_t plane_elems = (size_t)n * ldk;
-    // float storage: the diagonal block's inverse (128 x 128) after the planes
+    // float storage: the planes, then the diagonal pieces' inverses (4 x 32 x 32)
     const size_t plane_bytes = planes ? (4 * plane_elems * sizeof(unsigned short) + 255) & ~(size_t)255 : 0;
     unsigned short* pbuf = nullptr;
-    float* Linv = nullptr;
+    float* Xd = nullptr;
     if (sizeof(T) == 4) {
-        char* scratch = (char*)stream_scratch(plane_bytes + 128 * 128 * sizeof(float), s);
+        char* scratch = (char*)stream_scratch(plane_bytes + 4 * 1024 * sizeof(float), s);
         if (scratch == nullptr) return kErrHip;
         pbuf = (unsigned short*)scratch;
-        Linv = (float*)(scratch + plane_bytes);
+        Xd = (float*)(scratch + plane_bytes);
     }
     int group = 0;
     bool rest_pending = false;
@@ -113,10 +114,10 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
                 if (n - k - b > 0)
                     ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
             } else {  // the low-precision factor: diagonal inverse, panel as a product with it
-                ELEMAMD_HIP_TRY(launch_potrf_inv_f32(b, (float*)Akk, lda, Linv, dinfo, k, s));
+                ELEMAMD_HIP_TRY(launch_potrf_f32(b, (float*)Akk, lda, Xd, dinfo, k, s));
                 if (n - k - b > 0)
-                    ELEMAMD_HIP_TRY(launch_panel_inv_f32(
-                        n - k - b, b, Linv, (float*)Akk + b, lda, dinfo, k + b,
+                    ELEMAMD_HIP_TRY(launch_panel_f32(
+                        n - k - b, b, (const float*)Akk, lda, Xd, (float*)Akk + b, lda, dinfo, k + b,
                         planes ? Hi + (i64)(k + b - c0) * ldk + (k - c0) : nullptr,
                         planes ? Lo + (i64)(k + b - c0) * ldk + (k - c0) : nullptr, ldk, s));
             }

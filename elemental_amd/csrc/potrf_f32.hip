@@ -1,22 +1,24 @@
 // Panel step of the low-precision (float-storage) factorization: the
 // diagonal block L11 = chol(A11) of cholesky::LVar3
-// (include/elemental/lapack-like/Cholesky/LVar3.hpp:83-123, LVar3Unb at :19-49),
-// its inverse, and the panel solve L21 := A21 inv(L11)^T done as a product
-// with that inverse.  Three kernels, one workgroup each for the first two:
+// (include/elemental/lapack-like/Cholesky/LVar3.hpp:83-123, LVar3Unb at :19-49)
+// and the panel solve L21 := A21 L11^{-T}.  Two kernels:
 //
-// potrf_f32_kernel -- the block in LDS as float, right-looking in 32-column
-//   steps: wave 0 factors the 32 x 32 diagonal piece in FP64; waves 1-2 solve
-//   the rows below it by substitution (FP64, one row per lane); all waves
-//   apply the rank-32 update to the rest of the block (FP32 MFMA).  A block
-//   narrower than 128 is padded with the identity.
-// trtri_f32_kernel -- inv(L11) of the stored float factor: each wave inverts
-//   one 32 x 32 diagonal piece in FP64 (one column per lane), then the
-//   off-diagonal blocks follow by distance i - j,
-//       X_ij = -X_ii sum_{j <= l < i} L_il X_lj   (FP32 MFMA).
-// panel_inv_f32_kernel -- B := B inv(L11)^T per 64-row slab (FP32 MFMA), so
-//   the panel no longer runs a 128-step substitution per row; the refinement
-//   absorbs the larger rounding error of the inverse (this factor only
-//   preconditions).
+// potrf_f32_kernel -- one workgroup, the block in LDS as float, right-looking
+//   in 32-column steps: wave 0 factors the 32 x 32 diagonal piece in FP64;
+//   waves 1-2 solve the rows below it by substitution (FP64, one row per
+//   lane), their spare lanes solving identity rows for the piece's inverse;
+//   all
+//   waves apply the rank-32 update to the rest of the block (FP32 MFMA).  A
+//   block narrower than 128 is padded with the identity.
+// panel_f32_kernel -- one 64-row slab per workgroup, with the inverses of
+//   the four 32 x 32 diagonal pieces of L11 (by the diagonal-block kernel):
+//   each wave solves its 16 rows block by block,
+//       X_p = (A_p - sum_{q<p} X_q L_pq^T) inv(L_pp)^T   (FP32 MFMA),
+//   with no further synchronisation (the rows are independent).  Products
+//   with the 32 x 32 inverses replace the 128-step substitution per row; the
+//   refinement absorbs their larger rounding error (this factor only
+//   preconditions).  With planes requested the result is also written as
+//   bf16 hi/lo planes for the split-BF16x3 updates.
 #include "common.hpp"
 #include "kernels.hpp"
 #include "solve_dev.hpp"
@@ -26,17 +28,21 @@ namespace elemamd {
 constexpr int kPiLd = 132;  // LDS row stride (floats): MFMA operand reads are conflict-free
 
 // One wave: factor the 32 x 32 block of sA at (c0, c0) in FP64 into sD
-// (transposed) and back into sA as float, with the reciprocal diagonal in sRd.  Returns 0,
-// or the 1-based failing column.
+// (transposed) and back into sA as float, with the reciprocal diagonal in sRd.
+// Returns 0, or the 1-based failing column.
 // Lane (h, i) = (lane / 32, lane % 32) holds columns 16h .. 16h+15 of row i.
-// Column j travels through LDS (the wave's own writes and reads, in order):
-// its owners publish the entries below the pivot (zeros above it) and the
-// pivot, every lane reads the pivot and the 16 entries of its half, and the
-// rank-1 update runs unmasked because the published zeros leave finished
-// entries unchanged.  The pivot's reciprocal square root comes from the
-// hardware estimate refined by two Newton steps (error far below the float
-// the factor is stored in) and is kept, so nothing divides.
-__device__ inline int factor32(float (*sA)[kPiLd], double (*sD)[34], double* sCol, double* sRd, int c0, int lane) {
+// Column j travels through LDS (the wave's own writes and reads, in order;
+// two buffers by column parity so the next column's write need not wait for
+// this column's reads): its owners publish the entries below the pivot
+// (zeros above it) and the pivot, every lane reads the pivot and the 16
+// entries of its half, and the rank-1 update runs unmasked because the
+// published zeros leave finished entries unchanged.  Each update starts with
+// the next column's entry, so its publication can overlap the rest.  The
+// pivot's reciprocal square root is the hardware estimate refined by one
+// Newton step (error far below the float the factor is stored in) and is
+// kept, so nothing divides.
+__device__ inline int factor32(float (*sA)[kPiLd], double (*sD)[34], double (*sCol)[34], double* sRd, int c0,
+                               int lane) {
     // an opaque copy of the lane index keeps the compiler from hoisting the
     // per-lane masks of all four calls out of the caller's loop (it ran out
     // of registers doing so)
@@ -49,23 +55,25 @@ __device__ inline int factor32(float (*sA)[kPiLd], double (*sD)[34], double* sCo
 #pragma unroll
     for (int j = 0; j < 32; ++j) {
         const int jh = j >> 4, jq = j & 15;
+        double* col = sCol[j & 1];
         int jv = j;  // opaque per column: the lane masks are made here, not all up front
         asm volatile("" : "+s"(jv));
         const bool own = hh == (jv >> 4), diag = i == jv;
         if (own) {
-            sCol[i] = (i > jv) ? d[jq] : 0.0;
-            if (diag) sCol[32] = d[jq];
+            col[i] = (i > jv) ? d[jq] : 0.0;
+            if (diag) col[32] = d[jq];
         }
-        const double piv = sCol[32];
+        const double piv = col[32];
         if (!(piv > 0.0)) return j + 1;
-        const double hp = 0.5 * piv;
         double y = __builtin_amdgcn_rsq(piv);
-        y = fma(y, fma(-hp * y, y, 0.5), y);
-        y = fma(y, fma(-hp * y, y, 0.5), y);
-        const double lij = sCol[i] * y;
+        y = fma(y, fma(-0.5 * piv * y, y, 0.5), y);
+        const double lij = col[i] * y;
         if (own) d[jq] = diag ? piv * y : lij;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) d[q] = fma(-lij, sCol[16 * hh + q] * y, d[q]);
+        for (int u = 0; u < 16; ++u) {
+            const int q = (jq + 1 + u) & 15;  // the next column's entry first
+            d[q] = fma(-lij, col[16 * hh + q] * y, d[q]);
+        }
         if (lv == 0) sRd[j] = y;
     }
 #pragma unroll
@@ -77,14 +85,25 @@ __device__ inline int factor32(float (*sA)[kPiLd], double (*sD)[34], double* sCo
     return 0;
 }
 
-// Waves 1-2: the rows r in [c0 + 32, 128) of the 32 columns below the piece
-// factored by factor32, one row per lane, by substitution against L (FP64,
-// sD transposed): x_k = a_k / L_kk, then a_c -= x_k L_ck for c > k.
-__device__ inline void subpanel32(float (*sA)[kPiLd], const double (*sD)[34], const double* sRd, int c0, int row) {
-    if (row >= 128) return;
+// Waves 1-2, one row per lane, by substitution against the piece factored
+// by factor32 (FP64, L in sD transposed, 1/L_kk in sRd): x = a L^{-T}, i.e.
+// x_k = a_k / L_kk, then a_c -= x_k L_ck for c > k.
+//  * rows r in [c0 + 32, 128): a = row r of the 32 columns below the piece
+//    (in place in sA);
+//  * rows 128 + c (c < 32; lanes that have no row below the piece, and there
+//    are always at least 32): a = e_c, so x = column c of inv(L_piece),
+//    written to Xd (row-major 32 x 32 floats).
+__device__ inline void subpanel32(float (*sA)[kPiLd], const double (*sD)[34], const double* sRd, int c0, int row,
+                                  float* __restrict__ Xd) {
+    if (row >= 160) return;
+    const bool ident = row >= 128;
+    const int ci = row - 128, rr = ident ? 127 : row;
     double a[32];
 #pragma unroll
-    for (int c = 0; c < 32; ++c) a[c] = (double)sA[row][c0 + c];
+    for (int c = 0; c < 32; ++c) {
+        const double v = (double)sA[rr][c0 + c];
+        a[c] = ident ? (c == ci ? 1.0 : 0.0) : v;
+    }
 #pragma unroll
     for (int k = 0; k < 32; ++k) {
         a[k] *= sRd[k];
@@ -92,33 +111,12 @@ __device__ inline void subpanel32(float (*sA)[kPiLd], const double (*sD)[34], co
         for (int c = k + 1; c < 32; ++c) a[c] = fma(-a[k], sD[k][c], a[c]);
         asm volatile("" ::: "memory");  // one column of L in flight at a time (registers)
     }
+    if (ident) {
 #pragma unroll
-    for (int c = 0; c < 32; ++c) sA[row][c0 + c] = (float)a[c];
-}
-
-// One wave: the inverse of the 32 x 32 lower-triangular piece of sL at
-// (c0, c0), column c = lane (lanes 0..31), right-looking in FP64:
-// x_k *= 1/L_kk, then x_r -= L_rk x_k for r > k; into sX at (c0, c0)
-// (zero above the diagonal).
-__device__ inline void inverse32(float (*sX)[kPiLd], const float (*sL)[kPiLd], int c0, int lane) {
-    const int c = lane & 31;
-    double x[32];
+        for (int r = 0; r < 32; ++r) Xd[r * 32 + ci] = (float)a[r];
+    } else {
 #pragma unroll
-    for (int r = 0; r < 32; ++r) x[r] = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        const double dk = (double)sL[c0 + k][c0 + k];
-        double rk = __builtin_amdgcn_rcp(dk);  // hardware estimate, two Newton steps
-        rk = fma(rk, fma(-dk, rk, 1.0), rk);
-        rk = fma(rk, fma(-dk, rk, 1.0), rk);
-        x[k] *= rk;
-#pragma unroll
-        for (int r = k + 1; r < 32; ++r) x[r] = fma(-(double)sL[c0 + r][c0 + k], x[k], x[r]);
-        asm volatile("" ::: "memory");  // one column of L in flight at a time (registers)
-    }
-    if (lane < 32) {
-#pragma unroll
-        for (int r = 0; r < 32; ++r) sX[c0 + r][c0 + c] = (float)x[r];
+        for (int c = 0; c < 32; ++c) sA[row][c0 + c] = (float)a[c];
     }
 }
 
@@ -139,12 +137,15 @@ __device__ inline void tile_ab_t(f4_t& acc, const float (*P)[LP], int pr, int pc
 }
 
 // One b x b (b <= 128) diagonal block at A (float storage): L11 overwrites
-// its lower triangle.  A failing pivot records col_offset + column in *info.
+// its lower triangle, the inverses of its four 32 x 32 diagonal pieces go to
+// Xd (4 x 32 x 32 floats, row-major).  A failing pivot records
+// col_offset + column in *info.
 __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict__ A, i64 lda,
-                                                        int* __restrict__ info, int col_offset) {
+                                                        float* __restrict__ Xd, int* __restrict__ info,
+                                                        int col_offset) {
     __shared__ float sA[128][kPiLd];
     __shared__ double sD[32][34];  // the FP64 factor of the current piece, transposed
-    __shared__ double sCol[33], sRd[32];
+    __shared__ double sCol[2][34], sRd[32];
     __shared__ int sfail;
     if (info != nullptr && *info != 0 && *info <= col_offset) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
@@ -175,7 +176,7 @@ __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict
         }
         __syncthreads();
         if (sfail != 0) break;
-        if (w == 1 || w == 2) subpanel32(sA, sD, sRd, c0, c0 + 32 + 64 * (w - 1) + lane);
+        if (w == 1 || w == 2) subpanel32(sA, sD, sRd, c0, c0 + 32 + 64 * (w - 1) + lane, Xd + p * 1024);
         __syncthreads();
         // rank-32 update of the rest of the block (lower 16 x 16 tiles)
         const int nrt = (96 - c0) / 16;
@@ -205,127 +206,76 @@ __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict
     if (fail != 0 && t == 0 && info != nullptr) *info = col_offset + fail;
 }
 
-// Linv (128 x 128, column-major) := inv(L11) of the b x b float factor at L
-// (the identity beyond b); skipped, like the factor, after a failure.
-__global__ __launch_bounds__(256) void trtri_f32_kernel(int b, const float* __restrict__ L, i64 ldl,
-                                                        float* __restrict__ Linv, const int* __restrict__ info,
-                                                        int limit) {
-    __shared__ float sL[128][kPiLd];
-    __shared__ float sX[128][kPiLd];
-    __shared__ float sT[3][32][36];  // S_ij^T of the assembly, one per block of a distance
-    if (info != nullptr && *info != 0 && *info <= limit) return;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
-    {
-        float v[64];
-        const int r = t & 127;
-        const i64 rr = r < b ? r : b - 1;
-#pragma unroll
-        for (int q = 0; q < 64; ++q) {
-            const int c = (t >> 7) + 2 * q;
-            v[q] = L[rr + (i64)(c < b ? c : b - 1) * ldl];
-        }
-#pragma unroll
-        for (int q = 0; q < 64; ++q) {
-            const int c = (t >> 7) + 2 * q;
-            sL[r][c] = (r < b && c < b) ? (c <= r ? v[q] : 0.0f) : (r == c ? 1.0f : 0.0f);
-            sX[r][c] = 0.0f;
-        }
-    }
-    __syncthreads();
-    inverse32(sX, sL, 32 * w, lane);  // wave w: diagonal piece w
-    __syncthreads();
-    // assembly by distance: S_ij = sum_l L_il X_lj, then X_ij = -X_ii S_ij.
-    // sT holds S_ij transposed (row = column of S) so both products read
-    // their second operand row-wise.
-    for (int dist = 1; dist < 4; ++dist) {
-        const int nblk = 4 - dist;
-        for (int task = w; task < 4 * nblk; task += 4) {
-            const int jb = task >> 2, sub = task & 3, ib = jb + dist;
-            const int a = sub >> 1, bq = sub & 1;
-            f4_t acc = {};
-            for (int l = jb; l < ib; ++l) {
-                float pa[8], qb[8];
-#pragma unroll
-                for (int ks = 0; ks < 8; ++ks) {
-                    pa[ks] = sL[32 * ib + 16 * a + m][32 * l + 4 * ks + g];
-                    qb[ks] = sX[32 * l + 4 * ks + g][32 * jb + 16 * bq + m];
-                }
-#pragma unroll
-                for (int ks = 0; ks < 8; ++ks) mfma_acc(acc, pa[ks], qb[ks]);
-            }
-            mfma_retire();
-#pragma unroll
-            for (int v = 0; v < 4; ++v) sT[jb][16 * bq + m][16 * a + 4 * g + v] = acc[v];
-        }
-        __syncthreads();
-        for (int task = w; task < 4 * nblk; task += 4) {
-            const int jb = task >> 2, sub = task & 3, ib = jb + dist;
-            const int a = sub >> 1, bq = sub & 1;
-            f4_t acc = {};
-            tile_ab_t(acc, sX, 32 * ib + 16 * a, 32 * ib, sT[jb], 16 * bq, 0, m, g);
-            mfma_retire();
-#pragma unroll
-            for (int v = 0; v < 4; ++v) sX[32 * ib + 16 * a + 4 * g + v][32 * jb + 16 * bq + m] = -acc[v];
-        }
-        __syncthreads();
-    }
-#pragma unroll 4
-    for (int idx = t; idx < 128 * 128; idx += 256) Linv[idx] = sX[idx & 127][idx >> 7];
-}
-
-// B (m x b, b <= 128) := B inv(L11)^T with Linv from potrf_inv_f32_kernel;
-// one 64-row slab per workgroup, the result staged through LDS so the
+// B (m x b, b <= 128) := B L11^{-T}, L11 the b x b float factor at L; one
+// 64-row slab per workgroup, the result staged through LDS so the
 // column-major stores are contiguous.
-__global__ __launch_bounds__(256) void panel_inv_f32_kernel(int m, int b, const float* __restrict__ Linv,
-                                                            float* __restrict__ B, i64 ldb, const int* __restrict__ info,
-                                                            int limit, unsigned short* __restrict__ Hi,
-                                                            unsigned short* __restrict__ Lo, i64 ldk) {
-    __shared__ float sX[128][kPiLd];  // sX[c][k] = X(c, k)
+__global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const float* __restrict__ L, i64 ldl,
+                                                        const float* __restrict__ Xd, float* __restrict__ B, i64 ldb,
+                                                        const int* __restrict__ info, int limit,
+                                                        unsigned short* __restrict__ Hi,
+                                                        unsigned short* __restrict__ Lo, i64 ldk) {
+    __shared__ float sL[128][kPiLd];
+    __shared__ float sXd[4][32][36];  // inverses of the four 32 x 32 diagonal pieces
     __shared__ float sB[64][kPiLd];
     if (info != nullptr && *info != 0 && *info <= limit) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
     const i64 r0 = (i64)blockIdx.x * 64;
     {
-        float v[16];
-        for (int q0 = 0; q0 < 64; q0 += 16) {  // inv(L11): 64 elements per thread
+        float v[32];
+        // L11: row t % 128, columns t / 128 + 2q, identity beyond b (clamped loads, then select)
+        const int r = t & 127;
+        const i64 rr = r < b ? r : b - 1;
+        for (int q0 = 0; q0 < 64; q0 += 32) {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = Linv[t + 256 * (q0 + q)];
+            for (int q = 0; q < 32; ++q) {
+                const int c = (t >> 7) + 2 * (q0 + q);
+                v[q] = L[rr + (i64)(c < b ? c : b - 1) * ldl];
+            }
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int idx = t + 256 * (q0 + q);
-                sX[idx & 127][idx >> 7] = v[q];
+            for (int q = 0; q < 32; ++q) {
+                const int c = (t >> 7) + 2 * (q0 + q);
+                sL[r][c] = (r < b && c < b) ? (c <= r ? v[q] : 0.0f) : (r == c ? 1.0f : 0.0f);
             }
         }
-        // the slab: row r, columns c_lo + 4q, from clamped addresses
-        const int r = t & 63, c_lo = t >> 6;
-        const i64 rr = r0 + r < m ? r0 + r : m - 1;
-        for (int q0 = 0; q0 < 32; q0 += 16) {
+        // the slab: row t % 64, columns t / 64 + 4q
+        const int rs = t & 63, c_lo = t >> 6;
+        const i64 rb = r0 + rs < m ? r0 + rs : m - 1;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int c = c_lo + 4 * (q0 + q);
-                v[q] = B[rr + (i64)(c < b ? c : b - 1) * ldb];
-            }
+        for (int q = 0; q < 32; ++q) {
+            const int c = c_lo + 4 * q;
+            v[q] = B[rb + (i64)(c < b ? c : b - 1) * ldb];
+        }
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int c = c_lo + 4 * (q0 + q);
-                sB[r][c] = (r0 + r < m && c < b) ? v[q] : 0.0f;
-            }
+        for (int q = 0; q < 32; ++q) {
+            const int c = c_lo + 4 * q;
+            sB[rs][c] = (r0 + rs < m && c < b) ? v[q] : 0.0f;
         }
     }
+    for (int e = t; e < 4 * 1024; e += 256) sXd[e >> 10][(e >> 5) & 31][e & 31] = Xd[e];
     __syncthreads();
-    f4_t acc[8] = {};
-#pragma unroll 8
-    for (int ks = 0; ks < 32; ++ks) {
-        const float a = sB[16 * w + m16][4 * ks + g];
+    // wave w: rows 16w .. 16w+15 of the slab, pieces p = 0..3 in order
+    const int rw = 16 * w;
+    for (int p = 0; p < 4; ++p) {
+        f4_t acc[2] = {};
+        for (int q = 0; q < p; ++q) {  // S = sum_q X_q L_pq^T
+            tile_ab_t(acc[0], sB, rw, 32 * q, sL, 32 * p, 32 * q, m16, g);
+            tile_ab_t(acc[1], sB, rw, 32 * q, sL, 32 * p + 16, 32 * q, m16, g);
+        }
+        mfma_retire();
 #pragma unroll
-        for (int nt = 0; nt < 8; ++nt) mfma_acc(acc[nt], a, sX[16 * nt + m16][4 * ks + g]);
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sB[rw + 4 * g + v][32 * p + 16 * nt + m16] -= acc[nt][v];
+        // X_p = T inv(L_pp)^T (T = the updated columns, this wave's rows only)
+        f4_t x[2] = {};
+        tile_ab_t(x[0], sB, rw, 32 * p, sXd[p], 0, 0, m16, g);
+        tile_ab_t(x[1], sB, rw, 32 * p, sXd[p], 16, 0, m16, g);
+        mfma_retire();
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sB[rw + 4 * g + v][32 * p + 16 * nt + m16] = x[nt][v];
     }
-    mfma_retire();
-    __syncthreads();  // every wave is done reading sB
-#pragma unroll
-    for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) sB[16 * w + 4 * g + v][16 * nt + m16] = acc[nt][v];
     __syncthreads();
     for (int idx = t; idx < 64 * 128; idx += 256) {
         const int r = idx & 63, c = idx >> 6;
@@ -338,10 +288,10 @@ __global__ __launch_bounds__(256) void panel_inv_f32_kernel(int m, int b, const 
             s8_t h, l;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const float x = (c + q < b) ? sB[r][c + q] : 0.0f;
-                const unsigned short hb = f32_to_bf16_rne(x);
+                const float xv = (c + q < b) ? sB[r][c + q] : 0.0f;
+                const unsigned short hb = f32_to_bf16_rne(xv);
                 h[q] = (short)hb;
-                l[q] = (short)f32_to_bf16_rne(x - bf16_to_f32(hb));
+                l[q] = (short)f32_to_bf16_rne(xv - bf16_to_f32(hb));
             }
             *reinterpret_cast<s8_t*>(Hi + (r0 + r) * ldk + c) = h;
             *reinterpret_cast<s8_t*>(Lo + (r0 + r) * ldk + c) = l;
@@ -349,18 +299,17 @@ __global__ __launch_bounds__(256) void panel_inv_f32_kernel(int m, int b, const 
     }
 }
 
-hipError_t launch_potrf_inv_f32(int b, float* A, i64 lda, float* Linv, int* info, int col_offset, hipStream_t s) {
+hipError_t launch_potrf_f32(int b, float* A, i64 lda, float* Xd, int* info, int col_offset, hipStream_t s) {
     if (b <= 0) return hipSuccess;
-    hipLaunchKernelGGL(potrf_f32_kernel, dim3(1), dim3(256), 0, s, b, A, lda, info, col_offset);
-    hipLaunchKernelGGL(trtri_f32_kernel, dim3(1), dim3(256), 0, s, b, (const float*)A, lda, Linv, (const int*)info,
-                       col_offset + b);
+    hipLaunchKernelGGL(potrf_f32_kernel, dim3(1), dim3(256), 0, s, b, A, lda, Xd, info, col_offset);
     return hipGetLastError();
 }
 
-hipError_t launch_panel_inv_f32(int m, int b, const float* Linv, float* B, i64 ldb, const int* info, int limit,
-                                unsigned short* Hi, unsigned short* Lo, i64 ldk, hipStream_t s) {
+hipError_t launch_panel_f32(int m, int b, const float* L, i64 ldl, const float* Xd, float* B, i64 ldb,
+                            const int* info, int limit, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+                            hipStream_t s) {
     if (m <= 0 || b <= 0) return hipSuccess;
-    hipLaunchKernelGGL(panel_inv_f32_kernel, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, b, Linv, B, ldb,
+    hipLaunchKernelGGL(panel_f32_kernel, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, b, L, ldl, Xd, B, ldb,
                        info, limit, Hi, Lo, ldk);
     return hipGetLastError();
 }
