@@ -4,7 +4,7 @@ Workload (one "step" = one full solve of A X = B with inputs resident in HBM):
   n = 16384, kappa = 1e6 (A = Q diag(s) Q^T, Q = Householder-QR factor of an
   N(0,1) matrix, s log-spaced from 1 to 1e-6), B 16384 x 32 N(0,1).
   * mixed+IR : low-precision factor (FP64 diagonal potrf, FP32-stored panels,
-               FP32 or split-BF16x3 trailing update) refined with FP64 residuals
+               FP32, split-BF16x3 or split-BF16x6 trailing update) refined with FP64 residuals
                to normwise backward error <= 1e-14 (elemamd_hpd_solve_mixed_f64)
   * plain FP64 : Cholesky + SolveAfter in FP64 (elemamd_hpd_solve_f64)
 The headline value is mixed+IR solves/s; plain FP64 is reported beside it,
@@ -15,7 +15,7 @@ Other workloads (each prints its own single JSON line):
   --mode batch : 8192 matrices, n log-uniform in [16, 1024], A_i = G G^T/n + 0.1 I,
                  1-8 right-hand sides each, one captured batch (plain FP64 and FP32+IR)
   --mode sweep : n = 32768, kappa in {1e2, 1e4, 1e6, 1e8}, B 32768 x 8: plain FP64 vs
-                 FP32 vs split-BF16x3 trailing update with FP64 refinement
+                 FP32 vs split-BF16x3 vs split-BF16x6 trailing update with FP64 refinement
   --mode serve : Poisson arrivals of factor+solve requests, n log-uniform in [64, 8192],
                  kappa = 1e4, 30% reuse of a resident factor, at 3 arrival rates
   --mode roofline : only the trailing-update replays of the single workload (the
@@ -41,6 +41,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"f64": 78.6e12, "f32": 157.3e12, "bf16": 2.5e15}
+# MFMA products per algorithmic product: split-BF16x3 issues 3 bf16 MFMAs per
+# term of A A^T, split-BF16x6 6; "achieved" counts algorithmic flops only.
+MFMA_TERMS = {"f64": 1, "f32": 1, "bf16x3": 3, "bf16x6": 6}
 
 
 def parse():
@@ -52,7 +55,7 @@ def parse():
     p.add_argument("--k", type=int, default=32)
     p.add_argument("--kappa", type=float, default=1e6)
     p.add_argument("--nb", type=int, default=128)
-    p.add_argument("--trailing", default="bf16x3", choices=["f32", "bf16x3"])
+    p.add_argument("--trailing", default="bf16x3", choices=["f32", "bf16x3", "bf16x6"])
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline")
     p.add_argument("--cpu-runs", type=int, default=3)
@@ -92,7 +95,7 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
 
-UPDATE_DEPTH = {"f64": 256, "f32": 512, "bf16x3": 512}  # columns per trailing pass (lapack.hpp)
+UPDATE_DEPTH = {"f64": 256, "f32": 512, "bf16x3": 512, "bf16x6": 512}  # columns per trailing pass (lapack.hpp)
 
 
 def trailing_launches(n, nb, trailing):
@@ -133,7 +136,7 @@ def syrk_roofline(n, nb, trailing, stream):
     dt = torch.float64 if trailing == "f64" else torch.float32
     P = torch.randn(kmax, n, dtype=dt, device="cuda")  # column-major n x kmax panel, ld n
     C = torch.zeros(n, n, dtype=dt, device="cuda")
-    mode = {"f32": 1, "bf16x3": 2}.get(trailing)
+    mode = {"f32": 1, "bf16x3": 2, "bf16x6": 3}.get(trailing)
     s = ctypes.c_void_p(stream.cuda_stream)
 
     def launch(m, k):
@@ -336,7 +339,7 @@ def bench_sweep(args):
         row = {"kappa": kappa, "f64": {"ms": round(t64 * 1e3, 2), "backward_error": eta64,
                                        "tflops": round(n ** 3 / 3 / t64 / 1e12, 2),
                                        "frac_fp64_peak": round(n ** 3 / 3 / t64 / PEAK["f64"], 3)}}
-        for trailing in ("f32", "bf16x3"):
+        for trailing in ("f32", "bf16x3", "bf16x6"):
             opts = E.IROptions(trailing=trailing)
             ws = torch.empty(workspace_bytes(n, k, opts), dtype=torch.uint8, device="cuda")
             X.tensor().copy_(B.tensor())
@@ -537,7 +540,7 @@ def main():
 
     # ---- roofline of the dominant kernel (the trailing update of the mixed factor)
     avg_ms, flops_launch, launches = syrk_roofline(n, args.nb, args.trailing, stream)
-    peak = PEAK["bf16"] if args.trailing == "bf16x3" else PEAK["f32"]
+    peak = PEAK["f32"] if args.trailing == "f32" else PEAK["bf16"]
     achieved = flops_launch / (avg_ms * 1e-3)
     avg64, fl64, _ = syrk_roofline(n, args.nb, "f64", stream)
 
@@ -583,6 +586,7 @@ def main():
         "roofline": {
             "bound": "mfma", "kernel": f"syrk_ln_{args.trailing}", "achieved": achieved / 1e12,
             "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+            "mfma_terms": MFMA_TERMS[args.trailing],
             "traffic": profiled_traffic(n, args.nb, args.trailing),
             "launches": launches, "avg_launch_ms": avg_ms, "flops_per_launch": flops_launch,
             "fp64_syrk": {"achieved": fl64 / (avg64 * 1e-3) / 1e12, "peak": PEAK["f64"] / 1e12,

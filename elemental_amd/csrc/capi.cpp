@@ -85,23 +85,25 @@ int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const do
 
 int elemamd_herk_f32(int mode, int n, int k, float alpha, const float* A, int64_t lda, float beta, float* C,
                      int64_t ldc, void* stream) {
-    if (mode != ELEMAMD_TRAILING_F32 && mode != ELEMAMD_TRAILING_BF16X3) return ELEMAMD_ERR_ARG;
+    if (mode != ELEMAMD_TRAILING_F32 && mode != ELEMAMD_TRAILING_BF16X3 && mode != ELEMAMD_TRAILING_BF16X6)
+        return ELEMAMD_ERR_ARG;
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
     hipStream_t s = S(stream);
-    if (mode == ELEMAMD_TRAILING_BF16X3 && k > 0) {
-        // the factorization's form: split A into bf16 hi/lo planes once, then the planes update
+    if (mode != ELEMAMD_TRAILING_F32 && k > 0) {
+        // the factorization's form: split A into bf16 planes once, then the planes update
+        const int np = mode == ELEMAMD_TRAILING_BF16X3 ? 2 : 3;
         const i64 ldk = ((i64)k + 31) / 32 * 32;
-        auto* hi = (unsigned short*)stream_scratch(2 * (size_t)n * ldk * sizeof(unsigned short), s);
-        if (hi == nullptr) return ELEMAMD_ERR_HIP;
-        unsigned short* lo = hi + (size_t)n * ldk;
-        if (launch_split_bf16(n, k, A, lda, hi, lo, ldk, s) != hipSuccess) return ELEMAMD_ERR_HIP;
-        return launch_syrk_ln_bf16x3_planes(n, n, k, alpha, hi, lo, ldk, beta, C, ldc, nullptr, 0, 1, s) == hipSuccess
+        const i64 ps = (i64)n * ldk;
+        auto* planes = (unsigned short*)stream_scratch((size_t)np * ps * sizeof(unsigned short), s);
+        if (planes == nullptr) return ELEMAMD_ERR_HIP;
+        if (launch_split_bf16(np, n, k, A, lda, planes, ps, ldk, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+        return launch_syrk_ln_bf16_planes(np, n, n, k, alpha, planes, ps, ldk, beta, C, ldc, nullptr, 0, 1, s) ==
+                       hipSuccess
                    ? ELEMAMD_OK
                    : ELEMAMD_ERR_HIP;
     }
-    return launch_syrk_ln_f32(mode == ELEMAMD_TRAILING_BF16X3, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr,
-                              0, 1, s) == hipSuccess
+    return launch_syrk_ln_f32(false, n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr, 0, 1, s) == hipSuccess
                ? ELEMAMD_OK
                : ELEMAMD_ERR_HIP;
 }
@@ -173,7 +175,8 @@ int elemamd_hpd_solve_mixed_f64(int n, int k, const double* A, int64_t lda, doub
                                 const elemamd_ir_opts* opts, void* work, size_t work_bytes, elemamd_ir_report* report,
                                 void* stream) {
     MixedOpts o = to_opts(opts);
-    if (o.trailing != kTrailingF32 && o.trailing != kTrailingBf16x3) return ELEMAMD_ERR_ARG;
+    if (o.trailing != kTrailingF32 && o.trailing != kTrailingBf16x3 && o.trailing != kTrailingBf16x6)
+        return ELEMAMD_ERR_ARG;
     MixedReport r{};
     int rc = hpd_solve_mixed(n, k, A, lda, B, ldb, o, work, work_bytes, &r, S(stream));
     if (report) {

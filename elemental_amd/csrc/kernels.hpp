@@ -30,12 +30,12 @@ hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64
 // potrf_f32.hip -- panel step of the float-storage factor: the b x b
 // (b <= 128) diagonal block factored in place (FP64 on 32 x 32 pieces, whose
 // inverses go to Xd, 4 x 32 x 32 floats), and B (m x b) := B L^{-T} with the
-// factored block L and those inverses; Hi/Lo (may be null): the result's
-// bf16 hi/lo planes are also written, row r of B to Hi + r*ldk (columns
-// 0..b-1).
+// factored block L and those inverses; planes (may be null): the result's np
+// bf16 planes (launch_split_bf16's form) are also written, row r of B to
+// planes + p*ps + r*ldk (columns 0..b-1).
 hipError_t launch_potrf_f32(int b, float* A, i64 lda, float* Xd, int* info, int col_offset, hipStream_t s);
 hipError_t launch_panel_f32(int m, int b, const float* L, i64 ldl, const float* Xd, float* B, i64 ldb,
-                            const int* info, int limit, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+                            const int* info, int limit, unsigned short* planes, int np, i64 ps, i64 ldk,
                             hipStream_t s);
 
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
@@ -48,20 +48,22 @@ hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, con
                               i64 ldb, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
                               hipStream_t s);
 
-// Split-BF16x3 from pre-split panels: Hi/Lo bf16 planes of the panel
-// (row-major, row stride ldk, a multiple of 32), made by launch_split_bf16
-// from a float m x k panel; rows of the planes = rows (and columns) of C.
-hipError_t launch_split_bf16(int m, int k, const float* P, i64 ldp, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+// Split-BF16 from pre-split panels: np bf16 planes of the panel (plane p at
+// planes + p*ps, row-major, row stride ldk a multiple of 32), made by
+// launch_split_bf16 from a float m x k panel; rows of the planes = rows (and
+// columns) of C.  np = 2: split-BF16x3 (three MFMA terms), np = 3:
+// split-BF16x6 (six terms, down to 2^-24 of a product).
+hipError_t launch_split_bf16(int np, int m, int k, const float* P, i64 ldp, unsigned short* planes, i64 ps, i64 ldk,
                              hipStream_t s);
-hipError_t launch_syrk_ln_bf16x3_planes(int m, int n, int k, float alpha, const unsigned short* Hi,
-                                        const unsigned short* Lo, i64 ldk, float beta, float* C, i64 ldc,
-                                        const int* info, int kcol0, int nbp, hipStream_t s);
+hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, const unsigned short* planes, i64 ps,
+                                      i64 ldk, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
+                                      hipStream_t s);
 
 // One 128-column block column from the planes: C (m x 128; lower part of its
 // top 128 x 128 block and all rows below) -= P P_top^T, P = rows 0..m-1 of
 // the planes, k columns (the group's earlier panels).
-hipError_t launch_syrk_col_bf16x3_planes(int m, int k, const unsigned short* Hi, const unsigned short* Lo, i64 ldk,
-                                         float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
+hipError_t launch_syrk_col_bf16_planes(int np, int m, int k, const unsigned short* planes, i64 ps, i64 ldk, float* C,
+                                       i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
 
 // gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,

@@ -55,7 +55,7 @@ static hipError_t syrk_launch(int trailing, int m, int n, int k, const T* A, i64
 // update depth (columns per trailing pass)
 template <typename T>
 static int update_depth(int trailing) {
-    return sizeof(T) == 8 ? kUpdateDepthF64 : (trailing == kTrailingBf16x3 ? kUpdateDepthBf16x3 : kUpdateDepthF32);
+    return sizeof(T) == 8 ? kUpdateDepthF64 : (trailing == kTrailingF32 ? kUpdateDepthF32 : kUpdateDepthBf16x3);
 }
 
 template <typename T>
@@ -68,17 +68,18 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     const int G = std::max(1, update_depth<T>(trailing) / nb);
     const i64 gw64 = (i64)G * nb;
     const int gw = (int)std::min<i64>(gw64, n);
-    // split-BF16x3: the panel solve also writes its result as bf16 hi/lo
-    // planes (row-major, rows from the group's first column down), which
+    // split-BF16x3 / x6: the panel solve also writes its result as 2 / 3
+    // bf16 planes (row-major, rows from the group's first column down), which
     // every update of the group reads: the in-group block-column updates, the
-    // next group's block columns and the rest.  Two buffers, alternating by
+    // next group's block columns and the rest.  Two sets, alternating by
     // group: the rest-update of a group still reads its planes while the next
     // group's are written.
-    const bool planes = sizeof(T) == 4 && trailing == kTrailingBf16x3;
+    const int np = sizeof(T) == 4 ? (trailing == kTrailingBf16x3 ? 2 : trailing == kTrailingBf16x6 ? 3 : 0) : 0;
+    const bool planes = np > 0;
     const i64 ldk = ((i64)gw + 31) / 32 * 32;
     const size_t plane_elems = (size_t)n * ldk;
     // float storage: the planes, then the diagonal pieces' inverses (4 x 32 x 32)
-    const size_t plane_bytes = planes ? (4 * plane_elems * sizeof(unsigned short) + 255) & ~(size_t)255 : 0;
+    const size_t plane_bytes = planes ? (2 * np * plane_elems * sizeof(unsigned short) + 255) & ~(size_t)255 : 0;
     unsigned short* pbuf = nullptr;
     float* Xd = nullptr;
     if (sizeof(T) == 4) {
@@ -91,20 +92,17 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     bool rest_pending = false;
     for (int c0 = 0; c0 < n; c0 += gw, ++group) {
         const int c1 = (int)std::min<i64>(n, (i64)c0 + gw);
-        unsigned short *Hi = nullptr, *Lo = nullptr;  // this group's planes, row 0 = matrix row c0
-        if (planes) {
-            Hi = pbuf + (size_t)(group & 1) * 2 * plane_elems;
-            Lo = Hi + plane_elems;
-        }
+        unsigned short* Pl = nullptr;  // this group's planes (stride plane_elems), row 0 = matrix row c0
+        if (planes) Pl = pbuf + (size_t)(group & 1) * np * plane_elems;
         // the group's block columns, in order
         for (int k = c0; k < c1; k += nb) {
             const int b = std::min(nb, n - k);
             T* Akk = A + k + (i64)k * lda;
             if (k > c0) {  // panels c0 .. k-1 of this group into block column k
                 if (planes)
-                    ELEMAMD_HIP_TRY(launch_syrk_col_bf16x3_planes(n - k, k - c0, Hi + (i64)(k - c0) * ldk,
-                                                                  Lo + (i64)(k - c0) * ldk, ldk, (float*)Akk, lda,
-                                                                  dinfo, c0, nb, s));
+                    ELEMAMD_HIP_TRY(launch_syrk_col_bf16_planes(np, n - k, k - c0, Pl + (i64)(k - c0) * ldk,
+                                                                (i64)plane_elems, ldk, (float*)Akk, lda, dinfo, c0,
+                                                                nb, s));
                 else
                     ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, n - k, b, k - c0, A + k + (i64)c0 * lda, lda, Akk, lda,
                                                    dinfo, c0, nb, s));
@@ -118,8 +116,7 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
                 if (n - k - b > 0)
                     ELEMAMD_HIP_TRY(launch_panel_f32(
                         n - k - b, b, (const float*)Akk, lda, Xd, (float*)Akk + b, lda, dinfo, k + b,
-                        planes ? Hi + (i64)(k + b - c0) * ldk + (k - c0) : nullptr,
-                        planes ? Lo + (i64)(k + b - c0) * ldk + (k - c0) : nullptr, ldk, s));
+                        planes ? Pl + (i64)(k + b - c0) * ldk + (k - c0) : nullptr, np, (i64)plane_elems, ldk, s));
             }
         }
         const int m2 = n - c1;
@@ -129,19 +126,16 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         T* C = A + c1 + (i64)c1 * lda;        // trailing matrix, m2 x m2
         const int bn = std::min(gw, m2);      // next group's block columns
         if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
-        if (planes) {  // rows from c1 on
-            Hi += (i64)(c1 - c0) * ldk;
-            Lo += (i64)(c1 - c0) * ldk;
-        }
+        if (planes) Pl += (i64)(c1 - c0) * ldk;  // rows from c1 on
         const bool rest = m2 > bn;
         if (rest) {  // fork: the rest of the trailing matrix on aux
             ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
             ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
             if (planes)
-                ELEMAMD_HIP_TRY(launch_syrk_ln_bf16x3_planes(m2 - bn, m2 - bn, kg, -1.0f, Hi + (i64)bn * ldk,
-                                                             Lo + (i64)bn * ldk, ldk, 1.0f,
-                                                             (float*)(C + bn + (i64)bn * lda), lda, dinfo, c0, nb,
-                                                             sp->aux));
+                ELEMAMD_HIP_TRY(launch_syrk_ln_bf16_planes(np, m2 - bn, m2 - bn, kg, -1.0f, Pl + (i64)bn * ldk,
+                                                           (i64)plane_elems, ldk, 1.0f,
+                                                           (float*)(C + bn + (i64)bn * lda), lda, dinfo, c0, nb,
+                                                           sp->aux));
             else
                 ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2 - bn, m2 - bn, kg, P + bn, lda, C + bn + (i64)bn * lda,
                                                lda, dinfo, c0, nb, sp->aux));
@@ -150,8 +144,8 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         rest_pending = rest;
         // critical chain on the caller's stream: the next group's block columns
         if (planes)
-            ELEMAMD_HIP_TRY(launch_syrk_ln_bf16x3_planes(m2, bn, kg, -1.0f, Hi, Lo, ldk, 1.0f, (float*)C, lda, dinfo,
-                                                         c0, nb, s));
+            ELEMAMD_HIP_TRY(launch_syrk_ln_bf16_planes(np, m2, bn, kg, -1.0f, Pl, (i64)plane_elems, ldk, 1.0f,
+                                                       (float*)C, lda, dinfo, c0, nb, s));
         else
             ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2, bn, kg, P, lda, C, lda, dinfo, c0, nb, s));
     }

@@ -5,7 +5,7 @@
 
 namespace elemamd {
 
-enum : int { kTrailingF64 = 0, kTrailingF32 = 1, kTrailingBf16x3 = 2 };
+enum : int { kTrailingF64 = 0, kTrailingF32 = 1, kTrailingBf16x3 = 2, kTrailingBf16x6 = 3 };
 
 // Columns of panels applied per pass over the trailing matrix (a multiple of
 // the block size nb is used: max(1, depth / nb) panels).

@@ -212,8 +212,8 @@ __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict
 __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const float* __restrict__ L, i64 ldl,
                                                         const float* __restrict__ Xd, float* __restrict__ B, i64 ldb,
                                                         const int* __restrict__ info, int limit,
-                                                        unsigned short* __restrict__ Hi,
-                                                        unsigned short* __restrict__ Lo, i64 ldk) {
+                                                        unsigned short* __restrict__ planes, int np, i64 ps,
+                                                        i64 ldk) {
     __shared__ float sL[128][kPiLd];
     __shared__ float sXd[4][32][36];  // inverses of the four 32 x 32 diagonal pieces
     __shared__ float sB[64][kPiLd];
@@ -281,20 +281,23 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
         const int r = idx & 63, c = idx >> 6;
         if (r0 + r < m && c < b) B[r0 + r + (i64)c * ldb] = sB[r][c];
     }
-    if (Hi != nullptr) {  // bf16 hi/lo planes of the result: thread = (row, 8 columns), 16-byte stores
+    if (planes != nullptr) {  // bf16 planes of the result: thread = (row, 8 columns), 16-byte stores
         for (int e = t; e < 64 * 16; e += 256) {
             const int r = e >> 4, c = 8 * (e & 15);
             if (r0 + r >= m || c >= b) continue;
-            s8_t h, l;
+            float x[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const float xv = (c + q < b) ? sB[r][c + q] : 0.0f;
-                const unsigned short hb = f32_to_bf16_rne(xv);
-                h[q] = (short)hb;
-                l[q] = (short)f32_to_bf16_rne(xv - bf16_to_f32(hb));
+            for (int q = 0; q < 8; ++q) x[q] = (c + q < b) ? sB[r][c + q] : 0.0f;
+            for (int p = 0; p < np; ++p) {
+                s8_t h;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const unsigned short hb = f32_to_bf16_rne(x[q]);
+                    h[q] = (short)hb;
+                    x[q] -= bf16_to_f32(hb);
+                }
+                *reinterpret_cast<s8_t*>(planes + p * ps + (r0 + r) * ldk + c) = h;
             }
-            *reinterpret_cast<s8_t*>(Hi + (r0 + r) * ldk + c) = h;
-            *reinterpret_cast<s8_t*>(Lo + (r0 + r) * ldk + c) = l;
         }
     }
 }
@@ -306,11 +309,11 @@ hipError_t launch_potrf_f32(int b, float* A, i64 lda, float* Xd, int* info, int 
 }
 
 hipError_t launch_panel_f32(int m, int b, const float* L, i64 ldl, const float* Xd, float* B, i64 ldb,
-                            const int* info, int limit, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+                            const int* info, int limit, unsigned short* planes, int np, i64 ps, i64 ldk,
                             hipStream_t s) {
     if (m <= 0 || b <= 0) return hipSuccess;
     hipLaunchKernelGGL(panel_f32_kernel, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, b, L, ldl, Xd, B, ldb,
-                       info, limit, Hi, Lo, ldk);
+                       info, limit, planes, np, ps, ldk);
     return hipGetLastError();
 }
 

@@ -4,7 +4,7 @@
 // cholesky::SolveAfter (include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75)
 // arranged so that the sequential dependence between row blocks is short.
 //
-// Rows are cut into NB = 256-row blocks b = 0 .. T-1.  With the inverse
+// Rows are cut into NB = 512-row blocks b = 0 .. T-1.  With the inverse
 // X_b = inv(L_bb) of every diagonal block and the look-ahead products
 // P(i)_b = -X_b L_{b,b-i} (i = 1 .. d) computed once per factor, the forward
 // sweep is
@@ -35,12 +35,13 @@
 
 namespace elemamd {
 
-constexpr int kSwNB = 256;          // sweep block rows
+constexpr int kSwNB = 512;          // sweep block rows
 constexpr int kSwR = kSwNB / 64;    // row tiles (workgroups) per block
+constexpr int kSwH = 256;           // block-vector rows staged in LDS at a time
 constexpr int kSwLook = 3;          // look-ahead products d
 constexpr int kSwKP = 32;           // right-hand sides per sweep (row stride of the block vectors)
 constexpr int kSwVP = 48;           // LDS row stride of a block vector: B-operand reads are conflict-free
-constexpr int kSwVecQ = kSwNB * kSwKP / 2 / 256;  // 64-bit loads per thread for one block vector
+constexpr int kSwVecQ = kSwH * kSwKP / 2 / 256;   // 64-bit loads per thread for one staged half
 constexpr long kSwSpinLimit = 1L << 22;           // polls before a wait is declared lost (seconds)
 
 typedef unsigned long long u64;
@@ -52,6 +53,33 @@ __host__ __device__ inline i64 sweep_frag(int a, int c) {
     const int rt = a >> 6, w = (a >> 4) & 3, m = a & 15;
     const int j = c >> 2, g = c & 3;
     return ((((i64)(rt * 4 + w) * (kSwNB / 16) + (j >> 2)) * 64 + (m + 16 * g)) * 4 + (j & 3));
+}
+
+// acc[nt] += sum_k sA[16 w + m][k] sB[k][16 nt + m], k = 0 .. 63 (one wave's
+// 16 x 64 slice of a 64 x 64 x 64 product staged in LDS).  The operands are
+// read kD K-steps ahead of their MFMAs (see sweep_kernel's product).
+__device__ inline void lds_tile_product(f4_t (&acc)[4], const float (*sA)[65], const float (*sB)[65], int w, int m,
+                                        int g) {
+    constexpr int kD = 2;
+    float qa[kD + 1], qb[kD + 1][4];
+#pragma unroll
+    for (int ks = 0; ks < kD; ++ks) {
+        qa[ks] = sA[16 * w + m][4 * ks + g];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) qb[ks][nt] = sB[4 * ks + g][16 * nt + m];
+    }
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        if (ks + kD < 16) {
+            const int r = (ks + kD) % (kD + 1);
+            qa[r] = sA[16 * w + m][4 * (ks + kD) + g];
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) qb[r][nt] = sB[4 * (ks + kD) + g][16 * nt + m];
+        }
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) mfma_acc(acc[nt], qa[ks % (kD + 1)], qb[ks % (kD + 1)][nt]);
+    }
+    mfma_retire();
 }
 
 // ------------------------------------------------------------- preparation
@@ -113,12 +141,7 @@ __global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, const float* __r
                 sB[lane][kk] = X[(64 * l + lane) + (i64)(64 * c + kk) * kSwNB];     // X(64l + lane, 64c + kk)
             }
             __syncthreads();
-#pragma unroll
-            for (int ks = 0; ks < 16; ++ks)
-#pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    mfma_acc(acc[nt], sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m]);
-            mfma_retire();
+            lds_tile_product(acc, sA, sB, w, m, g);
             __syncthreads();
         }
 #pragma unroll
@@ -131,11 +154,7 @@ __global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, const float* __r
         }
         __syncthreads();
         f4_t x[4] = {};
-#pragma unroll
-        for (int ks = 0; ks < 16; ++ks)
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt) mfma_acc(x[nt], sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m]);
-        mfma_retire();
+        lds_tile_product(x, sA, sB, w, m, g);
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
@@ -188,12 +207,7 @@ __global__ __launch_bounds__(256) void sweep_tiles_kernel(int n, const float* __
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < 16; ++ks)
-#pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-                mfma_acc(acc[nt], sA[16 * w + m][4 * ks + g], sB[4 * ks + g][16 * nt + m]);
-        mfma_retire();
+        lds_tile_product(acc, sA, sB, w, m, g);
         __syncthreads();
     }
 #pragma unroll
@@ -213,7 +227,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
                                                     i64 ldr, float* __restrict__ Y, float* __restrict__ Z,
                                                     float* __restrict__ out, i64 ldo, int* __restrict__ ctrl,
                                                     int* __restrict__ err) {
-    __shared__ float sV[2][kSwNB][kSwVP];
+    __shared__ float sV[2][kSwH][kSwVP];
     __shared__ int s_ord;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
     const int T = (int)ceil_div(n, kSwNB);
@@ -250,8 +264,8 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
         __syncthreads();
     };
     u64 gv[kSwVecQ];
-    auto vec_load = [&](const float* src, int bb) {  // a published block vector: coherent loads
-        const u64* p = reinterpret_cast<const u64*>(src + (i64)bb * kSwNB * kSwKP);
+    auto vec_load = [&](const float* src, int bb, int h) {  // half h of a published block vector: coherent loads
+        const u64* p = reinterpret_cast<const u64*>(src + ((i64)bb * kSwNB + h * kSwH) * kSwKP);
 #pragma unroll
         for (int q = 0; q < kSwVecQ; ++q) gv[q] = coherent_load(&p[t + 256 * q]);
     };
@@ -289,12 +303,40 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
             a[4 * jq + 3] = v.w;
         }
     };
-    auto product = [&](int buf, f4_t (&acc)[NT]) {
+    // acc += A-operand K rows h*kSwH .. (h+1)*kSwH x the half staged in
+    // sV[buf].  The MFMA statements are volatile, so the compiler keeps the
+    // LDS reads where the source puts them: they run kD K-steps ahead of
+    // their MFMAs through a ring of registers instead of each MFMA waiting
+    // out the LDS latency of its own operand.
+    auto product = [&](int buf, f4_t (&acc)[NT], int h) {
+        constexpr int kD = 6 / NT, kJ = kSwH / 4;
+        float bq[kD + 1][NT];
 #pragma unroll
-        for (int j = 0; j < kSwNB / 4; ++j)
+        for (int j = 0; j < kD; ++j)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) mfma_acc(acc[nt], a[j], sV[buf][4 * j + g][16 * nt + m]);
+            for (int nt = 0; nt < NT; ++nt) bq[j][nt] = sV[buf][4 * j + g][16 * nt + m];
+#pragma unroll
+        for (int j = 0; j < kJ; ++j) {
+            if (j + kD < kJ)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) bq[(j + kD) % (kD + 1)][nt] = sV[buf][4 * (j + kD) + g][16 * nt + m];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) mfma_acc(acc[nt], a[h * kJ + j], bq[j % (kD + 1)][nt]);
+        }
         mfma_retire();
+    };
+    // acc += A-operand rows x block bb of src, the two halves through the two
+    // LDS buffers (the second half loads while the first multiplies).  The
+    // caller has synchronised the workgroup since the last product.
+    auto block_product = [&](const float* src, int bb, f4_t (&acc)[NT]) {
+        vec_load(src, bb, 0);
+        vec_store(0);
+        __syncthreads();
+        vec_load(src, bb, 1);
+        product(0, acc, 0);
+        vec_store(1);
+        __syncthreads();
+        product(1, acc, 1);
     };
 
     // own right-hand sides, in the accumulator layout (row orow + v, column 16 nt + m)
@@ -314,16 +356,11 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
     f4_t acc[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
-    int buf = 0;
     if (nU > 0) a_from_factor(blk(0));
     for (int sp = 0; sp < nU; ++sp) {
         wait_all(yflag + sp * kSwR);
-        vec_load(Y, blk(sp));
-        vec_store(buf);
-        __syncthreads();
-        product(buf, acc);
+        block_product(Y, blk(sp), acc);
         if (sp + 1 < nU) a_from_factor(blk(sp + 1));
-        buf ^= 1;
     }
 
     // 2. z = r - u (rows past n are zero), published
@@ -339,23 +376,15 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
     // 3. y = X z + sum_i P(i) y_{s-i}
     a_from_tile(0);
     wait_all(zflag + s * kSwR);
-    vec_load(Z, b);
-    vec_store(buf);
-    __syncthreads();
     f4_t y[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) y[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
-    product(buf, y);
-    buf ^= 1;
+    block_product(Z, b, y);
     for (int i = kSwLook; i >= 1; --i) {
         if (s - i < 0) continue;
         a_from_tile(i);
         wait_all(yflag + (s - i) * kSwR);
-        vec_load(Y, blk(s - i));
-        vec_store(buf);
-        __syncthreads();
-        product(buf, y);
-        buf ^= 1;
+        block_product(Y, blk(s - i), y);
     }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)

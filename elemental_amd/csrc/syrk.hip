@@ -100,74 +100,84 @@ hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, con
 }
 
 
-// Hi/Lo bf16 planes (row-major, k contiguous, ldk a multiple of 32, zero
-// padded) of a float panel P (m x k, column-major): thread = (row, 8 k).
-__global__ __launch_bounds__(256) void split_bf16_kernel(int m, int k, const float* __restrict__ P, i64 ldp,
-                                                         unsigned short* __restrict__ Hi,
-                                                         unsigned short* __restrict__ Lo, i64 ldk) {
+// np bf16 planes (row-major, k contiguous, ldk a multiple of 32, zero padded;
+// plane p at planes + p*ps) of a float panel P (m x k, column-major): plane 0
+// = bf16(x), plane p = bf16 of what the earlier planes leave of x.
+// Thread = (row, 8 k).
+__global__ __launch_bounds__(256) void split_bf16_kernel(int np, int m, int k, const float* __restrict__ P, i64 ldp,
+                                                         unsigned short* __restrict__ planes, i64 ps, i64 ldk) {
     const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
     const int k0 = blockIdx.y * 8;
     if (r >= m) return;
-    s8_t h, l;
+    float x[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int kk = k0 + q;
-        const float x = kk < k ? P[r + (i64)kk * ldp] : 0.0f;
-        const unsigned short hb = f32_to_bf16_rne(x);
-        h[q] = (short)hb;
-        l[q] = (short)f32_to_bf16_rne(x - bf16_to_f32(hb));
+    for (int q = 0; q < 8; ++q) x[q] = k0 + q < k ? P[r + (i64)(k0 + q) * ldp] : 0.0f;
+    for (int p = 0; p < np; ++p) {
+        s8_t h;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const unsigned short hb = f32_to_bf16_rne(x[q]);
+            h[q] = (short)hb;
+            x[q] -= bf16_to_f32(hb);
+        }
+        *reinterpret_cast<s8_t*>(planes + p * ps + r * ldk + k0) = h;
     }
-    *reinterpret_cast<s8_t*>(Hi + r * ldk + k0) = h;
-    *reinterpret_cast<s8_t*>(Lo + r * ldk + k0) = l;
 }
 
-__global__ __launch_bounds__(256, 2) void syrk_ln_bf16x3p_kernel(int m, int n, int k, float alpha,
-        const unsigned short* __restrict__ Hi, const unsigned short* __restrict__ Lo, i64 ldk, float beta,
-        float* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
+template <int NP>
+__global__ __launch_bounds__(256, 2) void syrk_ln_bf16p_kernel(int m, int n, int k, float alpha,
+        const unsigned short* __restrict__ planes, i64 ps, i64 ldk, float beta, float* __restrict__ C, i64 ldc,
+        const int* __restrict__ info, int kcol0, int nbp) {
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
     if (info != nullptr && ke == 0) return;
     int ti, tj;
     if (!syrk_tile(m, n, ti, tj)) return;
-    syrk_tile_bf16x3_planes(m, n, ke, alpha, Hi, Lo, ldk, Hi, Lo, beta, C, ldc, ti, tj);
+    syrk_tile_bf16_planes<NP>(m, n, ke, alpha, planes, ps, ldk, beta, C, ldc, ti, tj);
 }
 
-hipError_t launch_split_bf16(int m, int k, const float* P, i64 ldp, unsigned short* Hi, unsigned short* Lo, i64 ldk,
+hipError_t launch_split_bf16(int np, int m, int k, const float* P, i64 ldp, unsigned short* planes, i64 ps, i64 ldk,
                              hipStream_t s) {
     if (m <= 0 || k <= 0) return hipSuccess;
-    hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)(ldk / 8)), dim3(256), 0, s, m,
-                       k, P, ldp, Hi, Lo, ldk);
+    if (np < 1 || np > 3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)(ldk / 8)), dim3(256), 0, s, np,
+                       m, k, P, ldp, planes, ps, ldk);
     return hipGetLastError();
 }
 
-hipError_t launch_syrk_ln_bf16x3_planes(int m, int n, int k, float alpha, const unsigned short* Hi,
-                                        const unsigned short* Lo, i64 ldk, float beta, float* C, i64 ldc,
-                                        const int* info, int kcol0, int nbp, hipStream_t s) {
+hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, const unsigned short* planes, i64 ps,
+                                      i64 ldk, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
+                                      hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(syrk_ln_bf16x3p_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, Hi, Lo, ldk, beta, C,
-                       ldc, info, kcol0, nbp);
+    if (np == 2)
+        hipLaunchKernelGGL(syrk_ln_bf16p_kernel<2>, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, planes, ps, ldk,
+                           beta, C, ldc, info, kcol0, nbp);
+    else if (np == 3)
+        hipLaunchKernelGGL(syrk_ln_bf16p_kernel<3>, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, planes, ps, ldk,
+                           beta, C, ldc, info, kcol0, nbp);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 // One block column of the trailing matrix from pre-split planes (the
 // in-group update of the float factorization): C (m x 128, rows r >= c of
 // its top 128 x 128 block and every row below) := C - P P_top^T, P = the m x k
-// panel rows of the planes (hi/lo bf16, row-major, ld ldk; P_top = its first
-// 128 rows).  One 64-row slab per workgroup, wave w owns 16 rows and all 128
-// columns; the fragments are 16-byte loads straight from the planes into
-// registers (the planes' row-major order is the MFMA operand order), the
-// next 32-wide k chunk loading while the current one multiplies.  The
+// panel rows of the NP planes (row-major, ld ldk, plane stride ps; P_top = its
+// first 128 rows).  One 64-row slab per workgroup, wave w owns 16 rows and
+// all 128 columns; the fragments are 16-byte loads straight from the planes
+// into registers (the planes' row-major order is the MFMA operand order),
+// the next 32-wide k chunk loading while the current one multiplies.  The
 // accumulators start from C and the slab's operand is negated (sign bits).
-__global__ __launch_bounds__(256) void syrk_col_bf16x3p_kernel(int m, int k, const unsigned short* __restrict__ Hi,
-                                                               const unsigned short* __restrict__ Lo, i64 ldk,
-                                                               float* __restrict__ C, i64 ldc,
-                                                               const int* __restrict__ info, int kcol0, int nbp) {
+template <int NP>
+__global__ __launch_bounds__(256) void syrk_col_bf16p_kernel(int m, int k, const unsigned short* __restrict__ planes,
+                                                             i64 ps, i64 ldk, float* __restrict__ C, i64 ldc,
+                                                             const int* __restrict__ info, int kcol0, int nbp) {
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
     if (info != nullptr && ke == 0) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
     const int r0 = blockIdx.x * 64 + 16 * w;                   // this wave's rows
     const int ra = r0 + m16 < m ? r0 + m16 : m - 1;            // A-operand row (clamped; its outputs are dropped)
-    const unsigned short* ah_p = Hi + (i64)ra * ldk + 8 * g;
-    const unsigned short* al_p = Lo + (i64)ra * ldk + 8 * g;
+    const unsigned short* a_p = planes + (i64)ra * ldk + 8 * g;
     const s8_t neg = {(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
                       (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000};
     f4_t acc[8];
@@ -178,24 +188,24 @@ __global__ __launch_bounds__(256) void syrk_col_bf16x3p_kernel(int m, int k, con
             const int r = r0 + 4 * g + v, c = 16 * nt + m16;
             acc[nt][v] = (r < m && (r >= 128 || c <= r)) ? C[r + (i64)c * ldc] : 0.0f;
         }
-    s8_t fa[2][2], fb[2][8][2];  // [buffer][hi/lo], [buffer][column tile][hi/lo]
+    s8_t fa[2][NP], fb[2][8][NP];  // [buffer][plane], [buffer][column tile][plane]
     auto load = [&](int buf, int kc) {
-        fa[buf][0] = *reinterpret_cast<const s8_t*>(ah_p + kc) ^ neg;
-        fa[buf][1] = *reinterpret_cast<const s8_t*>(al_p + kc) ^ neg;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) fa[buf][p] = *reinterpret_cast<const s8_t*>(a_p + p * ps + kc) ^ neg;
 #pragma unroll
         for (int nt = 0; nt < 8; ++nt) {
             const i64 rb = (i64)(16 * nt + m16 < m ? 16 * nt + m16 : m - 1) * ldk + 8 * g + kc;
-            fb[buf][nt][0] = *reinterpret_cast<const s8_t*>(Hi + rb);
-            fb[buf][nt][1] = *reinterpret_cast<const s8_t*>(Lo + rb);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) fb[buf][nt][p] = *reinterpret_cast<const s8_t*>(planes + p * ps + rb);
         }
     };
     auto step = [&](int buf) {
 #pragma unroll
-        for (int nt = 0; nt < 8; ++nt) {
-            mfma_acc_bf16(acc[nt], fa[buf][1], fb[buf][nt][0]);
-            mfma_acc_bf16(acc[nt], fa[buf][0], fb[buf][nt][1]);
-            mfma_acc_bf16(acc[nt], fa[buf][0], fb[buf][nt][0]);
-        }
+        for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+            for (int o = NP - 1; o >= 0; --o)  // term order o = p + q, smallest first
+#pragma unroll
+                for (int p = o; p >= 0; --p) mfma_acc_bf16(acc[nt], fa[buf][p], fb[buf][nt][o - p]);
     };
     // ldk pads the planes with zeros up to a multiple of 32, so whole chunks are read
     const int nchunks = (ke + 31) / 32;
@@ -218,11 +228,17 @@ __global__ __launch_bounds__(256) void syrk_col_bf16x3p_kernel(int m, int k, con
         }
 }
 
-hipError_t launch_syrk_col_bf16x3_planes(int m, int k, const unsigned short* Hi, const unsigned short* Lo, i64 ldk,
-                                         float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
+hipError_t launch_syrk_col_bf16_planes(int np, int m, int k, const unsigned short* planes, i64 ps, i64 ldk, float* C,
+                                       i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
     if (m <= 0 || k <= 0) return hipSuccess;
-    hipLaunchKernelGGL(syrk_col_bf16x3p_kernel, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, k, Hi, Lo, ldk,
-                       C, ldc, info, kcol0, nbp);
+    if (np == 2)
+        hipLaunchKernelGGL(syrk_col_bf16p_kernel<2>, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, k, planes,
+                           ps, ldk, C, ldc, info, kcol0, nbp);
+    else if (np == 3)
+        hipLaunchKernelGGL(syrk_col_bf16p_kernel<3>, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, k, planes,
+                           ps, ldk, C, ldc, info, kcol0, nbp);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

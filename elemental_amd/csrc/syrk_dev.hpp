@@ -343,50 +343,54 @@ __device__ __forceinline__ void syrk_tile_bf16x3(int m, int n, int k, float alph
 }
 
 
-// Split-BF16x3 tile from pre-split panels: Hi/Lo hold bf16(x) and
-// bf16(x - bf16(x)) of the panel, row-major with ldk >= k (k contiguous, rows
-// of the panel = rows of C).  The split is done once per pass over the
-// trailing matrix instead of once per tile; the tile only moves 16-byte
-// groups of 8 bf16 from memory to LDS.  Same arithmetic as syrk_tile_bf16x3.
-__device__ __forceinline__ void syrk_tile_bf16x3_planes(int m, int n, int k, float alpha,
-        const unsigned short* __restrict__ Hi, const unsigned short* __restrict__ Lo, i64 ldk,
-        const unsigned short* __restrict__ HiJ, const unsigned short* __restrict__ LoJ, float beta,
-        float* __restrict__ C, i64 ldc, int ti, int tj) {
-    __shared__ __attribute__((aligned(16))) unsigned short sIh[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sIl[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sJh[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sJl[kSyrkTile][kBfLd];
+// Split-BF16 tile from pre-split panels.  The panel is stored as NP bf16
+// planes, plane p at planes + p*ps: plane 0 = bf16(x), plane p = bf16 of
+// what planes 0..p-1 leave of x (row-major, ldk >= k, k contiguous, rows of
+// the panel = rows of C).  NP = 2 is split-BF16x3 (hi*hi + hi*lo + lo*hi,
+// the terms down to 2^-16 of a product), NP = 3 split-BF16x6 (the six terms
+// p + q <= 2, down to 2^-24).  The bf16 MFMA sums its 32 products without
+// rounding bits below the largest product's 24-bit window (bits of smaller
+// products there are dropped, the accumulator add itself rounds to nearest),
+// so split-BF16x6 lands between split-BF16x3 and FP32 MFMA accuracy; the
+// drop is biased toward zero per product and grows with k.  The split is done once per
+// pass over the trailing matrix instead of once per tile; the tile only
+// moves 16-byte groups of 8 bf16 from memory to LDS.  Smallest terms first.
+template <int NP>
+__device__ __forceinline__ void syrk_tile_bf16_planes(int m, int n, int k, float alpha,
+        const unsigned short* __restrict__ planes, i64 ps, i64 ldk, float beta, float* __restrict__ C, i64 ldc,
+        int ti, int tj) {
+    __shared__ __attribute__((aligned(16))) unsigned short sI[NP][kSyrkTile][kBfLd];
+    __shared__ __attribute__((aligned(16))) unsigned short sJ[NP][kSyrkTile][kBfLd];
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
     const bool skip = (ti == tj) && (wj > wi);
     const int lr = tid & 127, kh = tid >> 7;  // row, k half (16 values = 2 x 8)
     const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
-    const unsigned short* hI = Hi + (i64)(rowI ? i0 + lr : 0) * ldk;
-    const unsigned short* lI = Lo + (i64)(rowI ? i0 + lr : 0) * ldk;
-    const unsigned short* hJ = HiJ + (i64)(rowJ ? j0 + lr : 0) * ldk;
-    const unsigned short* lJ = LoJ + (i64)(rowJ ? j0 + lr : 0) * ldk;
-    s8_t g[8];  // I hi, I lo, J hi, J lo; two groups of 8 each
+    const unsigned short* pI = planes + (i64)(rowI ? i0 + lr : 0) * ldk;
+    const unsigned short* pJ = planes + (i64)(rowJ ? j0 + lr : 0) * ldk;
+    s8_t gI[NP][2], gJ[NP][2];  // [plane][group of 8]
     const s8_t zero = {0, 0, 0, 0, 0, 0, 0, 0};
     auto gload = [&](int kc0) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int kk = kc0 + kh * 16 + 8 * h;
             const bool kin = kk < k;  // ldk pads to a multiple of 32 with zeros
-            g[0 + h] = (rowI && kin) ? *reinterpret_cast<const s8_t*>(hI + kk) : zero;
-            g[2 + h] = (rowI && kin) ? *reinterpret_cast<const s8_t*>(lI + kk) : zero;
-            g[4 + h] = (rowJ && kin) ? *reinterpret_cast<const s8_t*>(hJ + kk) : zero;
-            g[6 + h] = (rowJ && kin) ? *reinterpret_cast<const s8_t*>(lJ + kk) : zero;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                gI[p][h] = (rowI && kin) ? *reinterpret_cast<const s8_t*>(pI + p * ps + kk) : zero;
+                gJ[p][h] = (rowJ && kin) ? *reinterpret_cast<const s8_t*>(pJ + p * ps + kk) : zero;
+            }
         }
     };
     auto sstore = [&]() {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            *reinterpret_cast<s8_t*>(&sIh[lr][kh * 16 + 8 * h]) = g[0 + h];
-            *reinterpret_cast<s8_t*>(&sIl[lr][kh * 16 + 8 * h]) = g[2 + h];
-            *reinterpret_cast<s8_t*>(&sJh[lr][kh * 16 + 8 * h]) = g[4 + h];
-            *reinterpret_cast<s8_t*>(&sJl[lr][kh * 16 + 8 * h]) = g[6 + h];
-        }
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                *reinterpret_cast<s8_t*>(&sI[p][lr][kh * 16 + 8 * h]) = gI[p][h];
+                *reinterpret_cast<s8_t*>(&sJ[p][lr][kh * 16 + 8 * h]) = gJ[p][h];
+            }
     };
     // C - P P^T (the factorization): accumulate onto C, read while the first
     // chunk loads, with the J operands negated (sign bits flipped, exact)
@@ -410,6 +414,8 @@ __device__ __forceinline__ void syrk_tile_bf16x3_planes(int m, int n, int k, flo
 #pragma unroll
             for (int b = 0; b < 4; ++b) acc[a][b] = (f4_t){0.f, 0.f, 0.f, 0.f};
     }
+    const s8_t neg = {(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
+                      (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000};
     const int nchunks = (int)ceil_div(k, kBfKC);
     gload(0);
     for (int c = 0; c < nchunks; ++c) {
@@ -419,36 +425,31 @@ __device__ __forceinline__ void syrk_tile_bf16x3_planes(int m, int n, int k, flo
         if (c + 1 < nchunks) gload((c + 1) * kBfKC);
         if (!skip) {
             const int kofs = 8 * (lane >> 4);
-            s8_t ah[4], al[4], bh[4], bl[4];
+            s8_t fa[NP][4], fb[NP][4];
 #pragma unroll
-            for (int mb = 0; mb < 4; ++mb) {
-                const int r = wj + 16 * mb + (lane & 15);
-                ah[mb] = *reinterpret_cast<const s8_t*>(&sJh[r][kofs]);
-                al[mb] = *reinterpret_cast<const s8_t*>(&sJl[r][kofs]);
-                if (acc_c) {  // flip the bf16 sign bits
-                    ah[mb] ^= (s8_t){(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
-                                     (short)0x8000, (short)0x8000, (short)0x8000};
-                    al[mb] ^= (s8_t){(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
-                                     (short)0x8000, (short)0x8000, (short)0x8000};
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int mb = 0; mb < 4; ++mb) {
+                    fa[p][mb] = *reinterpret_cast<const s8_t*>(&sJ[p][wj + 16 * mb + (lane & 15)][kofs]);
+                    if (acc_c) fa[p][mb] ^= neg;  // flip the bf16 sign bits
                 }
-            }
 #pragma unroll
-            for (int nb = 0; nb < 4; ++nb) {
-                const int r = wi + 16 * nb + (lane & 15);
-                bh[nb] = *reinterpret_cast<const s8_t*>(&sIh[r][kofs]);
-                bl[nb] = *reinterpret_cast<const s8_t*>(&sIl[r][kofs]);
-            }
+            for (int p = 0; p < NP; ++p)
+#pragma unroll
+                for (int nb = 0; nb < 4; ++nb)
+                    fb[p][nb] = *reinterpret_cast<const s8_t*>(&sI[p][wi + 16 * nb + (lane & 15)][kofs]);
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
                 for (int nb = 0; nb < 4; ++nb) {
                     f4_t a = acc[mb][nb];
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, al[mb]),
-                                                                __builtin_bit_cast(bf8_t, bh[nb]), a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, ah[mb]),
-                                                                __builtin_bit_cast(bf8_t, bl[nb]), a, 0, 0, 0);
-                    a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, ah[mb]),
-                                                                __builtin_bit_cast(bf8_t, bh[nb]), a, 0, 0, 0);
+#pragma unroll
+                    for (int o = NP - 1; o >= 0; --o)  // term order o = p + q, smallest first
+#pragma unroll
+                        for (int p = o; p >= 0; --p)
+                            a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8_t, fa[p][mb]),
+                                                                        __builtin_bit_cast(bf8_t, fb[o - p][nb]), a,
+                                                                        0, 0, 0);
                     acc[mb][nb] = a;
                 }
         }

@@ -1,6 +1,6 @@
 """Mixed-precision HPD solve with FP64 iterative refinement (no reference
 counterpart): the low-precision factor (FP64 diagonal potrf, FP32-stored
-panels, FP32 or split-BF16x3 trailing update) is refined to the FP64
+panels, FP32, split-BF16x3 or split-BF16x6 trailing update) is refined to the FP64
 normwise backward-error target, falling back to GMRES-IR."""
 from __future__ import annotations
 
@@ -9,7 +9,7 @@ import dataclasses
 
 from ._lib import IROpts, IRReportC, lib
 
-TRAILING = {"f32": 1, "bf16x3": 2}
+TRAILING = {"f32": 1, "bf16x3": 2, "bf16x6": 3}
 
 
 @dataclasses.dataclass

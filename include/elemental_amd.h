@@ -44,6 +44,7 @@ extern "C" {
 #define ELEMAMD_TRAILING_F64 0
 #define ELEMAMD_TRAILING_F32 1
 #define ELEMAMD_TRAILING_BF16X3 2
+#define ELEMAMD_TRAILING_BF16X6 3
 
 /* Library version string. */
 const char *elemamd_version(void);
@@ -81,10 +82,13 @@ int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const do
                      double beta, double *C, int64_t ldc, void *stream);
 
 /* Float-storage trailing update of the mixed-precision factor (no reference
- * counterpart): mode ELEMAMD_TRAILING_F32 (FP32 MFMA) or
+ * counterpart): mode ELEMAMD_TRAILING_F32 (FP32 MFMA),
  * ELEMAMD_TRAILING_BF16X3 (split-BF16x3, FP32 accumulation: A is split into
  * bf16 hi/lo planes in a per-stream scratch buffer, then the three products
- * hi*hi + hi*lo + lo*hi accumulate in FP32). */
+ * hi*hi + hi*lo + lo*hi accumulate in FP32) or ELEMAMD_TRAILING_BF16X6
+ * (three planes hi/mid/lo and the six products down to 2^-24 of a term;
+ * the bf16 MFMA drops the bits of small products below the largest one's
+ * 24-bit window, so the result is less accurate than FP32's). */
 int elemamd_herk_f32(int mode, int n, int k, float alpha, const float *A, int64_t lda, float beta, float *C,
                      int64_t ldc, void *stream);
 
@@ -133,7 +137,7 @@ int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double *A, 
 
 typedef struct {
     int nb;               /* factorization block size, 1..128 (0 -> 128) */
-    int trailing;         /* ELEMAMD_TRAILING_F32 or ELEMAMD_TRAILING_BF16X3 */
+    int trailing;         /* ELEMAMD_TRAILING_F32, _BF16X3 or _BF16X6 */
     int max_iter;         /* refinement steps before giving up (0 -> 30) */
     int gmres_restart;    /* Krylov dimension of a GMRES-IR cycle: 0 -> 20, negative disables the fallback */
     int force_gmres;      /* 1: use GMRES-IR from the first correction */

@@ -114,10 +114,11 @@ def test_herk_f64_integer_exact():
     assert np.array_equal(np.triu(got, 1), np.triu(C, 1))
 
 
-@pytest.mark.parametrize("mode,tol", [(1, 2e-6), (2, 2e-4)])
+@pytest.mark.parametrize("mode,tol", [(1, 2e-6), (2, 2e-4), (3, 2e-6)])
 def test_herk_lowprec(oracle, mode, tol):
-    # FP32 MFMA (mode 1) and split-BF16x3 (mode 2) against the FP64 product;
-    # tolerance relative to k*max|a|^2: ~ float eps and ~2^-16 respectively
+    # FP32 MFMA (mode 1), split-BF16x3 (mode 2) and split-BF16x6 (mode 3)
+    # against the FP64 product; tolerance relative to k*max|a|^2: ~ float
+    # eps, ~2^-16 and ~ float eps respectively
     n, k = 700, 128
     A = oracle.normal((n, k), seed=5)
     C = oracle.normal((n, n), seed=6)
@@ -271,7 +272,8 @@ def _eta_inf(A, X, B, oracle):
 
 
 @pytest.mark.parametrize("trailing,kappa,gmres", [("f32", 1e6, False), ("bf16x3", 1e4, False),
-                                                  ("f32", 1e6, True), ("bf16x3", 1e6, False)])
+                                                  ("f32", 1e6, True), ("bf16x3", 1e6, False),
+                                                  ("bf16x6", 1e6, False)])
 def test_mixed_ir_reaches_fp64_backward_error(oracle, trailing, kappa, gmres):
     n, k = 1536, 8
     A = inputs.spectrum_host(n, kappa, seed=21)
@@ -290,6 +292,23 @@ def test_mixed_ir_reaches_fp64_backward_error(oracle, trailing, kappa, gmres):
     assert oracle.hpd_solve(Lr, Xr, 128, host_blas=True) == 0
     fwd = np.linalg.norm(X - Xr) / np.linalg.norm(Xr)
     assert fwd <= 100 * kappa * EPS
+
+
+def test_mixed_bf16x6_between_bf16x3_and_fp32():
+    """Split-BF16x6 keeps every product term down to 2^-24, split-BF16x3 the
+    terms down to 2^-16: the x6 factor is the better preconditioner, so
+    refinement takes fewer steps than with x3, and no fewer than with the
+    FP32 MFMA (the bf16 MFMA's truncated product sum keeps x6 short of FP32)."""
+    n, k = 2048, 4
+    A = inputs.spectrum_host(n, 1e6, seed=41)
+    B = inputs.normal_host(n, k, seed=42)
+    its = {}
+    for trailing in ("f32", "bf16x3", "bf16x6"):
+        Ad, Bd = dev(A), dev(B)
+        rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing=trailing, gmres_restart=-1))
+        assert rep.converged and not rep.used_gmres, (trailing, rep)
+        its[trailing] = rep.iterations
+    assert its["f32"] <= its["bf16x6"] < its["bf16x3"], its
 
 
 @pytest.mark.parametrize("n,k", [(64, 1), (200, 3), (256, 17), (777, 33), (1300, 40), (2048, 32)])
