@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--kappa", type=float, default=1e6)
     p.add_argument("--nb", type=int, default=128)
     p.add_argument("--trailing", default="bf16x3", choices=["f32", "bf16x3", "bf16x6"])
+    p.add_argument("--refine", default="cg", choices=["ir", "cg"],
+                   help="refinement of the mixed solve: classic IR or preconditioned CG")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline")
     p.add_argument("--cpu-runs", type=int, default=3)
@@ -487,7 +489,7 @@ def main():
 
     Awork = E.Matrix(n, n)
     X = E.Matrix(n, k)
-    opts = E.IROptions(nb=args.nb, trailing=args.trailing)
+    opts = E.IROptions(nb=args.nb, trailing=args.trailing, refine=args.refine)
     from elemental_amd.mixed import workspace_bytes
 
     ws = torch.empty(workspace_bytes(n, k, opts), dtype=torch.uint8, device=dev)
@@ -567,7 +569,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"HPD factor+solve n={n} k={k} kappa={args.kappa:g} (A=Q diag(s) Q^T, log-spaced s)",
-            "n": n, "k": k, "kappa": args.kappa, "nb": args.nb, "trailing": args.trailing,
+            "n": n, "k": k, "kappa": args.kappa, "nb": args.nb, "trailing": args.trailing, "refine": args.refine,
             "parallelism": "single GPU",
         },
         "mixed_ir": {

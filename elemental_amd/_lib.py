@@ -37,6 +37,7 @@ class IROpts(ctypes.Structure):
         ("force_gmres", ctypes.c_int),
         ("allow_shift", ctypes.c_int),
         ("tol", ctypes.c_double),
+        ("refine", ctypes.c_int),
     ]
 
 

@@ -469,4 +469,44 @@ hipError_t launch_axpy_f64(int m, int n, double alpha, const double* X, i64 ldx,
     return hipGetLastError();
 }
 
+// ------------------------------------------- preconditioned CG refinement
+// Per column c (all n x k blocks with ld n, scalars length k):
+//   alpha = rz/pq (0 for an inactive column or a non-positive pq),
+//   X += alpha P,  R -= alpha Q.
+__global__ void cg_xr_kernel(int m, const double* __restrict__ rz, const double* __restrict__ pq,
+                             const int* __restrict__ active, const double* __restrict__ P,
+                             const double* __restrict__ Q, double* __restrict__ X, double* __restrict__ R) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= m) return;
+    const double d = pq[j], a = (active[j] && d > 0.0 && d < INFINITY) ? rz[j] / d : 0.0;
+    const i64 e = i + (i64)j * m;
+    X[e] += a * P[e];
+    R[e] -= a * Q[e];
+}
+hipError_t launch_cg_xr(int m, int n, const double* rz, const double* pq, const int* active, const double* P,
+                        const double* Q, double* X, double* R, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cg_xr_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s, m, rz, pq, active,
+                       P, Q, X, R);
+    return hipGetLastError();
+}
+
+// P := Z + beta P with the flexible (Polak-Ribiere) coefficient
+// beta = -z.q / p.q (Q = A P of the step just taken); beta = 0 restarts.
+__global__ void cg_p_kernel(int m, const double* __restrict__ zq, const double* __restrict__ pq,
+                            const double* __restrict__ Z, double* __restrict__ P) {
+    const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= m) return;
+    const double d = pq[j], b = (d > 0.0 && d < INFINITY) ? -zq[j] / d : 0.0;
+    const i64 e = i + (i64)j * m;
+    P[e] = Z[e] + b * P[e];
+}
+hipError_t launch_cg_p(int m, int n, const double* zq, const double* pq, const double* Z, double* P, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(cg_p_kernel, dim3((unsigned)ceil_div(m, 256), (unsigned)n), dim3(256), 0, s, m, zq, pq, Z, P);
+    return hipGetLastError();
+}
+
 }  // namespace elemamd

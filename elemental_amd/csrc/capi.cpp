@@ -162,6 +162,7 @@ static MixedOpts to_opts(const elemamd_ir_opts* o) {
         m.force_gmres = o->force_gmres;
         m.allow_shift = o->allow_shift;
         m.tol = o->tol > 0.0 ? o->tol : 1e-14;
+        m.refine = o->refine;
     }
     return m;
 }
@@ -177,6 +178,7 @@ int elemamd_hpd_solve_mixed_f64(int n, int k, const double* A, int64_t lda, doub
     MixedOpts o = to_opts(opts);
     if (o.trailing != kTrailingF32 && o.trailing != kTrailingBf16x3 && o.trailing != kTrailingBf16x6)
         return ELEMAMD_ERR_ARG;
+    if (o.refine != ELEMAMD_REFINE_IR && o.refine != ELEMAMD_REFINE_CG) return ELEMAMD_ERR_ARG;
     MixedReport r{};
     int rc = hpd_solve_mixed(n, k, A, lda, B, ldb, o, work, work_bytes, &r, S(stream));
     if (report) {

@@ -141,6 +141,11 @@ hipError_t launch_col_scale(int n, int k, const double* Y, i64 ldy, const double
 hipError_t launch_copy_f32_to_f64_scaled(int m, int n, const float* D, i64 ldd, const double* colscale, double* Y,
                                          i64 ldy, hipStream_t s);
 hipError_t launch_axpy_f64(int m, int n, double alpha, const double* X, i64 ldx, double* Y, i64 ldy, hipStream_t s);
+// Preconditioned CG refinement steps (blocks n-row, ld m): X += alpha P and
+// R -= alpha Q with alpha = rz/pq per active column; P := Z - (zq/pq) P.
+hipError_t launch_cg_xr(int m, int n, const double* rz, const double* pq, const int* active, const double* P,
+                        const double* Q, double* X, double* R, hipStream_t s);
+hipError_t launch_cg_p(int m, int n, const double* zq, const double* pq, const double* Z, double* P, hipStream_t s);
 hipError_t launch_shift_diag(int n, const double* A, i64 lda, float* S, i64 lds, double sigma, hipStream_t s);
 
 }  // namespace elemamd

@@ -240,10 +240,14 @@ struct MixedWork {
     double* h;     // (m+1) x k Gram-Schmidt coefficients
     void* sweep;   // block vectors and flags of the correction sweeps
     double* part;  // partial sums of the split symmetric product
+    double* P;     // n x k CG search directions
+    double* Q;     // n x k A P
+    double* Z;     // n x k preconditioned residual
+    int* active;   // k CG column flags (0: converged, frozen)
     size_t bytes;
 };
 
-MixedWork carve_refine(void* work, size_t cap, int n, int k, int m) {
+MixedWork carve_refine(void* work, size_t cap, int n, int k, int m, bool cg) {
     Carve c(work, cap);
     MixedWork w;
     const size_t nk = (size_t)n * k;
@@ -257,6 +261,10 @@ MixedWork carve_refine(void* work, size_t cap, int n, int k, int m) {
     w.h = c.take<double>((size_t)(m + 2) * k);
     w.sweep = c.take<char>(sweep_work_bytes(n));
     w.part = c.take<double>(symm_part_count(n, k));
+    w.P = c.take<double>(cg ? nk : 0);
+    w.Q = c.take<double>(cg ? nk : 0);
+    w.Z = c.take<double>(cg ? nk : 0);
+    w.active = c.take<int>(cg ? (size_t)k : 0);
     w.bytes = c.off + 256;
     return w;
 }
@@ -399,6 +407,103 @@ int gmres_cycle(Ctx& c, int m, double inner_tol, int* iters_out) {
     return kOk;
 }
 
+// Normwise backward error per column from the device column maxima of R
+// (dev[0..k)) and X (dev[k..2k)), copied to host; returns the maximum.
+int host_eta(Ctx& c, const std::vector<double>& bn, std::vector<double>& eta_c, double& eta) {
+    const int n = c.n, k = c.k;
+    std::vector<double> nx(2 * (size_t)k);
+    ELEMAMD_HIP_TRY(launch_col_absmax(n, k, c.w.R, n, c.w.nrm + k, c.s));
+    ELEMAMD_HIP_TRY(launch_col_absmax(n, k, c.w.X, n, c.w.nrm + 2 * k, c.s));
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(nx.data(), c.w.nrm + k, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, c.s));
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(c.s));
+    eta_c.resize(k);
+    eta = 0.0;
+    for (int col = 0; col < k; ++col) {
+        const double den = c.anorm * nx[k + col] + bn[col];
+        const double e = (den > 0.0) ? nx[col] / den : (nx[col] > 0.0 ? INFINITY : 0.0);
+        eta_c[col] = std::isnan(e) ? INFINITY : e;
+        eta = std::max(eta, eta_c[col]);
+    }
+    return kOk;
+}
+
+// Preconditioned conjugate gradients on A X = B, one independent recurrence
+// per column, from X = 0, R = B: the preconditioner is the low-precision
+// factor (the correction sweeps, in float), everything else FP64.  The
+// preconditioner's rounding differs from call to call, so the direction
+// update uses the flexible coefficient beta = -z_new.q / p.q (Q = A P),
+// which equals the Fletcher-Reeves one for a fixed preconditioner.  When the
+// recursive residual meets the target the true residual B - A X is formed;
+// columns still short of it restart from it.  Columns that reach the target
+// are frozen.  Returns with *conv = 1 when every column's true backward
+// error is <= tol, otherwise (stagnation, step limit, breakdown) with X
+// holding the best iterate so far for classic refinement to continue from.
+int cg_refine(Ctx& c, const MixedOpts& o, const std::vector<double>& bn, MixedReport& r, double& eta, int* conv) {
+    const int n = c.n, k = c.k;
+    const size_t nk = (size_t)n * k;
+    MixedWork& w = c.w;
+    hipStream_t s = c.s;
+    double* rz = w.nrm + 3 * k;
+    double* pq = w.nrm + 4 * k;
+    double* zq = w.nrm + 5 * k;
+    *conv = 0;
+    std::vector<int> act(k, 1), act_dev(k, -1);
+    std::vector<double> eta_c;
+    auto push_active = [&]() -> hipError_t {
+        if (act == act_dev) return hipSuccess;
+        act_dev = act;
+        return hipMemcpyAsync(w.active, act_dev.data(), sizeof(int) * k, hipMemcpyHostToDevice, s);
+    };
+    ELEMAMD_HIP_TRY(launch_col_absmax(n, k, w.R, n, w.nrm + k, s));  // preconditioner scale of R = B
+    bool restart = true;
+    double best = INFINITY;
+    int since_best = 0;
+    for (;;) {
+        ELEMAMD_HIP_TRY(push_active());
+        if (restart) {  // P = Z = M^{-1} R
+            int rc = precond_apply(c, w.R, w.Z, w.nrm + k);
+            if (rc) return rc;
+            ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.R, n, (i64)nk, w.Z, n, rz, s));
+            ELEMAMD_HIP_TRY(launch_copy_f64(n, k, w.Z, n, w.P, n, s));
+            restart = false;
+        }
+        ELEMAMD_HIP_TRY(launch_symm_lower(n, k, c.A, c.lda, w.P, n, w.Q, n, w.part, s));
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.P, n, (i64)nk, w.Q, n, pq, s));
+        ELEMAMD_HIP_TRY(launch_cg_xr(n, k, rz, pq, w.active, w.P, w.Q, w.X, w.R, s));
+        r.iterations += 1;
+        int rc = host_eta(c, bn, eta_c, eta);
+        if (rc) return rc;
+        if (!(eta < INFINITY)) return kOk;  // breakdown: classic refinement takes over
+        if (eta <= o.tol) {  // confirm with the true residual
+            ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, c.A, c.lda, w.X, n, c.Bin, c.ldb, w.R, n, w.part, s));
+            rc = host_eta(c, bn, eta_c, eta);
+            if (rc) return rc;
+            if (eta <= o.tol) {
+                *conv = 1;
+                return kOk;
+            }
+            for (int col = 0; col < k; ++col) act[col] = eta_c[col] > o.tol;
+            restart = true;
+            if (r.iterations >= o.max_iter) return kOk;
+            continue;
+        }
+        if (r.iterations >= o.max_iter) return kOk;
+        if (eta < 0.5 * best) {
+            best = eta;
+            since_best = 0;
+        } else if (++since_best >= 3) {
+            return kOk;  // stagnation
+        }
+        for (int col = 0; col < k; ++col)
+            if (eta_c[col] <= o.tol) act[col] = 0;
+        rc = precond_apply(c, w.R, w.Z, w.nrm + k);
+        if (rc) return rc;
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.Z, n, (i64)nk, w.Q, n, zq, s));
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.R, n, (i64)nk, w.Z, n, rz, s));
+        ELEMAMD_HIP_TRY(launch_cg_p(n, k, zq, pq, w.Z, w.P, s));
+    }
+}
+
 }  // namespace
 
 MixedFactor carve_factor(void* mem, int n) {
@@ -424,7 +529,9 @@ size_t mixed_factor_bytes(int n) {
     return c.off + 256;
 }
 
-size_t mixed_refine_bytes(int n, int k, const MixedOpts& o) { return carve_refine(nullptr, 0, n, k, o.gmres_restart).bytes; }
+size_t mixed_refine_bytes(int n, int k, const MixedOpts& o) {
+    return carve_refine(nullptr, 0, n, k, o.gmres_restart, o.refine == kRefineCG).bytes;
+}
 
 size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o) {
     return mixed_factor_bytes(n) + mixed_refine_bytes(n, k, o);
@@ -510,7 +617,7 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
         return kOk;
     }
     const int m = o.gmres_restart;
-    MixedWork w = carve_refine(work, work_bytes, n, k, m);
+    MixedWork w = carve_refine(work, work_bytes, n, k, m, o.refine == kRefineCG);
     if (work == nullptr || work_bytes < w.bytes) return kErrWorkspace;
     Ctx c{n, k, A, lda, B, ldb, w, &f, s, f.anorm};
     std::vector<double> bn, xn, rn;
@@ -526,10 +633,22 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
     double prev = INFINITY;
     int stall = 0;
     double eta = INFINITY;
-    for (int it = 0; it <= o.max_iter; ++it) {
+    int it0 = 0, cg_conv = 0;
+    if (o.refine == kRefineCG && !gmres) {
+        int rc = cg_refine(c, o, bn, r, eta, &cg_conv);
+        if (rc) return rc;
+        if (eta < INFINITY) {
+            it0 = 1;  // classic refinement continues from the CG iterate
+        } else {      // breakdown: classic refinement from X = 0
+            ELEMAMD_HIP_TRY(launch_copy_f64(n, k, B, ldb, w.R, n, s));
+            ELEMAMD_HIP_TRY(launch_scale_f64(n, k, 0.0, w.X, n, s));
+        }
+    }
+    const int base = r.iterations;
+    for (int it = it0; !cg_conv && it <= o.max_iter; ++it) {
         if (it > 0) {
             ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, A, lda, w.X, n, B, ldb, w.R, n, w.part, s));
-            r.iterations = it;
+            r.iterations = base + it;
         }
         // norms
         int rc = host_colmax(c, w.R, w.nrm + k, rn);

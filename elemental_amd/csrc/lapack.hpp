@@ -6,6 +6,11 @@
 namespace elemamd {
 
 enum : int { kTrailingF64 = 0, kTrailingF32 = 1, kTrailingBf16x3 = 2, kTrailingBf16x6 = 3 };
+// Refinement of the mixed solve: classic iterative refinement (GMRES-IR on
+// stagnation) or preconditioned CG on A with the low-precision factor as the
+// preconditioner and FP64 products, residuals and iterates (classic IR takes
+// over if it stagnates).
+enum : int { kRefineIR = 0, kRefineCG = 1 };
 
 // Columns of panels applied per pass over the trailing matrix (a multiple of
 // the block size nb is used: max(1, depth / nb) panels).
@@ -40,6 +45,7 @@ struct MixedOpts {
     int force_gmres = 0;
     int allow_shift = 1;
     double tol = 1e-14;
+    int refine = kRefineIR;
 };
 
 struct MixedReport {

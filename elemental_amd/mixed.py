@@ -10,6 +10,7 @@ import dataclasses
 from ._lib import IROpts, IRReportC, lib
 
 TRAILING = {"f32": 1, "bf16x3": 2, "bf16x6": 3}
+REFINE = {"ir": 0, "cg": 1}
 
 
 @dataclasses.dataclass
@@ -21,10 +22,11 @@ class IROptions:
     force_gmres: bool = False
     allow_shift: bool = True
     tol: float = 1e-14
+    refine: str = "ir"  # "ir": classic refinement (GMRES-IR fallback), "cg": preconditioned CG
 
     def c(self) -> IROpts:
         return IROpts(self.nb, TRAILING[self.trailing], self.max_iter, self.gmres_restart if self.gmres_restart else -1,
-                      int(self.force_gmres), int(self.allow_shift), self.tol)
+                      int(self.force_gmres), int(self.allow_shift), self.tol, REFINE[self.refine])
 
 
 @dataclasses.dataclass

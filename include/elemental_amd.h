@@ -143,7 +143,17 @@ typedef struct {
     int force_gmres;      /* 1: use GMRES-IR from the first correction */
     int allow_shift;      /* 1: on low-precision breakdown, retry the factor of A + sigma*diag(A) */
     double tol;           /* target normwise backward error (inf-norms); 0 -> 1e-14 */
+    int refine;           /* ELEMAMD_REFINE_IR (classic) or ELEMAMD_REFINE_CG (preconditioned CG) */
 } elemamd_ir_opts;
+
+/* Refinement of the mixed solve.  IR: x += M^{-1} (b - A x) with the
+ * low-precision factor M, GMRES-IR on stagnation.  CG: conjugate gradients
+ * on A x = b preconditioned by M (flexible direction update, FP64 products,
+ * iterates and residuals; the true residual confirms convergence), classic
+ * IR taking over if it stagnates.  report->iterations counts FP64 products
+ * with A (residuals for IR, CG steps for CG). */
+#define ELEMAMD_REFINE_IR 0
+#define ELEMAMD_REFINE_CG 1
 
 typedef struct {
     int info;             /* reserved, 0 */

@@ -271,15 +271,16 @@ def _eta_inf(A, X, B, oracle):
                for c in range(B.shape[1]))
 
 
-@pytest.mark.parametrize("trailing,kappa,gmres", [("f32", 1e6, False), ("bf16x3", 1e4, False),
-                                                  ("f32", 1e6, True), ("bf16x3", 1e6, False),
-                                                  ("bf16x6", 1e6, False)])
-def test_mixed_ir_reaches_fp64_backward_error(oracle, trailing, kappa, gmres):
+@pytest.mark.parametrize("trailing,kappa,gmres,refine", [("f32", 1e6, False, "ir"), ("bf16x3", 1e4, False, "ir"),
+                                                         ("f32", 1e6, True, "ir"), ("bf16x3", 1e6, False, "ir"),
+                                                         ("bf16x6", 1e6, False, "ir"), ("f32", 1e6, False, "cg"),
+                                                         ("bf16x3", 1e6, False, "cg"), ("bf16x3", 1e8, False, "cg")])
+def test_mixed_ir_reaches_fp64_backward_error(oracle, trailing, kappa, gmres, refine):
     n, k = 1536, 8
     A = inputs.spectrum_host(n, kappa, seed=21)
     B = inputs.normal_host(n, k, seed=22)
     Ad, Bd = dev(A), dev(B)
-    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing=trailing, force_gmres=gmres))
+    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing=trailing, force_gmres=gmres, refine=refine))
     X = host(Bd)
     assert rep.converged, rep
     eta = _eta_inf(A, X, B, oracle)
@@ -292,6 +293,23 @@ def test_mixed_ir_reaches_fp64_backward_error(oracle, trailing, kappa, gmres):
     assert oracle.hpd_solve(Lr, Xr, 128, host_blas=True) == 0
     fwd = np.linalg.norm(X - Xr) / np.linalg.norm(Xr)
     assert fwd <= 100 * kappa * EPS
+
+
+@pytest.mark.parametrize("n,k", [(2048, 4), (777, 33)])
+def test_mixed_cg_fewer_products_than_ir(oracle, n, k):
+    """Preconditioned CG with the split-BF16x3 factor: fewer FP64 products
+    with A than classic refinement to the same backward error, every column
+    (33 = two correction chunks) confirmed by the true residual."""
+    A = inputs.spectrum_host(n, 1e6, seed=51 + n)
+    B = inputs.normal_host(n, k, seed=52 + n)
+    its = {}
+    for refine in ("ir", "cg"):
+        Ad, Bd = dev(A), dev(B)
+        rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing="bf16x3", refine=refine, gmres_restart=-1))
+        assert rep.converged and not rep.used_gmres, (refine, rep)
+        assert _eta_inf(A, host(Bd), B, oracle) <= 1e-14
+        its[refine] = rep.iterations
+    assert its["cg"] < its["ir"], its
 
 
 def test_mixed_bf16x6_between_bf16x3_and_fp32():
