@@ -337,19 +337,42 @@ __device__ inline double block_max_256(double v) {
     return red[0];
 }
 
-__global__ __launch_bounds__(256) void col_absmax_kernel(int m, const double* X, i64 ldx, double* colmax) {
-    const int c = blockIdx.x;
-    double v = 0.0;
-    for (i64 i = threadIdx.x; i < m; i += 256) {
-        const double a = fabs(X[i + c * ldx]);
-        v = (a > v || a != a) ? a : v;  // propagate NaN
+// Reductions over one column per workgroup of 1024 threads: each thread
+// keeps 4 independent loads in flight, then the waves reduce by shuffles.
+template <typename Op>
+__device__ inline double block_reduce_1024(double v, Op op) {
+    __shared__ double part[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) part[t >> 6] = v;
+    __syncthreads();
+    if (t < 64) {
+        v = t < 16 ? part[t] : part[0];
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o));
     }
-    v = block_max_256(v);
+    return v;  // valid in thread 0
+}
+
+__global__ __launch_bounds__(1024) void col_absmax_kernel(int m, const double* X, i64 ldx, double* colmax) {
+    const int c = blockIdx.x;
+    const double* x = X + (i64)c * ldx;
+    auto mx = [](double a, double b) { return (a > b || a != a) ? a : b; };  // propagates NaN
+    double v = 0.0;
+    for (i64 i0 = threadIdx.x; i0 < m; i0 += 4 * 1024) {
+        double a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) a[u] = i0 + 1024 * u < m ? fabs(x[i0 + 1024 * u]) : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v = mx(a[u], v);
+    }
+    v = block_reduce_1024(v, mx);
     if (threadIdx.x == 0) colmax[c] = v;
 }
 hipError_t launch_col_absmax(int m, int k, const double* X, i64 ldx, double* colmax, hipStream_t s) {
     if (k <= 0) return hipSuccess;
-    hipLaunchKernelGGL(col_absmax_kernel, dim3((unsigned)k), dim3(256), 0, s, m, X, ldx, colmax);
+    hipLaunchKernelGGL(col_absmax_kernel, dim3((unsigned)k), dim3(1024), 0, s, m, X, ldx, colmax);
     return hipGetLastError();
 }
 
@@ -370,26 +393,31 @@ hipError_t launch_vec_max(int n, const double* v, double* out, hipStream_t s) {
 // set lives at V + i*vstride (an n x k block with leading dim ldv).
 
 // out[i + c*nv] = sum_r V_i(r, c) * W(r, c)   (one workgroup per (c, i))
-__global__ __launch_bounds__(256) void batched_dots_kernel(int n, int nv, const double* V, i64 ldv, i64 vstride,
-                                                           const double* W, i64 ldw, double* out) {
-    __shared__ double red[256];
+__global__ __launch_bounds__(1024) void batched_dots_kernel(int n, int nv, const double* V, i64 ldv, i64 vstride,
+                                                            const double* W, i64 ldw, double* out) {
     const int c = blockIdx.x, i = blockIdx.y;
     const double* v = V + (i64)i * vstride + (i64)c * ldv;
     const double* w = W + (i64)c * ldw;
-    double s = 0.0;
-    for (i64 r = threadIdx.x; r < n; r += 256) s += v[r] * w[r];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (i64 r0 = threadIdx.x; r0 < n; r0 += 4 * 1024) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool in = r0 + 1024 * u < n;
+            a[u] = in ? v[r0 + 1024 * u] : 0.0;
+            b[u] = in ? w[r0 + 1024 * u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) sum[u] = fma(a[u], b[u], sum[u]);
     }
-    if (threadIdx.x == 0) out[i + (i64)c * nv] = red[0];
+    double x = (sum[0] + sum[1]) + (sum[2] + sum[3]);
+    x = block_reduce_1024(x, [](double p, double q) { return p + q; });
+    if (threadIdx.x == 0) out[i + (i64)c * nv] = x;
 }
 hipError_t launch_batched_dots(int n, int k, int nv, const double* V, i64 ldv, i64 vstride, const double* W, i64 ldw,
                                double* out, hipStream_t s) {
     if (k <= 0 || nv <= 0) return hipSuccess;
-    hipLaunchKernelGGL(batched_dots_kernel, dim3((unsigned)k, (unsigned)nv), dim3(256), 0, s, n, nv, V, ldv, vstride,
+    hipLaunchKernelGGL(batched_dots_kernel, dim3((unsigned)k, (unsigned)nv), dim3(1024), 0, s, n, nv, V, ldv, vstride,
                        W, ldw, out);
     return hipGetLastError();
 }

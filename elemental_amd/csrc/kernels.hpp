@@ -59,11 +59,17 @@ hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, 
                                       i64 ldk, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
                                       hipStream_t s);
 
-// One 128-column block column from the planes: C (m x 128; lower part of its
-// top 128 x 128 block and all rows below) -= P P_top^T, P = rows 0..m-1 of
-// the planes, k columns (the group's earlier panels).
-hipError_t launch_syrk_col_bf16_planes(int np, int m, int k, const unsigned short* planes, i64 ps, i64 ldk, float* C,
-                                       i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
+// One block column (ncol <= 128 columns) from the planes: C (m x ncol;
+// lower part of its top ncol x ncol block and all rows below) -= P P_top^T,
+// P = rows 0..m-1 of the planes, k columns (the group's earlier panels);
+// only rows [rbeg, rend) (rend >= min(m, ncol)).
+hipError_t launch_syrk_col_bf16_planes(int np, int m, int ncol, int rbeg, int rend, int k, const unsigned short* planes,
+                                       i64 ps, i64 ldk, float* C, i64 ldc, const int* info, int kcol0, int nbp,
+                                       hipStream_t s);
+// Its diagonal block alone (ncol <= 128), with many workgroups: C (ncol x
+// ncol, lower) -= P P^T, P = the first ncol rows of the planes.
+hipError_t launch_syrk_diag_bf16_planes(int np, int ncol, int k, const unsigned short* planes, i64 ps, i64 ldk,
+                                        float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
 
 // gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,

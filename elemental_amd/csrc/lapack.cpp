@@ -74,7 +74,12 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     // next group's block columns and the rest.  Two sets, alternating by
     // group: the rest-update of a group still reads its planes while the next
     // group's are written.
-    const int np = sizeof(T) == 4 ? (trailing == kTrailingBf16x3 ? 2 : trailing == kTrailingBf16x6 ? 3 : 0) : 0;
+    // (plane chunks are 32 columns: block columns must start on chunk
+    // boundaries; split-BF16x3 has an in-kernel-split form for other nb)
+    const int np = sizeof(T) == 4 && nb % 32 == 0
+                       ? (trailing == kTrailingBf16x3 ? 2 : trailing == kTrailingBf16x6 ? 3 : 0)
+                       : 0;
+    if (sizeof(T) == 4 && trailing == kTrailingBf16x6 && np == 0) return kErrArg;
     const bool planes = np > 0;
     const i64 ldk = ((i64)gw + 31) / 32 * 32;
     const size_t plane_elems = (size_t)n * ldk;
@@ -98,14 +103,27 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         for (int k = c0; k < c1; k += nb) {
             const int b = std::min(nb, n - k);
             T* Akk = A + k + (i64)k * lda;
+            bool below_pending = false;  // rows below the diagonal block updated on aux2
             if (k > c0) {  // panels c0 .. k-1 of this group into block column k
-                if (planes)
-                    ELEMAMD_HIP_TRY(launch_syrk_col_bf16_planes(np, n - k, k - c0, Pl + (i64)(k - c0) * ldk,
-                                                                (i64)plane_elems, ldk, (float*)Akk, lda, dinfo, c0,
-                                                                nb, s));
-                else
+                if (planes) {
+                    // look-ahead: the diagonal block first (the potrf waits for
+                    // it alone), the rows below it on aux2 under the potrf
+                    const unsigned short* Pk = Pl + (i64)(k - c0) * ldk;
+                    if (n - k > b) {
+                        ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_c, s));
+                        ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux2, sp->ev_c, 0));
+                        ELEMAMD_HIP_TRY(launch_syrk_col_bf16_planes(np, n - k, b, b, n - k, k - c0, Pk,
+                                                                    (i64)plane_elems, ldk, (float*)Akk, lda, dinfo,
+                                                                    c0, nb, sp->aux2));
+                        ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_d, sp->aux2));
+                        below_pending = true;
+                    }
+                    ELEMAMD_HIP_TRY(launch_syrk_diag_bf16_planes(np, b, k - c0, Pk, (i64)plane_elems, ldk,
+                                                                 (float*)Akk, lda, dinfo, c0, nb, s));
+                } else {
                     ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, n - k, b, k - c0, A + k + (i64)c0 * lda, lda, Akk, lda,
                                                    dinfo, c0, nb, s));
+                }
             }
             if constexpr (sizeof(T) == 8) {
                 ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
@@ -113,6 +131,7 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
                     ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
             } else {  // the low-precision factor: diagonal inverse, panel as a product with it
                 ELEMAMD_HIP_TRY(launch_potrf_f32(b, (float*)Akk, lda, Xd, dinfo, k, s));
+                if (below_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_d, 0));
                 if (n - k - b > 0)
                     ELEMAMD_HIP_TRY(launch_panel_f32(
                         n - k - b, b, (const float*)Akk, lda, Xd, (float*)Akk + b, lda, dinfo, k + b,

@@ -1,5 +1,6 @@
 #include "runtime.hpp"
 
+#include <initializer_list>
 #include <map>
 #include <utility>
 
@@ -17,14 +18,16 @@ struct Cache {
             if (kv.second.p) (void)hipFree(kv.second.p);
     }
 };
+void release(StreamPair& p) {
+    for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d})
+        if (*e) (void)hipEventDestroy(*e), *e = nullptr;
+    for (hipStream_t* t : {&p.aux, &p.aux2})
+        if (*t) (void)hipStreamDestroy(*t), *t = nullptr;
+}
 struct PairCache {
     std::map<std::pair<int, hipStream_t>, StreamPair> pairs;
     ~PairCache() {
-        for (auto& kv : pairs) {
-            if (kv.second.ev_a) (void)hipEventDestroy(kv.second.ev_a);
-            if (kv.second.ev_b) (void)hipEventDestroy(kv.second.ev_b);
-            if (kv.second.aux) (void)hipStreamDestroy(kv.second.aux);
-        }
+        for (auto& kv : pairs) release(kv.second);
     }
 };
 }  // namespace
@@ -34,10 +37,15 @@ StreamPair* stream_pair(hipStream_t stream) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
     StreamPair& p = cache.pairs[{dev, stream}];
-    if (p.aux == nullptr) {
-        if (hipStreamCreateWithFlags(&p.aux, hipStreamNonBlocking) != hipSuccess) return nullptr;
-        if (hipEventCreateWithFlags(&p.ev_a, hipEventDisableTiming) != hipSuccess) return nullptr;
-        if (hipEventCreateWithFlags(&p.ev_b, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (p.aux == nullptr) {  // all or nothing
+        bool ok = hipStreamCreateWithFlags(&p.aux, hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&p.aux2, hipStreamNonBlocking) == hipSuccess;
+        for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d})
+            ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            release(p);
+            return nullptr;
+        }
     }
     return &p;
 }

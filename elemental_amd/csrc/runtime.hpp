@@ -15,8 +15,9 @@ void* stream_scratch(size_t bytes, hipStream_t stream);
 // with `stream` for the look-ahead factorization: work forked onto `aux`
 // after an event recorded on `stream` and joined back the same way.
 struct StreamPair {
-    hipStream_t aux = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipStream_t aux = nullptr;   // bulk trailing updates
+    hipStream_t aux2 = nullptr;  // look-ahead block-column updates
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr;
 };
 // Owned by the calling thread; nullptr members on failure.
 StreamPair* stream_pair(hipStream_t stream);

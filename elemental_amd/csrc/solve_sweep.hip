@@ -28,6 +28,7 @@
 // wait is on a workgroup that is running or done (the R row tiles of one
 // step need to be resident together), and every wait is bounded.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -264,12 +265,12 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
         __syncthreads();
     };
     u64 gv[kSwVecQ];
-    auto vec_load = [&](const float* src, int bb, int h) {  // half h of a published block vector: coherent loads
+    auto vec_load = [&](const float* src, int bb, int h) __attribute__((always_inline)) {  // half h of a published block vector: coherent loads
         const u64* p = reinterpret_cast<const u64*>(src + ((i64)bb * kSwNB + h * kSwH) * kSwKP);
 #pragma unroll
         for (int q = 0; q < kSwVecQ; ++q) gv[q] = coherent_load(&p[t + 256 * q]);
     };
-    auto vec_store = [&](int buf) {
+    auto vec_store = [&](int buf) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = 0; q < kSwVecQ; ++q) {
             const int e = t + 256 * q, r = e >> 4, cp = e & 15;
@@ -279,23 +280,24 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
             *reinterpret_cast<float2*>(&sV[buf][r][2 * cp]) = v;
         }
     };
-    float a[kSwNB / 4];
+    float a[kSwNB / 4];  // the A operand of one product: K half h in a[64 h .. 64 h + 63]
+    constexpr int kJH = kSwH / 4;
     // M(arow, bb*NB + 4j + g): L below the diagonal, its mirror above.  Rows
     // and columns past n read clamped (finite) entries: they meet zero rows
     // of the block vectors or produce output rows that are never kept.
-    auto a_from_factor = [&](int bb) {
+    auto a_from_factor = [&](int bb, int h) __attribute__((always_inline)) {
         const i64 c0 = (i64)bb * kSwNB + g;
 #pragma unroll
-        for (int j = 0; j < kSwNB / 4; ++j) {
+        for (int j = h * kJH; j < (h + 1) * kJH; ++j) {
             const i64 col = c0 + 4 * j < n ? c0 + 4 * j : (i64)n - 1;
             a[j] = S[arow + col * lds];
         }
     };
-    auto a_from_tile = [&](int i) {
+    auto a_from_tile = [&](int i, int h) __attribute__((always_inline)) {
         const float4* p = reinterpret_cast<const float4*>(tiles + ((i64)i * T + s) * kSwNB * kSwNB) +
                           (i64)(rt * 4 + w) * (kSwNB / 16) * 64 + lane;
 #pragma unroll
-        for (int jq = 0; jq < kSwNB / 16; ++jq) {
+        for (int jq = h * (kJH / 4); jq < (h + 1) * (kJH / 4); ++jq) {
             const float4 v = p[(i64)jq * 64];
             a[4 * jq] = v.x;
             a[4 * jq + 1] = v.y;
@@ -303,12 +305,20 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
             a[4 * jq + 3] = v.w;
         }
     };
+    const int nU = s > kSwLook ? s - kSwLook : 0;  // older blocks, folded in before z
+    // Half h of the A operand of a product: from the factor (block column
+    // idx) or a sweep tile (term idx of this step), chosen at compile time so
+    // that every path issues the same loads and the waits stay exact.
+    auto a_half = [&](auto from_factor, int idx, int h) __attribute__((always_inline)) {
+        if constexpr (decltype(from_factor)::value) a_from_factor(idx, h);
+        else a_from_tile(idx, h);
+    };
     // acc += A-operand K rows h*kSwH .. (h+1)*kSwH x the half staged in
     // sV[buf].  The MFMA statements are volatile, so the compiler keeps the
     // LDS reads where the source puts them: they run kD K-steps ahead of
     // their MFMAs through a ring of registers instead of each MFMA waiting
     // out the LDS latency of its own operand.
-    auto product = [&](int buf, f4_t (&acc)[NT], int h) {
+    auto product = [&](int buf, f4_t (&acc)[NT], int h) __attribute__((always_inline)) {
         constexpr int kD = 6 / NT, kJ = kSwH / 4;
         float bq[kD + 1][NT];
 #pragma unroll
@@ -325,19 +335,28 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
         }
         mfma_retire();
     };
-    // acc += A-operand rows x block bb of src, the two halves through the two
-    // LDS buffers (the second half loads while the first multiplies).  The
-    // caller has synchronised the workgroup since the last product.
-    auto block_product = [&](const float* src, int bb, f4_t (&acc)[NT]) {
+    // acc += (A operand cur) x block bb of src, the two halves of the block
+    // vector through the two LDS buffers.  Loads of the A operand run one
+    // half ahead: half 0 of this product was issued while the previous one
+    // multiplied its half 1; half 1 is issued here before the first half
+    // multiplies, and half 0 of the next product (A operand nxt) before the
+    // second.  Every product ends with its MFMAs retired (product()), so
+    // the registers a half frees can be reloaded at once.
+    auto block_product = [&](auto cur_factor, auto nxt_factor, const float* src, int bb, f4_t (&acc)[NT], int cur,
+                             int nxt) __attribute__((always_inline)) {
         vec_load(src, bb, 0);
+        a_half(cur_factor, cur, 1);
         vec_store(0);
         __syncthreads();
         vec_load(src, bb, 1);
         product(0, acc, 0);
+        a_half(nxt_factor, nxt, 0);
         vec_store(1);
         __syncthreads();
         product(1, acc, 1);
     };
+    const std::true_type kFactor{};
+    const std::false_type kTile{};
 
     // own right-hand sides, in the accumulator layout (row orow + v, column 16 nt + m)
     float r[NT][4];
@@ -352,15 +371,15 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
         }
 
     // 1. older blocks
-    const int nU = s > kSwLook ? s - kSwLook : 0;
     f4_t acc[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
-    if (nU > 0) a_from_factor(blk(0));
+    if (nU > 0) a_from_factor(blk(0), 0);
+    else a_from_tile(0, 0);
     for (int sp = 0; sp < nU; ++sp) {
         wait_all(yflag + sp * kSwR);
-        block_product(Y, blk(sp), acc);
-        if (sp + 1 < nU) a_from_factor(blk(sp + 1));
+        if (sp + 1 < nU) block_product(kFactor, kFactor, Y, blk(sp), acc, blk(sp), blk(sp + 1));
+        else block_product(kFactor, kTile, Y, blk(sp), acc, blk(sp), 0);
     }
 
     // 2. z = r - u (rows past n are zero), published
@@ -373,18 +392,17 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
         }
     publish_flag(&zflag[s * kSwR + rt], t);
 
-    // 3. y = X z + sum_i P(i) y_{s-i}
-    a_from_tile(0);
+    // 3. y = X z + sum_i P(i) y_{s-i}: terms i = 0, then d .. 1 while s - i >=
+    // 0 (after the last one, half 0 of term 0 is loaded again: unused)
     wait_all(zflag + s * kSwR);
     f4_t y[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) y[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
-    block_product(Z, b, y);
-    for (int i = kSwLook; i >= 1; --i) {
-        if (s - i < 0) continue;
-        a_from_tile(i);
+    const int i_top = s < kSwLook ? s : kSwLook;
+    block_product(kTile, kTile, Z, b, y, 0, i_top);
+    for (int i = i_top; i >= 1; --i) {
         wait_all(yflag + (s - i) * kSwR);
-        block_product(Y, blk(s - i), y);
+        block_product(kTile, kTile, Y, blk(s - i), y, i, i - 1);
     }
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)

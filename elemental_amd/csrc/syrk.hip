@@ -23,6 +23,8 @@
 //   F32    : v_mfma_f32_16x16x4_f32 on float storage
 //   BF16X3 : split-bf16 (hi + lo) of the float panel, three products
 //            hi*hi + hi*lo + lo*hi with v_mfma_f32_16x16x32_bf16, fp32 accumulate
+#include <algorithm>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "syrk_dev.hpp"
@@ -160,8 +162,9 @@ hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, 
 }
 
 // One block column of the trailing matrix from pre-split planes (the
-// in-group update of the float factorization): C (m x 128, rows r >= c of
-// its top 128 x 128 block and every row below) := C - P P_top^T, P = the m x k
+// in-group update of the float factorization): C (m x ncol, ncol <= 128;
+// rows r >= c of its top ncol x ncol block and every row below; rows
+// rbeg <= r < m of it) := C - P P_top^T, P = the m x k
 // panel rows of the NP planes (row-major, ld ldk, plane stride ps; P_top = its
 // first 128 rows).  One 64-row slab per workgroup, wave w owns 16 rows and
 // all 128 columns; the fragments are 16-byte loads straight from the planes
@@ -170,12 +173,13 @@ hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, 
 // accumulators start from C and the slab's operand is negated (sign bits).
 template <int NP>
 __global__ __launch_bounds__(256) void syrk_col_bf16p_kernel(int m, int k, const unsigned short* __restrict__ planes,
-                                                             i64 ps, i64 ldk, float* __restrict__ C, i64 ldc,
-                                                             const int* __restrict__ info, int kcol0, int nbp) {
+                                                             i64 ps, i64 ldk, int ncol, int rbeg, float* __restrict__ C,
+                                                             i64 ldc, const int* __restrict__ info, int kcol0,
+                                                             int nbp) {
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
     if (info != nullptr && ke == 0) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
-    const int r0 = blockIdx.x * 64 + 16 * w;                   // this wave's rows
+    const int r0 = rbeg + blockIdx.x * 64 + 16 * w;            // this wave's rows
     const int ra = r0 + m16 < m ? r0 + m16 : m - 1;            // A-operand row (clamped; its outputs are dropped)
     const unsigned short* a_p = planes + (i64)ra * ldk + 8 * g;
     const s8_t neg = {(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(256) void syrk_col_bf16p_kernel(int m, int k, const
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = r0 + 4 * g + v, c = 16 * nt + m16;
-            acc[nt][v] = (r < m && (r >= 128 || c <= r)) ? C[r + (i64)c * ldc] : 0.0f;
+            acc[nt][v] = (r < m && c < ncol && (r >= ncol || c <= r)) ? C[r + (i64)c * ldc] : 0.0f;
         }
     s8_t fa[2][NP], fb[2][8][NP];  // [buffer][plane], [buffer][column tile][plane]
     auto load = [&](int buf, int kc) {
@@ -224,19 +228,95 @@ __global__ __launch_bounds__(256) void syrk_col_bf16p_kernel(int m, int k, const
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = r0 + 4 * g + v, cc = 16 * nt + m16;
-            if (r < m && (r >= 128 || cc <= r)) C[r + (i64)cc * ldc] = acc[nt][v];
+            if (r < m && cc < ncol && (r >= ncol || cc <= r)) C[r + (i64)cc * ldc] = acc[nt][v];
         }
 }
 
-hipError_t launch_syrk_col_bf16_planes(int np, int m, int k, const unsigned short* planes, i64 ps, i64 ldk, float* C,
-                                       i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
-    if (m <= 0 || k <= 0) return hipSuccess;
+hipError_t launch_syrk_col_bf16_planes(int np, int m, int ncol, int rbeg, int rend, int k, const unsigned short* planes,
+                                       i64 ps, i64 ldk, float* C, i64 ldc, const int* info, int kcol0, int nbp,
+                                       hipStream_t s) {
+    // rows [rbeg, rend) of the m-row block column; rend >= min(m, ncol) (the
+    // top block's rows are the B operand)
+    rend = std::min(rend, m);
+    if (m <= 0 || k <= 0 || ncol <= 0 || rbeg >= rend) return hipSuccess;
+    if (ncol > 128 || rbeg < 0 || rend < std::min(m, ncol)) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)ceil_div(rend - rbeg, 64));
     if (np == 2)
-        hipLaunchKernelGGL(syrk_col_bf16p_kernel<2>, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, k, planes,
-                           ps, ldk, C, ldc, info, kcol0, nbp);
+        hipLaunchKernelGGL(syrk_col_bf16p_kernel<2>, grid, dim3(256), 0, s, rend, k, planes, ps, ldk, ncol, rbeg, C,
+                           ldc, info, kcol0, nbp);
     else if (np == 3)
-        hipLaunchKernelGGL(syrk_col_bf16p_kernel<3>, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, k, planes,
-                           ps, ldk, C, ldc, info, kcol0, nbp);
+        hipLaunchKernelGGL(syrk_col_bf16p_kernel<3>, grid, dim3(256), 0, s, rend, k, planes, ps, ldk, ncol, rbeg, C,
+                           ldc, info, kcol0, nbp);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// The diagonal block alone: C (ncol x ncol, lower) -= P P^T, P = the first
+// ncol rows of the planes, k columns -- the update the panel step waits for.
+// One 16 x 16 output tile per workgroup (the lower ones), its four waves
+// splitting the k chunks (each wave issues the loads of up to 4 chunks at
+// once, then multiplies), partial sums added in LDS in a fixed order.
+template <int NP>
+__global__ __launch_bounds__(256) void syrk_diag_bf16p_kernel(int ncol, int k, const unsigned short* __restrict__ planes,
+                                                              i64 ps, i64 ldk, float* __restrict__ C, i64 ldc,
+                                                              const int* __restrict__ info, int kcol0, int nbp) {
+    const int ke = syrk_valid_k(info, k, kcol0, nbp);
+    if (info != nullptr && ke == 0) return;
+    __shared__ f4_t part[3][64];
+    int ti, tj;
+    tri_tile(blockIdx.x, ti, tj);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
+    const int ra = 16 * ti + m16 < ncol ? 16 * ti + m16 : ncol - 1;  // clamped rows: their outputs are dropped
+    const int rb = 16 * tj + m16 < ncol ? 16 * tj + m16 : ncol - 1;
+    const unsigned short* pa = planes + (i64)ra * ldk + 8 * g;
+    const unsigned short* pb = planes + (i64)rb * ldk + 8 * g;
+    constexpr int kQ = 4;  // chunks per wave in flight
+    const int nchunks = (ke + 31) / 32;  // ldk pads with zeros to a multiple of 32
+    f4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c0 = w; c0 < nchunks; c0 += 4 * kQ) {
+        s8_t fa[kQ][NP], fb[kQ][NP];
+        const s8_t zero = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const int c = c0 + 4 * q;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                fa[q][p] = c < nchunks ? *reinterpret_cast<const s8_t*>(pa + p * ps + 32 * c) : zero;
+                fb[q][p] = c < nchunks ? *reinterpret_cast<const s8_t*>(pb + p * ps + 32 * c) : zero;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kQ; ++q)
+#pragma unroll
+            for (int o = NP - 1; o >= 0; --o)  // term order o = p + q, smallest first
+#pragma unroll
+                for (int p = o; p >= 0; --p) mfma_acc_bf16(acc, fa[q][p], fb[q][o - p]);
+    }
+    mfma_retire();
+    if (w > 0) part[w - 1][lane] = acc;
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const float sum = ((acc[v] + part[0][lane][v]) + part[1][lane][v]) + part[2][lane][v];
+        const int r = 16 * ti + 4 * g + v, c = 16 * tj + m16;
+        if (r < ncol && c <= r) C[r + (i64)c * ldc] -= sum;
+    }
+}
+
+hipError_t launch_syrk_diag_bf16_planes(int np, int ncol, int k, const unsigned short* planes, i64 ps, i64 ldk,
+                                        float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
+    if (ncol <= 0 || k <= 0) return hipSuccess;
+    if (ncol > 128) return hipErrorInvalidValue;
+    const int T = (int)ceil_div(ncol, 16);
+    const dim3 grid((unsigned)(T * (T + 1) / 2));
+    if (np == 2)
+        hipLaunchKernelGGL(syrk_diag_bf16p_kernel<2>, grid, dim3(256), 0, s, ncol, k, planes, ps, ldk, C, ldc, info,
+                           kcol0, nbp);
+    else if (np == 3)
+        hipLaunchKernelGGL(syrk_diag_bf16p_kernel<3>, grid, dim3(256), 0, s, ncol, k, planes, ps, ldk, C, ldc, info,
+                           kcol0, nbp);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

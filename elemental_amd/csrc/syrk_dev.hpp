@@ -66,7 +66,7 @@ __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
             const int kk = kc0 + lk + 2 * q;
             const bool kin = kk < k;
             gI[q] = (rowI && kin) ? pI[(i64)kk * lda] : 0.0;
-            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * lda] : 0.0;
+            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * ldb] : 0.0;
         }
     };
     auto sstore = [&](int buf) {
@@ -175,7 +175,7 @@ __device__ __forceinline__ void syrk_tile_f32(int m, int n, int k, float alpha,
             const int kk = kc0 + lk + 2 * q;
             const bool kin = kk < k;
             gI[q] = (rowI && kin) ? pI[(i64)kk * lda] : 0.0f;
-            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * lda] : 0.0f;
+            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * ldb] : 0.0f;
         }
     };
     auto sstore = [&](int buf) {
@@ -268,7 +268,7 @@ __device__ __forceinline__ void syrk_tile_bf16x3(int m, int n, int k, float alph
             const int kk = kc0 + kh * 16 + q;
             const bool kin = kk < k;
             gI[q] = (rowI && kin) ? pI[(i64)kk * lda] : 0.0f;
-            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * lda] : 0.0f;
+            gJ[q] = (rowJ && kin) ? pJ[(i64)kk * ldb] : 0.0f;
         }
     };
     auto sstore = [&]() {

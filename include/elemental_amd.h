@@ -137,7 +137,7 @@ int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double *A, 
 
 typedef struct {
     int nb;               /* factorization block size, 1..128 (0 -> 128) */
-    int trailing;         /* ELEMAMD_TRAILING_F32, _BF16X3 or _BF16X6 */
+    int trailing;         /* ELEMAMD_TRAILING_F32, _BF16X3 or _BF16X6 (_BF16X6: nb a multiple of 32) */
     int max_iter;         /* refinement steps before giving up (0 -> 30) */
     int gmres_restart;    /* Krylov dimension of a GMRES-IR cycle: 0 -> 20, negative disables the fallback */
     int force_gmres;      /* 1: use GMRES-IR from the first correction */

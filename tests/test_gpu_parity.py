@@ -295,6 +295,22 @@ def test_mixed_ir_reaches_fp64_backward_error(oracle, trailing, kappa, gmres, re
     assert fwd <= 100 * kappa * EPS
 
 
+@pytest.mark.parametrize("trailing,nb", [("bf16x3", 64), ("bf16x3", 96), ("bf16x3", 37), ("bf16x6", 64),
+                                         ("bf16x3", 128)])
+def test_mixed_block_sizes(oracle, trailing, nb):
+    """Plane-path block columns of 64 and 96 (look-ahead update of the rows
+    below the diagonal block when nb is a multiple of 64), the in-kernel split
+    for nb = 37, on a matrix with a partial last block."""
+    n, k = 1000, 3
+    A = inputs.spectrum_host(n, 1e4, seed=61 + nb)
+    B = inputs.normal_host(n, k, seed=62 + nb)
+    Ad, Bd = dev(A), dev(B)
+    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing=trailing, nb=nb, gmres_restart=-1))
+    assert rep.converged and not rep.used_gmres and not rep.shifted, rep
+    assert rep.iterations <= 8, rep
+    assert _eta_inf(A, host(Bd), B, oracle) <= 1e-14
+
+
 @pytest.mark.parametrize("n,k", [(2048, 4), (777, 33)])
 def test_mixed_cg_fewer_products_than_ir(oracle, n, k):
     """Preconditioned CG with the split-BF16x3 factor: fewer FP64 products
