@@ -5,11 +5,10 @@
 //
 // potrf_f32_kernel -- one workgroup, the block in LDS as float, right-looking
 //   in 32-column steps: wave 0 factors the 32 x 32 diagonal piece in FP64;
-//   waves 1-2 solve the rows below it by substitution (FP64, one row per
-//   lane), their spare lanes solving identity rows for the piece's inverse;
-//   all
-//   waves apply the rank-32 update to the rest of the block (FP32 MFMA).  A
-//   block narrower than 128 is padded with the identity.
+//   all waves invert it (recursive doubling, FP64) and multiply the rows
+//   below by the inverse (FP64 MFMA), then apply the rank-32 update to the
+//   rest of the block (FP32 MFMA).  A block narrower than 128 is padded with
+//   the identity.
 // panel_f32_kernel -- one 64-row slab per workgroup, with the inverses of
 //   the four 32 x 32 diagonal pieces of L11 (by the diagonal-block kernel):
 //   each wave solves its 16 rows block by block,
@@ -85,38 +84,70 @@ __device__ inline int factor32(float (*sA)[kPiLd], double (*sD)[34], double (*sC
     return 0;
 }
 
-// Waves 1-2, one row per lane, by substitution against the piece factored
-// by factor32 (FP64, L in sD transposed, 1/L_kk in sRd): x = a L^{-T}, i.e.
-// x_k = a_k / L_kk, then a_c -= x_k L_ck for c > k.
-//  * rows r in [c0 + 32, 128): a = row r of the 32 columns below the piece
-//    (in place in sA);
-//  * rows 128 + c (c < 32; lanes that have no row below the piece, and there
-//    are always at least 32): a = e_c, so x = column c of inv(L_piece),
-//    written to Xd (row-major 32 x 32 floats).
-__device__ inline void subpanel32(float (*sA)[kPiLd], const double (*sD)[34], const double* sRd, int c0, int row,
-                                  float* __restrict__ Xd) {
-    if (row >= 160) return;
-    const bool ident = row >= 128;
-    const int ci = row - 128, rr = ident ? 127 : row;
-    double a[32];
-#pragma unroll
-    for (int c = 0; c < 32; ++c) {
-        const double v = (double)sA[rr][c0 + c];
-        a[c] = ident ? (c == ci ? 1.0 : 0.0) : v;
+// All threads: X = inv(L) of the piece factored by factor32 (FP64, L in sD
+// transposed, 1/L_kk in sRd) by recursive doubling over diagonal blocks of
+// size b = 1, 2, 4, 8, 16:  for L = [L11 0; L21 L22],
+//     X21 = -X22 (L21 X11)
+// with X11, X22 already inverted; every element of a level is one thread (a
+// dot of length <= b), so the 32-step chain of a substitution becomes 10
+// short steps.  sX: row-major, zero above the diagonal; sT: scratch.
+__device__ inline void invert32(const double (*sD)[34], const double* sRd, double (*sX)[33], double (*sT)[33],
+                                int t) {
+    for (int e = t; e < 1024; e += 256) {
+        const int i = e >> 5, j = e & 31;
+        sX[i][j] = (i == j) ? sRd[i] : 0.0;
     }
-#pragma unroll
-    for (int k = 0; k < 32; ++k) {
-        a[k] *= sRd[k];
-#pragma unroll
-        for (int c = k + 1; c < 32; ++c) a[c] = fma(-a[k], sD[k][c], a[c]);
-        asm volatile("" ::: "memory");  // one column of L in flight at a time (registers)
+    __syncthreads();
+    for (int b = 1; b < 32; b *= 2) {
+        const int ne = 16 * b;  // elements of all the level's X21 blocks
+        const int q = t / (b * b), i = (t % (b * b)) / b, j = t % b;
+        const int r0 = 2 * b * q, rb = r0 + b;
+        if (t < ne) {  // T = L21 X11 (X11 lower: m >= j)
+            double acc = 0.0;
+            for (int m = j; m < b; ++m) acc = fma(sD[r0 + m][rb + i], sX[r0 + m][r0 + j], acc);
+            sT[rb + i][r0 + j] = acc;
+        }
+        __syncthreads();
+        if (t < ne) {  // X21 = -X22 T (X22 lower: m <= i)
+            double acc = 0.0;
+            for (int m = 0; m <= i; ++m) acc = fma(sX[rb + i][rb + m], sT[rb + m][r0 + j], acc);
+            sX[rb + i][r0 + j] = -acc;
+        }
+        __syncthreads();
     }
-    if (ident) {
+}
+
+// All threads: the rows below the piece, r in [c0 + 32, 128), columns
+// c0 .. c0 + 31 of sA := A L^{-T} = A X^T with X = inv(L) (FP64 MFMA
+// 16x16x4, 16 x 16 output tiles dealt to the waves).  Every wave reads its
+// operands, then the workgroup synchronises, then results overwrite sA.
+__device__ inline void below_times_xt(float (*sA)[kPiLd], const double (*sX)[33], int c0, int w, int lane) {
+    const int m = lane & 15, g = lane >> 4;
+    const int ntile = 2 * ((96 - c0) / 16);  // row tiles x 2 column tiles
+    double fa[3][8], fb[3][8];
 #pragma unroll
-        for (int r = 0; r < 32; ++r) Xd[r * 32 + ci] = (float)a[r];
-    } else {
+    for (int u = 0; u < 3; ++u) {
+        const int tt = w + 4 * u, rt = tt >> 1, ct = tt & 1;
+        const int ra = tt < ntile ? c0 + 32 + 16 * rt + m : 127;  // clamped when the tile does not exist
 #pragma unroll
-        for (int c = 0; c < 32; ++c) sA[row][c0 + c] = (float)a[c];
+        for (int kb = 0; kb < 8; ++kb) {
+            fa[u][kb] = (double)sA[ra][c0 + 4 * kb + g];  // A[i = m][k = 4 kb + g]
+            fb[u][kb] = sX[16 * ct + m][4 * kb + g];      // B[k][j = m] = X[j][k]
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int tt = w + 4 * u, rt = tt >> 1, ct = tt & 1;
+        if (tt >= ntile) continue;
+        d4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kb = 0; kb < 8; ++kb) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[u][kb], fb[u][kb], acc, 0, 0, 0);
+        // a dependent zero-operand MFMA right before the accumulator is read
+        // (the wait states between an FP64 MFMA and a read of its result)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sA[c0 + 32 + 16 * rt + g + 4 * v][c0 + 16 * ct + m] = (float)acc[v];
     }
 }
 
@@ -146,6 +177,7 @@ __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict
     __shared__ float sA[128][kPiLd];
     __shared__ double sD[32][34];  // the FP64 factor of the current piece, transposed
     __shared__ double sCol[2][34], sRd[32];
+    __shared__ double sX[32][33], sT[32][33];  // the current piece's inverse, scratch
     __shared__ int sfail;
     if (info != nullptr && *info != 0 && *info <= col_offset) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
@@ -176,7 +208,9 @@ __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict
         }
         __syncthreads();
         if (sfail != 0) break;
-        if (w == 1 || w == 2) subpanel32(sA, sD, sRd, c0, c0 + 32 + 64 * (w - 1) + lane, Xd + p * 1024);
+        invert32(sD, sRd, sX, sT, t);
+        for (int e = t; e < 1024; e += 256) Xd[p * 1024 + e] = (float)sX[e >> 5][e & 31];
+        below_times_xt(sA, sX, c0, w, lane);
         __syncthreads();
         // rank-32 update of the rest of the block (lower 16 x 16 tiles)
         const int nrt = (96 - c0) / 16;
