@@ -134,6 +134,8 @@ hipError_t launch_copy_f64(int m, int n, const double* X, i64 ldx, double* Y, i6
 // staged through double-buffered LDS; the product runs on
 // v_mfma_f64_16x16x4_f64 with MFMA rows = right-hand sides and MFMA columns =
 // output rows, so each 16 lanes store 128 contiguous bytes of a column.
+// One LDS buffer (the next tile waits in registers), so three workgroups
+// share a CU and hide each other's HBM latency.
 // At k = 32 the FP64 work (2 n^2 k flops) and the bytes read (8 n^2) are
 // balanced on MI355X, so the kernel sits near both roofs.
 // ABS = true reads |A| and X = ones: out[p] = sum_q |Asym[p, q]| (row sums
@@ -143,12 +145,12 @@ constexpr int kSymCols = 32;
 constexpr int kSymLd = kSymT + 1;
 
 template <bool ABS>
-__global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double alpha, const double* __restrict__ A,
+__global__ __launch_bounds__(256, 3) void symm_lower_kernel(int n, int k, double alpha, const double* __restrict__ A,
                                                          i64 lda, const double* __restrict__ X, i64 ldx, double beta,
                                                          const double* __restrict__ B, i64 ldb, double* __restrict__ R,
                                                          i64 ldr, int nsplit, double* __restrict__ part) {
-    __shared__ double sM[2][kSymT][kSymLd];     // sM[q][p] = Asym(p0+p, q0+q)
-    __shared__ double sX[2][kSymT][kSymCols + 1];  // sX[q][c] = X(q0+q, c_base+c)
+    __shared__ double sM[kSymT][kSymLd];     // sM[q][p] = Asym(p0+p, q0+q)
+    __shared__ double sX[kSymT][kSymCols + 1];  // sX[q][c] = X(q0+q, c_base+c)
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int I = blockIdx.x;
     const i64 p0 = (i64)I * kSymT;
@@ -195,49 +197,49 @@ __global__ __launch_bounds__(256) void symm_lower_kernel(int n, int k, double al
                         : ((q < n && c < kcl) ? X[q + (i64)(c_base + c) * ldx] : 0.0);
         }
     };
-    auto sstore = [&](int J, int buf) {
+    auto sstore = [&](int J) {
         if (J < I) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) sM[buf][w + 4 * s][lane] = gm[s];
+            for (int s = 0; s < 16; ++s) sM[w + 4 * s][lane] = gm[s];
         } else if (J > I) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) sM[buf][lane][w + 4 * s] = gm[s];
+            for (int s = 0; s < 16; ++s) sM[lane][w + 4 * s] = gm[s];
         } else {
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
                 const int qq = w + 4 * s;
                 if (lane >= qq) {
-                    sM[buf][qq][lane] = gm[s];
-                    sM[buf][lane][qq] = gm[s];
+                    sM[qq][lane] = gm[s];
+                    sM[lane][qq] = gm[s];
                 }
             }
         }
 #pragma unroll
-        for (int s = 0; s < 8; ++s) sX[buf][lane][w + 4 * s] = gx[s];
+        for (int s = 0; s < 8; ++s) sX[lane][w + 4 * s] = gx[s];
     };
 
     d4_t acc[2];
     acc[0] = (d4_t){0.0, 0.0, 0.0, 0.0};
     acc[1] = (d4_t){0.0, 0.0, 0.0, 0.0};
     gload(Jb);
-    sstore(Jb, 0);
+    sstore(Jb);
     __syncthreads();
     for (int J = Jb; J < Je; ++J) {
-        const int buf = (J - Jb) & 1;
-        // branch-free body: the last iteration restages its own tile into the
-        // spare buffer, which is never read again
+        // branch-free body: the last iteration restages its own tile, which
+        // is never read again
         gload((J + 1) < Je ? J + 1 : J);
         // wave w: output rows p = 16w .. 16w+15, both 16-column blocks of c
 #pragma unroll
         for (int ks = 0; ks < kSymT / 4; ++ks) {
             const int q = 4 * ks + (lane >> 4);
-            const double bfrag = sM[buf][q][16 * w + (lane & 15)];
-            const double a0 = sX[buf][q][(lane & 15)];
-            const double a1 = sX[buf][q][16 + (lane & 15)];
+            const double bfrag = sM[q][16 * w + (lane & 15)];
+            const double a0 = sX[q][(lane & 15)];
+            const double a1 = sX[q][16 + (lane & 15)];
             acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag, acc[0], 0, 0, 0);
             acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag, acc[1], 0, 0, 0);
         }
-        sstore((J + 1) < Je ? J + 1 : J, buf ^ 1);
+        __syncthreads();  // every wave is done with this tile
+        sstore((J + 1) < Je ? J + 1 : J);
         __syncthreads();
     }
     // dependent zero-operand MFMAs right before the accumulators are read, so
@@ -285,7 +287,7 @@ __global__ void symm_combine_kernel(int n, int k, int nsplit, const double* __re
 // flight per CU (the kernel is bound by HBM latency otherwise).
 int symm_splits(int n, int k) {
     const i64 rows = ceil_div(n, kSymT) * ceil_div(k, kSymCols);
-    i64 s = ceil_div(1024, rows);
+    i64 s = ceil_div(2048, rows);
     s = std::min<i64>(std::max<i64>(s, 1), 8);
     s = std::min<i64>(s, ceil_div(n, kSymT));
     return (int)s;
