@@ -190,24 +190,45 @@ __global__ __launch_bounds__(256) void sweep_tiles_kernel(int n, const float* __
         }
         return;
     }
-    // X_b lower triangular: forward X(a, q) needs q < 64(ra+1); backward X(q, a) needs q >= 64 ra
+    // X_b lower triangular: forward X(a, q) needs q < 64(ra+1); backward X(q, a) needs q >= 64 ra.
+    // The next 64-wide chunk of both operands loads into registers while
+    // the current one multiplies.
     const int q_lo = dir ? 64 * ra : 0, q_hi = dir ? kSwNB : 64 * (ra + 1);
-    f4_t acc[4] = {};
-    for (int q0 = q_lo; q0 < q_hi; q0 += 64) {
+    float ga[16], gb[16];
+    auto gload = [&](int q0) {
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int e = w + 4 * u;
             if (!dir) {
-                sA[lane][e] = X[(64 * ra + lane) + (i64)(q0 + e) * kSwNB];  // X(a, q)
+                ga[u] = X[(64 * ra + lane) + (i64)(q0 + e) * kSwNB];  // X(a, q)
                 const i64 row = (i64)b * kSwNB + q0 + lane, col = (i64)bb * kSwNB + 64 * rc + e;
-                sB[lane][e] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // L(b*NB + q, bb*NB + c)
+                gb[u] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // L(b*NB + q, bb*NB + c)
             } else {
-                sA[e][lane] = X[(q0 + lane) + (i64)(64 * ra + e) * kSwNB];  // X(q, a)
+                ga[u] = X[(q0 + lane) + (i64)(64 * ra + e) * kSwNB];  // X(q, a)
                 const i64 row = (i64)bb * kSwNB + 64 * rc + lane, col = (i64)b * kSwNB + q0 + e;
-                sB[e][lane] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // U(b*NB + q, bb*NB + c) = L(bb*NB + c, b*NB + q)
+                gb[u] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // U(b*NB + q, bb*NB + c) = L(bb*NB + c, b*NB + q)
             }
         }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = w + 4 * u;
+            if (!dir) {
+                sA[lane][e] = ga[u];
+                sB[lane][e] = gb[u];
+            } else {
+                sA[e][lane] = ga[u];
+                sB[e][lane] = gb[u];
+            }
+        }
+    };
+    f4_t acc[4] = {};
+    gload(q_lo);
+    for (int q0 = q_lo; q0 < q_hi; q0 += 64) {
+        sstore();
         __syncthreads();
+        if (q0 + 64 < q_hi) gload(q0 + 64);
         lds_tile_product(acc, sA, sB, w, m, g);
         __syncthreads();
     }
