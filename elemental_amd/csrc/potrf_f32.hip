@@ -248,7 +248,11 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
                                                         const int* __restrict__ info, int limit,
                                                         unsigned short* __restrict__ planes, int np, i64 ps,
                                                         i64 ldk) {
-    __shared__ float sL[128][kPiLd];
+    // LDS kept under half a CU (80 KB) so workgroups of this kernel fit beside
+    // two of the trailing update's: L11 only as its six 32 x 32 blocks below
+    // the diagonal pieces (block (p, q), q < p, at p (p - 1) / 2 + q; zero
+    // outside the b x b block, whose padding is the identity)
+    __shared__ float sLb[6][32][36];
     __shared__ float sXd[4][32][36];  // inverses of the four 32 x 32 diagonal pieces
     __shared__ float sB[64][kPiLd];
     if (info != nullptr && *info != 0 && *info <= limit) return;
@@ -256,19 +260,23 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
     const i64 r0 = (i64)blockIdx.x * 64;
     {
         float v[32];
-        // L11: row t % 128, columns t / 128 + 2q, identity beyond b (clamped loads, then select)
-        const int r = t & 127;
-        const i64 rr = r < b ? r : b - 1;
-        for (int q0 = 0; q0 < 64; q0 += 32) {
+        // block e / 1024: row (e / 32) % 32, column e % 32 (clamped loads, then select)
+        for (int e0 = 0; e0 < 6 * 1024; e0 += 32 * 256) {
 #pragma unroll
             for (int q = 0; q < 32; ++q) {
-                const int c = (t >> 7) + 2 * (q0 + q);
-                v[q] = L[rr + (i64)(c < b ? c : b - 1) * ldl];
+                const int e = e0 + t + 256 * q;
+                const int blk = e >> 10, rr = (e >> 5) & 31, cc = e & 31;
+                const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
+                const int r = 32 * p + rr, c = 32 * qb + cc;
+                v[q] = (e < 6 * 1024) ? L[(r < b ? r : b - 1) + (i64)(c < b ? c : b - 1) * ldl] : 0.0f;
             }
 #pragma unroll
             for (int q = 0; q < 32; ++q) {
-                const int c = (t >> 7) + 2 * (q0 + q);
-                sL[r][c] = (r < b && c < b) ? (c <= r ? v[q] : 0.0f) : (r == c ? 1.0f : 0.0f);
+                const int e = e0 + t + 256 * q;
+                const int blk = e >> 10, rr = (e >> 5) & 31, cc = e & 31;
+                const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
+                const int r = 32 * p + rr, c = 32 * qb + cc;
+                if (e < 6 * 1024) sLb[blk][rr][cc] = (r < b && c < b) ? v[q] : 0.0f;
             }
         }
         // the slab: row t % 64, columns t / 64 + 4q
@@ -292,8 +300,9 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
     for (int p = 0; p < 4; ++p) {
         f4_t acc[2] = {};
         for (int q = 0; q < p; ++q) {  // S = sum_q X_q L_pq^T
-            tile_ab_t(acc[0], sB, rw, 32 * q, sL, 32 * p, 32 * q, m16, g);
-            tile_ab_t(acc[1], sB, rw, 32 * q, sL, 32 * p + 16, 32 * q, m16, g);
+            const float(*Lpq)[36] = sLb[p * (p - 1) / 2 + q];
+            tile_ab_t(acc[0], sB, rw, 32 * q, Lpq, 0, 0, m16, g);
+            tile_ab_t(acc[1], sB, rw, 32 * q, Lpq, 16, 0, m16, g);
         }
         mfma_retire();
 #pragma unroll
