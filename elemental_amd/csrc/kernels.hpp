@@ -100,7 +100,12 @@ hipError_t launch_trsm_persist(bool trans, int n, int kc, const T* L, i64 ldl, T
 size_t sweep_tiles_count(int n);
 size_t sweep_xinv_count(int n);
 size_t sweep_work_bytes(int n);
-hipError_t launch_sweep_prepare(int n, float* S, i64 lds, const float* Dinv, float* Xp, float* tiles, hipStream_t s);
+// The strict upper triangle of S := the mirror of its lower one (the
+// backward sweep's operand); it touches nothing launch_sweep_prepare or
+// the diagonal-block inverses read, so it may run beside them.
+hipError_t launch_sweep_mirror(int n, float* S, i64 lds, hipStream_t s);
+hipError_t launch_sweep_prepare(int n, const float* S, i64 lds, const float* Dinv, float* Xp, float* tiles,
+                                hipStream_t s);
 // B := inv(L^T) inv(L) B for B n x k (column-major); work: sweep_work_bytes(n).
 hipError_t launch_sweep_solve(int n, int k, const float* S, i64 lds, const float* tiles, float* B, i64 ldb, void* work,
                               hipStream_t s);

@@ -587,9 +587,18 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
         f.shifted = 1;
     }
     if (info != 0) return kErrNotConverged;  // broke down even after shifting
+    // the mirror (upper triangle) on aux beside the inverses and sweep
+    // operands (lower triangle reads only)
+    StreamPair* sp = stream_pair(s);
+    if (sp == nullptr) return kErrHip;
+    ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
+    ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
+    ELEMAMD_HIP_TRY(launch_sweep_mirror(n, f.S, n, sp->aux));
+    ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_b, sp->aux));
     int rc = solve_prepare<float>(n, f.S, n, f.Dinv, s);
     if (rc) return rc;
     ELEMAMD_HIP_TRY(launch_sweep_prepare(n, f.S, n, f.Dinv, f.Xp, f.tiles, s));
+    ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
     return kOk;
 }
 

@@ -466,10 +466,17 @@ hipError_t sweep_read_error(int n, void* work, int* out, hipStream_t s) {
     return hipStreamSynchronize(s);
 }
 
-hipError_t launch_sweep_prepare(int n, float* S, i64 lds, const float* Dinv, float* Xp, float* tiles, hipStream_t s) {
+hipError_t launch_sweep_mirror(int n, float* S, i64 lds, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    const unsigned nt64 = (unsigned)ceil_div(n, 64), T = (unsigned)ceil_div(n, kSwNB);
+    const unsigned nt64 = (unsigned)ceil_div(n, 64);
     hipLaunchKernelGGL(sweep_mirror_kernel, dim3(nt64, nt64), dim3(256), 0, s, n, S, lds);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_prepare(int n, const float* S, i64 lds, const float* Dinv, float* Xp, float* tiles,
+                                hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned T = (unsigned)ceil_div(n, kSwNB);
     hipLaunchKernelGGL(sweep_xinv_kernel, dim3(kSwR, T), dim3(256), 0, s, n, S, lds, Dinv, Xp);
     hipLaunchKernelGGL(sweep_tiles_kernel, dim3(kSwR * kSwR, (kSwLook + 1) * T, 2), dim3(256), 0, s, n, S, lds, Xp,
                        tiles);
