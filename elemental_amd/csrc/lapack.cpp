@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -426,6 +427,15 @@ int gmres_cycle(Ctx& c, int m, double inner_tol, int* iters_out) {
     return kOk;
 }
 
+// ELEMAMD_CG_TRACE=1 prints the CG refinement's backward errors to stderr.
+bool cg_trace() {
+    static const bool on = [] {
+        const char* v = std::getenv("ELEMAMD_CG_TRACE");
+        return v != nullptr && v[0] == '1';
+    }();
+    return on;
+}
+
 // Normwise backward error per column from the device column maxima of R
 // (dev[0..k)) and X (dev[k..2k)), copied to host; returns the maximum.
 int host_eta(Ctx& c, const std::vector<double>& bn, std::vector<double>& eta_c, double& eta) {
@@ -492,11 +502,17 @@ int cg_refine(Ctx& c, const MixedOpts& o, const std::vector<double>& bn, MixedRe
         r.iterations += 1;
         int rc = host_eta(c, bn, eta_c, eta);
         if (rc) return rc;
+        if (cg_trace()) std::fprintf(stderr, "cg step %d: recursive eta %.3e\n", r.iterations, eta);
         if (!(eta < INFINITY)) return kOk;  // breakdown: classic refinement takes over
         if (eta <= o.tol) {  // confirm with the true residual
             ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, c.A, c.lda, w.X, n, c.Bin, c.ldb, w.R, n, w.part, s));
             rc = host_eta(c, bn, eta_c, eta);
             if (rc) return rc;
+            if (cg_trace()) {
+                int nbad = 0;
+                for (int col = 0; col < k; ++col) nbad += eta_c[col] > o.tol;
+                std::fprintf(stderr, "cg true residual: eta %.3e, %d columns above the target\n", eta, nbad);
+            }
             if (eta <= o.tol) {
                 *conv = 1;
                 return kOk;

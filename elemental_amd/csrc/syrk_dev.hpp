@@ -365,31 +365,38 @@ __device__ __forceinline__ void syrk_tile_bf16_planes(int m, int n, int k, float
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
     const bool skip = (ti == tj) && (wj > wi);
-    const int lr = tid & 127, kh = tid >> 7;  // row, k half (16 values = 2 x 8)
-    const bool rowI = (i0 + lr) < m, rowJ = (j0 + lr) < n;
-    const unsigned short* pI = planes + (i64)(rowI ? i0 + lr : 0) * ldk;
-    const unsigned short* pJ = planes + (i64)(rowJ ? j0 + lr : 0) * ldk;
-    s8_t gI[NP][2], gJ[NP][2];  // [plane][group of 8]
+    // staging: four threads per row read its 32-wide chunk (64 contiguous
+    // bytes per plane), rows lrow and lrow + 64
+    const int lq = tid & 3, lrow = tid >> 2;
+    bool rowI[2], rowJ[2];
+    const unsigned short *pI[2], *pJ[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int r = lrow + 64 * h;
+        rowI[h] = (i0 + r) < m;
+        rowJ[h] = (j0 + r) < n;
+        pI[h] = planes + (i64)(rowI[h] ? i0 + r : 0) * ldk + 8 * lq;
+        pJ[h] = planes + (i64)(rowJ[h] ? j0 + r : 0) * ldk + 8 * lq;
+    }
+    s8_t gI[NP][2], gJ[NP][2];  // [plane][row pass]
     const s8_t zero = {0, 0, 0, 0, 0, 0, 0, 0};
     auto gload = [&](int kc0) {
+        const bool kin = kc0 + 8 * lq < k;  // ldk pads to a multiple of 32 with zeros
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int kk = kc0 + kh * 16 + 8 * h;
-            const bool kin = kk < k;  // ldk pads to a multiple of 32 with zeros
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
-                gI[p][h] = (rowI && kin) ? *reinterpret_cast<const s8_t*>(pI + p * ps + kk) : zero;
-                gJ[p][h] = (rowJ && kin) ? *reinterpret_cast<const s8_t*>(pJ + p * ps + kk) : zero;
+                gI[p][h] = (rowI[h] && kin) ? *reinterpret_cast<const s8_t*>(pI[h] + p * ps + kc0) : zero;
+                gJ[p][h] = (rowJ[h] && kin) ? *reinterpret_cast<const s8_t*>(pJ[h] + p * ps + kc0) : zero;
             }
-        }
     };
     auto sstore = [&]() {
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
             for (int p = 0; p < NP; ++p) {
-                *reinterpret_cast<s8_t*>(&sI[p][lr][kh * 16 + 8 * h]) = gI[p][h];
-                *reinterpret_cast<s8_t*>(&sJ[p][lr][kh * 16 + 8 * h]) = gJ[p][h];
+                *reinterpret_cast<s8_t*>(&sI[p][lrow + 64 * h][8 * lq]) = gI[p][h];
+                *reinterpret_cast<s8_t*>(&sJ[p][lrow + 64 * h][8 * lq]) = gJ[p][h];
             }
     };
     // C - P P^T (the factorization): accumulate onto C, read while the first
