@@ -260,12 +260,13 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
     const i64 r0 = (i64)blockIdx.x * 64;
     {
         float v[32];
-        // block e / 1024: row (e / 32) % 32, column e % 32 (clamped loads, then select)
+        // block e / 1024: row e % 32 (lanes along the column), column (e / 32) % 32
+        // (clamped loads, then select)
         for (int e0 = 0; e0 < 6 * 1024; e0 += 32 * 256) {
 #pragma unroll
             for (int q = 0; q < 32; ++q) {
                 const int e = e0 + t + 256 * q;
-                const int blk = e >> 10, rr = (e >> 5) & 31, cc = e & 31;
+                const int blk = e >> 10, rr = e & 31, cc = (e >> 5) & 31;
                 const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
                 const int r = 32 * p + rr, c = 32 * qb + cc;
                 v[q] = (e < 6 * 1024) ? L[(r < b ? r : b - 1) + (i64)(c < b ? c : b - 1) * ldl] : 0.0f;
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
 #pragma unroll
             for (int q = 0; q < 32; ++q) {
                 const int e = e0 + t + 256 * q;
-                const int blk = e >> 10, rr = (e >> 5) & 31, cc = e & 31;
+                const int blk = e >> 10, rr = e & 31, cc = (e >> 5) & 31;
                 const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
                 const int r = 32 * p + rr, c = 32 * qb + cc;
                 if (e < 6 * 1024) sLb[blk][rr][cc] = (r < b && c < b) ? v[q] : 0.0f;
