@@ -179,6 +179,9 @@ __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict
     __shared__ double sCol[2][34], sRd[32];
     __shared__ double sX[32][33], sT[32][33];  // the current piece's inverse, scratch
     __shared__ int sfail;
+    // the factorization's chain: its waves issue ahead of the trailing
+    // update's waves sharing the SIMD (wave priority, not queue priority)
+    __builtin_amdgcn_s_setprio(3);
     if (info != nullptr && *info != 0 && *info <= col_offset) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
     {
@@ -255,6 +258,7 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
     __shared__ float sLb[6][32][36];
     __shared__ float sXd[4][32][36];  // inverses of the four 32 x 32 diagonal pieces
     __shared__ float sB[64][kPiLd];
+    __builtin_amdgcn_s_setprio(3);  // chain kernel (see potrf_f32_kernel)
     if (info != nullptr && *info != 0 && *info <= limit) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
     const i64 r0 = (i64)blockIdx.x * 64;

@@ -176,6 +176,7 @@ __global__ __launch_bounds__(256) void syrk_col_bf16p_kernel(int m, int k, const
                                                              i64 ps, i64 ldk, int ncol, int rbeg, float* __restrict__ C,
                                                              i64 ldc, const int* __restrict__ info, int kcol0,
                                                              int nbp) {
+    __builtin_amdgcn_s_setprio(3);  // chain kernel: issues ahead of the trailing update's waves
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
     if (info != nullptr && ke == 0) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
@@ -261,6 +262,7 @@ template <int NP>
 __global__ __launch_bounds__(256) void syrk_diag_bf16p_kernel(int ncol, int k, const unsigned short* __restrict__ planes,
                                                               i64 ps, i64 ldk, float* __restrict__ C, i64 ldc,
                                                               const int* __restrict__ info, int kcol0, int nbp) {
+    __builtin_amdgcn_s_setprio(3);  // chain kernel: issues ahead of the trailing update's waves
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
     if (info != nullptr && ke == 0) return;
     __shared__ f4_t part[3][64];
