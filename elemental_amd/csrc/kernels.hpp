@@ -55,9 +55,11 @@ hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, con
 // split-BF16x6 (six terms, down to 2^-24 of a product).
 hipError_t launch_split_bf16(int np, int m, int k, const float* P, i64 ldp, unsigned short* planes, i64 ps, i64 ldk,
                              hipStream_t s);
+// chain = 1: the launch is on the factorization's critical chain (its waves
+// take issue priority over other work sharing a SIMD).
 hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, const unsigned short* planes, i64 ps,
                                       i64 ldk, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
-                                      hipStream_t s);
+                                      hipStream_t s, int chain = 0);
 
 // One block column (ncol <= 128 columns) from the planes: C (m x ncol;
 // lower part of its top ncol x ncol block and all rows below) -= P P_top^T,

@@ -165,7 +165,7 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         // critical chain on the caller's stream: the next group's block columns
         if (planes)
             ELEMAMD_HIP_TRY(launch_syrk_ln_bf16_planes(np, m2, bn, kg, -1.0f, Pl, (i64)plane_elems, ldk, 1.0f,
-                                                       (float*)C, lda, dinfo, c0, nb, s));
+                                                       (float*)C, lda, dinfo, c0, nb, s, 1));
         else
             ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2, bn, kg, P, lda, C, lda, dinfo, c0, nb, s));
     }

@@ -129,7 +129,8 @@ __global__ __launch_bounds__(256) void split_bf16_kernel(int np, int m, int k, c
 template <int NP>
 __global__ __launch_bounds__(256, 2) void syrk_ln_bf16p_kernel(int m, int n, int k, float alpha,
         const unsigned short* __restrict__ planes, i64 ps, i64 ldk, float beta, float* __restrict__ C, i64 ldc,
-        const int* __restrict__ info, int kcol0, int nbp) {
+        const int* __restrict__ info, int kcol0, int nbp, int chain) {
+    if (chain) __builtin_amdgcn_s_setprio(3);  // the factorization's chain waits for this launch
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
     if (info != nullptr && ke == 0) return;
     int ti, tj;
@@ -148,14 +149,14 @@ hipError_t launch_split_bf16(int np, int m, int k, const float* P, i64 ldp, unsi
 
 hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, const unsigned short* planes, i64 ps,
                                       i64 ldk, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
-                                      hipStream_t s) {
+                                      hipStream_t s, int chain) {
     if (m <= 0 || n <= 0) return hipSuccess;
     if (np == 2)
         hipLaunchKernelGGL(syrk_ln_bf16p_kernel<2>, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, planes, ps, ldk,
-                           beta, C, ldc, info, kcol0, nbp);
+                           beta, C, ldc, info, kcol0, nbp, chain);
     else if (np == 3)
         hipLaunchKernelGGL(syrk_ln_bf16p_kernel<3>, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, planes, ps, ldk,
-                           beta, C, ldc, info, kcol0, nbp);
+                           beta, C, ldc, info, kcol0, nbp, chain);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
