@@ -478,14 +478,65 @@ int cg_refine(Ctx& c, const MixedOpts& o, const std::vector<double>& bn, MixedRe
     *conv = 0;
     std::vector<int> act(k, 1), act_dev(k, -1);
     std::vector<double> eta_c;
+    StreamPair* sp = stream_pair(s);
+    if (sp == nullptr) return kErrHip;
+    // column maxima of R and X land in pinned memory behind an event, so the
+    // next step's preconditioner can be enqueued before the host reads them
+    // (one grow-only buffer per host thread, kept until the thread ends)
+    struct Pinned {
+        double* p = nullptr;
+        size_t cap = 0;
+        ~Pinned() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+    static thread_local Pinned nx;
+    if (nx.cap < 2 * (size_t)k) {
+        if (nx.p) ELEMAMD_HIP_TRY(hipHostFree(nx.p));
+        nx.p = nullptr;
+        nx.cap = 0;
+        ELEMAMD_HIP_TRY(hipHostMalloc((void**)&nx.p, sizeof(double) * 2 * k, hipHostMallocDefault));
+        nx.cap = 2 * (size_t)k;
+    }
     auto push_active = [&]() -> hipError_t {
         if (act == act_dev) return hipSuccess;
         act_dev = act;
         return hipMemcpyAsync(w.active, act_dev.data(), sizeof(int) * k, hipMemcpyHostToDevice, s);
     };
+    auto eta_enqueue = [&]() -> hipError_t {
+        hipError_t e = launch_col_absmax(n, k, w.R, n, w.nrm + k, s);
+        if (e == hipSuccess) e = launch_col_absmax(n, k, w.X, n, w.nrm + 2 * k, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(nx.p, w.nrm + k, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipEventRecord(sp->ev_d, s);
+        return e;
+    };
+    auto eta_finish = [&]() -> hipError_t {
+        hipError_t e = hipEventSynchronize(sp->ev_d);
+        if (e != hipSuccess) return e;
+        eta_c.resize(k);
+        eta = 0.0;
+        for (int col = 0; col < k; ++col) {
+            const double den = c.anorm * nx.p[k + col] + bn[col];
+            const double v = (den > 0.0) ? nx.p[col] / den : (nx.p[col] > 0.0 ? INFINITY : 0.0);
+            eta_c[col] = std::isnan(v) ? INFINITY : v;
+            eta = std::max(eta, eta_c[col]);
+        }
+        return hipSuccess;
+    };
+    // Z = M^{-1} R, then P = Z - (z.q / p.q) P (flexible beta); touches
+    // neither X nor R, so it may be enqueued before this step's convergence
+    // is known (it is wasted only on the last step)
+    auto direction = [&]() -> int {
+        int rc = precond_apply(c, w.R, w.Z, w.nrm + k);
+        if (rc) return rc;
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.Z, n, (i64)nk, w.Q, n, zq, s));
+        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.R, n, (i64)nk, w.Z, n, rz, s));
+        ELEMAMD_HIP_TRY(launch_cg_p(n, k, zq, pq, w.Z, w.P, s));
+        return kOk;
+    };
     ELEMAMD_HIP_TRY(launch_col_absmax(n, k, w.R, n, w.nrm + k, s));  // preconditioner scale of R = B
     bool restart = true;
-    double best = INFINITY;
+    double best = INFINITY, prev = INFINITY;
     int since_best = 0;
     for (;;) {
         ELEMAMD_HIP_TRY(push_active());
@@ -500,13 +551,22 @@ int cg_refine(Ctx& c, const MixedOpts& o, const std::vector<double>& bn, MixedRe
         ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.P, n, (i64)nk, w.Q, n, pq, s));
         ELEMAMD_HIP_TRY(launch_cg_xr(n, k, rz, pq, w.active, w.P, w.Q, w.X, w.R, s));
         r.iterations += 1;
-        int rc = host_eta(c, bn, eta_c, eta);
-        if (rc) return rc;
+        ELEMAMD_HIP_TRY(eta_enqueue());
+        // the backward error drops about tenfold per step: unless the last
+        // one was within 20x of the target, this step will not converge, so
+        // the next direction is enqueued now
+        const bool ahead = prev > 20.0 * o.tol && r.iterations < o.max_iter;
+        if (ahead) {
+            int rc = direction();
+            if (rc) return rc;
+        }
+        ELEMAMD_HIP_TRY(eta_finish());
+        prev = eta;
         if (cg_trace()) std::fprintf(stderr, "cg step %d: recursive eta %.3e\n", r.iterations, eta);
         if (!(eta < INFINITY)) return kOk;  // breakdown: classic refinement takes over
         if (eta <= o.tol) {  // confirm with the true residual
             ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, c.A, c.lda, w.X, n, c.Bin, c.ldb, w.R, n, w.part, s));
-            rc = host_eta(c, bn, eta_c, eta);
+            int rc = host_eta(c, bn, eta_c, eta);
             if (rc) return rc;
             if (cg_trace()) {
                 int nbad = 0;
@@ -531,11 +591,10 @@ int cg_refine(Ctx& c, const MixedOpts& o, const std::vector<double>& bn, MixedRe
         }
         for (int col = 0; col < k; ++col)
             if (eta_c[col] <= o.tol) act[col] = 0;
-        rc = precond_apply(c, w.R, w.Z, w.nrm + k);
-        if (rc) return rc;
-        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.Z, n, (i64)nk, w.Q, n, zq, s));
-        ELEMAMD_HIP_TRY(launch_batched_dots(n, k, 1, w.R, n, (i64)nk, w.Z, n, rz, s));
-        ELEMAMD_HIP_TRY(launch_cg_p(n, k, zq, pq, w.Z, w.P, s));
+        if (!ahead) {
+            int rc = direction();
+            if (rc) return rc;
+        }
     }
 }
 
