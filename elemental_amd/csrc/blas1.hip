@@ -67,31 +67,37 @@ hipError_t launch_scale_f64(int m, int n, double alpha, double* A, i64 lda, hipS
     return hipGetLastError();
 }
 
-// S(i, j) = float(A(i, j)) for i >= j.  A workgroup covers 256 rows x 8
-// columns (one row per thread, its 8 loads in flight together); workgroups
-// wholly above the diagonal return at once.
-__global__ __launch_bounds__(256) void copy_lower_f64_to_f32_kernel(int n, const double* __restrict__ A, i64 lda,
+// S(i, j) = float(A(i, j)) for i >= j and j0 <= j < j1.  A workgroup covers
+// 256 rows x 8 columns (one row per thread, its 8 loads in flight together);
+// workgroups wholly above the diagonal return at once.
+__global__ __launch_bounds__(256) void copy_lower_f64_to_f32_kernel(int n, int j0, int j1,
+                                                                    const double* __restrict__ A, i64 lda,
                                                                     float* __restrict__ S, i64 lds) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;
-    const int j0 = blockIdx.y * 8;
-    if ((i64)blockIdx.x * 256 + 255 < j0) return;
+    const int jb = j0 + blockIdx.y * 8;
+    if ((i64)blockIdx.x * 256 + 255 < jb) return;
     double v[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int j = j0 + q;
-        v[q] = (i < n && j < n && i >= j) ? A[i + (i64)j * lda] : 0.0;
+        const int j = jb + q;
+        v[q] = (i < n && j < j1 && i >= j) ? A[i + (i64)j * lda] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const int j = j0 + q;
-        if (i < n && j < n && i >= j) S[i + (i64)j * lds] = (float)v[q];
+        const int j = jb + q;
+        if (i < n && j < j1 && i >= j) S[i + (i64)j * lds] = (float)v[q];
     }
 }
-hipError_t launch_copy_lower_f64_to_f32(int n, const double* A, i64 lda, float* S, i64 lds, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(copy_lower_f64_to_f32_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)ceil_div(n, 8)),
-                       dim3(256), 0, s, n, A, lda, S, lds);
+hipError_t launch_copy_lower_f64_to_f32_cols(int n, int j0, int j1, const double* A, i64 lda, float* S, i64 lds,
+                                            hipStream_t s) {
+    j1 = j1 < n ? j1 : n;
+    if (n <= 0 || j0 >= j1) return hipSuccess;
+    hipLaunchKernelGGL(copy_lower_f64_to_f32_kernel, dim3((unsigned)ceil_div(n, 256), (unsigned)ceil_div(j1 - j0, 8)),
+                       dim3(256), 0, s, n, j0, j1, A, lda, S, lds);
     return hipGetLastError();
+}
+hipError_t launch_copy_lower_f64_to_f32(int n, const double* A, i64 lda, float* S, i64 lds, hipStream_t s) {
+    return launch_copy_lower_f64_to_f32_cols(n, 0, n, A, lda, S, lds, s);
 }
 
 __global__ void copy_f64_to_f32_scaled_kernel(int m, const double* X, i64 ldx, const double* cs, float* Y, i64 ldy) {

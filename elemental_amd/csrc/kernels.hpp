@@ -121,6 +121,9 @@ hipError_t launch_fill_int(int* p, int value, int count, hipStream_t s);
 hipError_t launch_fill_normal_f64(double* out, i64 rows, i64 cols, i64 ld, uint64_t seed, i64 offset, hipStream_t s);
 hipError_t launch_scale_f64(int m, int n, double alpha, double* A, i64 lda, hipStream_t s);
 hipError_t launch_copy_lower_f64_to_f32(int n, const double* A, i64 lda, float* S, i64 lds, hipStream_t s);
+// columns j0 <= j < j1 only
+hipError_t launch_copy_lower_f64_to_f32_cols(int n, int j0, int j1, const double* A, i64 lda, float* S, i64 lds,
+                                            hipStream_t s);
 hipError_t launch_copy_f64_to_f32_scaled(int m, int n, const double* X, i64 ldx, const double* colscale, float* Y,
                                          i64 ldy, hipStream_t s);
 hipError_t launch_axpy_f32_to_f64_scaled(int m, int n, const float* D, i64 ldd, const double* colscale, double* X,

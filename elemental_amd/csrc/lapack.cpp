@@ -60,7 +60,7 @@ static int update_depth(int trailing) {
 }
 
 template <typename T>
-int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s) {
+int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s, hipEvent_t trailing_ready) {
     if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr) return kErrArg;
     ELEMAMD_HIP_TRY(launch_fill_int(dinfo, 0, 1, s));
     if (n == 0) return kOk;
@@ -146,6 +146,7 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         T* C = A + c1 + (i64)c1 * lda;        // trailing matrix, m2 x m2
         const int bn = std::min(gw, m2);      // next group's block columns
         if (rest_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
+        if (group == 0 && trailing_ready != nullptr) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, trailing_ready, 0));
         if (planes) Pl += (i64)(c1 - c0) * ldk;  // rows from c1 on
         const bool rest = m2 > bn;
         if (rest) {  // fork: the rest of the trailing matrix on aux
@@ -173,8 +174,8 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     return kOk;
 }
 
-template int cholesky_lower<double>(int, double*, i64, int, int, int*, hipStream_t);
-template int cholesky_lower<float>(int, float*, i64, int, int, int*, hipStream_t);
+template int cholesky_lower<double>(int, double*, i64, int, int, int*, hipStream_t, hipEvent_t);
+template int cholesky_lower<float>(int, float*, i64, int, int, int*, hipStream_t, hipEvent_t);
 
 // ----------------------------------------------------------- SolveAfter
 // cholesky::SolveAfter(LOWER, NORMAL) (SolveAfter.hpp:20-75): forward then
@@ -673,10 +674,26 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     ELEMAMD_HIP_TRY(hipMemcpyAsync(&f.anorm, dtmp, sizeof(double), hipMemcpyDeviceToHost, s));
     double sigma = 0.0;
     int info = 0;
+    StreamPair* sp = stream_pair(s);
+    if (sp == nullptr) return kErrHip;
+    // the float copy: the first group of block columns on the caller's
+    // stream, the columns right of it on aux under that group's factor steps
+    // (the factorization waits for them before its first trailing update)
+    const int gw = std::min(n, std::max(1, update_depth<float>(o.trailing) / std::max(1, o.nb)) * o.nb);
     for (int attempt = 0; attempt < 4; ++attempt) {
-        ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32(n, A, lda, f.S, n, s));
-        if (sigma > 0.0) ELEMAMD_HIP_TRY(launch_shift_diag(n, A, lda, f.S, n, sigma, s));
-        int rc = cholesky_lower<float>(n, f.S, n, o.nb, o.trailing, f.dinfo, s);
+        hipEvent_t ready = nullptr;
+        if (sigma == 0.0 && gw < n && o.nb > 0) {
+            ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32_cols(n, 0, gw, A, lda, f.S, n, s));
+            ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
+            ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
+            ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32_cols(n, gw, n, A, lda, f.S, n, sp->aux));
+            ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_f, sp->aux));
+            ready = sp->ev_f;
+        } else {
+            ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32(n, A, lda, f.S, n, s));
+            if (sigma > 0.0) ELEMAMD_HIP_TRY(launch_shift_diag(n, A, lda, f.S, n, sigma, s));
+        }
+        int rc = cholesky_lower<float>(n, f.S, n, o.nb, o.trailing, f.dinfo, s, ready);
         if (rc) return rc;
         ELEMAMD_HIP_TRY(hipMemcpyAsync(&info, f.dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
         ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
@@ -689,8 +706,6 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     if (info != 0) return kErrNotConverged;  // broke down even after shifting
     // the mirror (upper triangle) on aux beside the inverses and sweep
     // operands (lower triangle reads only)
-    StreamPair* sp = stream_pair(s);
-    if (sp == nullptr) return kErrHip;
     ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
     ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
     ELEMAMD_HIP_TRY(launch_sweep_mirror(n, f.S, n, sp->aux));

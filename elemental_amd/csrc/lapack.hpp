@@ -20,8 +20,13 @@ constexpr int kUpdateDepthBf16x3 = 512;
 
 // cholesky::LVar3 on device memory; T = double (FP64 path) or float (the
 // low-precision factor, diagonal blocks still factored in double).
+// trailing_ready (may be null): an event after which the columns right of
+// the first group of block columns hold their input (they may still be
+// being written when the call starts; the first update that reads them
+// waits for it).
 template <typename T>
-int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s);
+int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s,
+                   hipEvent_t trailing_ready = nullptr);
 
 // cholesky::SolveAfter(LOWER, NORMAL): B := inv(L^T) inv(L) B.
 // `work` holds solve_work_bytes<T>(n, k) bytes of device memory.

@@ -19,7 +19,7 @@ struct Cache {
     }
 };
 void release(StreamPair& p) {
-    for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d})
+    for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d, &p.ev_f})
         if (*e) (void)hipEventDestroy(*e), *e = nullptr;
     for (hipStream_t* t : {&p.aux, &p.aux2})
         if (*t) (void)hipStreamDestroy(*t), *t = nullptr;
@@ -40,7 +40,7 @@ StreamPair* stream_pair(hipStream_t stream) {
     if (p.aux == nullptr) {  // all or nothing
         bool ok = hipStreamCreateWithFlags(&p.aux, hipStreamNonBlocking) == hipSuccess &&
                   hipStreamCreateWithFlags(&p.aux2, hipStreamNonBlocking) == hipSuccess;
-        for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d})
+        for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d, &p.ev_f})
             ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
         if (!ok) {
             release(p);
