@@ -694,7 +694,10 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
             if (sigma > 0.0) ELEMAMD_HIP_TRY(launch_shift_diag(n, A, lda, f.S, n, sigma, s));
         }
         int rc = cholesky_lower<float>(n, f.S, n, o.nb, o.trailing, f.dinfo, s, ready);
-        if (rc) return rc;
+        if (rc) {  // the copy on aux ends before the caller's stream moves on
+            if (ready != nullptr) (void)hipStreamWaitEvent(s, ready, 0);
+            return rc;
+        }
         ELEMAMD_HIP_TRY(hipMemcpyAsync(&info, f.dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
         ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
         f.lowprec_info = info;
