@@ -90,6 +90,9 @@ hipError_t launch_trsm_chain(bool trans, int n, int kc, const T* L, i64 ldl, T* 
 size_t solve_dinv_count(int n);
 template <typename T>
 hipError_t launch_trtri_diag64(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s);
+// the 64-blocks I0 <= I < I1 only
+template <typename T>
+hipError_t launch_trtri_diag64_range(int n, int I0, int I1, const T* L, i64 ldl, T* Dinv, hipStream_t s);
 template <typename T>
 hipError_t launch_trsm_persist(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
                                int* ctrl, const int* info, hipStream_t s);
@@ -108,6 +111,13 @@ size_t sweep_work_bytes(int n);
 hipError_t launch_sweep_mirror(int n, float* S, i64 lds, hipStream_t s);
 hipError_t launch_sweep_prepare(int n, const float* S, i64 lds, const float* Dinv, float* Xp, float* tiles,
                                 hipStream_t s);
+// Column blocks of the sweeps (512): block b's inverse and sweep operands
+// (both directions) read only S's columns of block b (lower triangle) and the
+// diagonal-block inverses of its rows, so they may be made as soon as those
+// columns of the factor are final.
+int sweep_block_cols();
+hipError_t launch_sweep_prepare_blocks(int n, int b0, int b1, const float* S, i64 lds, const float* Dinv, float* Xp,
+                                       float* tiles, hipStream_t s);
 // B := inv(L^T) inv(L) B for B n x k (column-major); work: sweep_work_bytes(n).
 hipError_t launch_sweep_solve(int n, int k, const float* S, i64 lds, const float* tiles, float* B, i64 ldb, void* work,
                               hipStream_t s);

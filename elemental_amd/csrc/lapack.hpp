@@ -1,6 +1,8 @@
 // Host drivers of the path (internal C++ interface under the C ABI).
 #pragma once
 
+#include <functional>
+
 #include "common.hpp"
 
 namespace elemamd {
@@ -24,9 +26,12 @@ constexpr int kUpdateDepthBf16x3 = 512;
 // the first group of block columns hold their input (they may still be
 // being written when the call starts; the first update that reads them
 // waits for it).
+// after_group (may be null): called on the host once the launches of each
+// group of block columns are enqueued, with the group's end column c1; work
+// ordered after that point of s sees columns [0, c1) of the factor final.
 template <typename T>
 int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s,
-                   hipEvent_t trailing_ready = nullptr);
+                   hipEvent_t trailing_ready = nullptr, const std::function<int(int)>* after_group = nullptr);
 
 // cholesky::SolveAfter(LOWER, NORMAL): B := inv(L^T) inv(L) B.
 // `work` holds solve_work_bytes<T>(n, k) bytes of device memory.

@@ -19,9 +19,9 @@ struct Cache {
     }
 };
 void release(StreamPair& p) {
-    for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d, &p.ev_f})
+    for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d, &p.ev_f, &p.ev_g, &p.ev_h})
         if (*e) (void)hipEventDestroy(*e), *e = nullptr;
-    for (hipStream_t* t : {&p.aux, &p.aux2})
+    for (hipStream_t* t : {&p.aux, &p.aux2, &p.aux3})
         if (*t) (void)hipStreamDestroy(*t), *t = nullptr;
 }
 struct PairCache {
@@ -39,8 +39,9 @@ StreamPair* stream_pair(hipStream_t stream) {
     StreamPair& p = cache.pairs[{dev, stream}];
     if (p.aux == nullptr) {  // all or nothing
         bool ok = hipStreamCreateWithFlags(&p.aux, hipStreamNonBlocking) == hipSuccess &&
-                  hipStreamCreateWithFlags(&p.aux2, hipStreamNonBlocking) == hipSuccess;
-        for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d, &p.ev_f})
+                  hipStreamCreateWithFlags(&p.aux2, hipStreamNonBlocking) == hipSuccess &&
+                  hipStreamCreateWithFlags(&p.aux3, hipStreamNonBlocking) == hipSuccess;
+        for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d, &p.ev_f, &p.ev_g, &p.ev_h})
             ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
         if (!ok) {
             release(p);

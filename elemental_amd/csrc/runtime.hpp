@@ -17,7 +17,9 @@ void* stream_scratch(size_t bytes, hipStream_t stream);
 struct StreamPair {
     hipStream_t aux = nullptr;   // bulk trailing updates
     hipStream_t aux2 = nullptr;  // look-ahead block-column updates
-    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr, ev_f = nullptr;
+    hipStream_t aux3 = nullptr;  // solve preparation beside the factorization
+    hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr, ev_f = nullptr, ev_g = nullptr,
+               ev_h = nullptr;
 };
 // Owned by the calling thread; nullptr members on failure.
 StreamPair* stream_pair(hipStream_t stream);

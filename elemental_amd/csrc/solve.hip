@@ -37,12 +37,12 @@ namespace elemamd {
 // Column c is built row by row: W(i,c) = -(sum_{c<=k<i} L(i,k) W(k,c)) / L(i,i),
 // the dot product split over four thread groups and reduced in LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void trtri_diag64_kernel(int n, const T* __restrict__ L, i64 ldl,
+__global__ __launch_bounds__(256) void trtri_diag64_kernel(int n, int I0, const T* __restrict__ L, i64 ldl,
                                                            T* __restrict__ Dinv) {
     __shared__ T sL[kSB][kSLd];  // sL[i][k]
     __shared__ T sW[kSB][kSLd];  // sW[k][c]
     __shared__ T red[4][kSB];
-    const int I = blockIdx.x, t = threadIdx.x;
+    const int I = I0 + blockIdx.x, t = threadIdx.x;
     const i64 r0 = (i64)I * kSB;
     const int bs = (int)min<i64>(kSB, n - r0);
     for (int idx = t; idx < kSB * kSB; idx += 256) {
@@ -302,10 +302,15 @@ size_t solve_ctrl_ints(int n, int kc) {
 }
 
 template <typename T>
-hipError_t launch_trtri_diag64(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(trtri_diag64_kernel<T>, dim3((unsigned)ceil_div(n, kSB)), dim3(256), 0, s, n, L, ldl, Dinv);
+hipError_t launch_trtri_diag64_range(int n, int I0, int I1, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
+    I1 = (int)std::min<i64>(I1, ceil_div(n, kSB));
+    if (n <= 0 || I0 >= I1) return hipSuccess;
+    hipLaunchKernelGGL(trtri_diag64_kernel<T>, dim3((unsigned)(I1 - I0)), dim3(256), 0, s, n, I0, L, ldl, Dinv);
     return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_trtri_diag64(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
+    return launch_trtri_diag64_range<T>(n, 0, (int)ceil_div(n, kSB), L, ldl, Dinv, s);
 }
 
 template <typename T>
@@ -345,6 +350,7 @@ size_t solve_dinv_count(int n) { return (size_t)ceil_div(n, kSB) * kSB * kSB; }
 
 template hipError_t launch_trtri_diag64<double>(int, const double*, i64, double*, hipStream_t);
 template hipError_t launch_trtri_diag64<float>(int, const float*, i64, float*, hipStream_t);
+template hipError_t launch_trtri_diag64_range<float>(int, int, int, const float*, i64, float*, hipStream_t);
 template hipError_t launch_trsm_persist<double>(bool, int, int, const double*, i64, double*, i64, const double*, int*,
                                                 const int*, hipStream_t);
 template hipError_t launch_trsm_persist<float>(bool, int, int, const float*, i64, float*, i64, const float*, int*,

@@ -109,11 +109,11 @@ __global__ __launch_bounds__(256) void sweep_mirror_kernel(int n, float* __restr
 // or columns past n), column slab c of it per workgroup, from the inverses of
 // the 64-row diagonal blocks:  X_cc = Dinv_c,
 //     X_ic = -Dinv_i sum_{c <= l < i} L_il X_lc   (i > c).
-__global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, const float* __restrict__ S, i64 lds,
+__global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, int b0, const float* __restrict__ S, i64 lds,
                                                          const float* __restrict__ Dinv, float* __restrict__ Xp) {
     __shared__ float sA[64][65];  // [row][k]
     __shared__ float sB[64][65];  // [k][col]
-    const int c = blockIdx.x, b = blockIdx.y;
+    const int c = blockIdx.x, b = b0 + blockIdx.y;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
     const int nT64 = (int)ceil_div(n, 64);
     float* X = Xp + (i64)b * kSwNB * kSwNB;
@@ -169,16 +169,18 @@ __global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, const float* __r
 // for direction dir (0 forward, 1 backward), step s, term i (0: the diagonal
 // inverse, i >= 1: the look-ahead product with the block i steps back).
 // One 64 x 64 sub-tile (ra, rc) per workgroup.
-__global__ __launch_bounds__(256) void sweep_tiles_kernel(int n, const float* __restrict__ S, i64 lds,
-                                                          const float* __restrict__ Xp, float* __restrict__ tiles) {
+__global__ __launch_bounds__(256) void sweep_tiles_kernel(int n, int b0, int nbk, const float* __restrict__ S,
+                                                          i64 lds, const float* __restrict__ Xp,
+                                                          float* __restrict__ tiles) {
     __shared__ float sA[64][65];  // [a][q]
     __shared__ float sB[64][65];  // [q][c]
     const int T = (int)ceil_div(n, kSwNB);
     const int ra = blockIdx.x / kSwR, rc = blockIdx.x % kSwR;
-    const int i = blockIdx.y / T, s = blockIdx.y % T, dir = blockIdx.z;
+    // blocks b0 .. b0+nbk-1 (block b is step b forward, step T-1-b backward)
+    const int i = blockIdx.y / nbk, b = b0 + blockIdx.y % nbk, dir = blockIdx.z;
+    const int s = dir ? T - 1 - b : b;
     if (i > s) return;  // term i of step s exists for s >= i only
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
-    const int b = dir ? T - 1 - s : s;
     const int bb = dir ? b + i : b - i;
     const float* X = Xp + (i64)b * kSwNB * kSwNB;
     float* out = tiles + (((i64)dir * (kSwLook + 1) + i) * T + s) * kSwNB * kSwNB;
@@ -473,14 +475,22 @@ hipError_t launch_sweep_mirror(int n, float* S, i64 lds, hipStream_t s) {
     return hipGetLastError();
 }
 
+int sweep_block_cols() { return kSwNB; }
+
+hipError_t launch_sweep_prepare_blocks(int n, int b0, int b1, const float* S, i64 lds, const float* Dinv, float* Xp,
+                                       float* tiles, hipStream_t s) {
+    b1 = (int)std::min<i64>(b1, ceil_div(n, kSwNB));
+    if (n <= 0 || b0 >= b1) return hipSuccess;
+    const int nbk = b1 - b0;
+    hipLaunchKernelGGL(sweep_xinv_kernel, dim3(kSwR, (unsigned)nbk), dim3(256), 0, s, n, b0, S, lds, Dinv, Xp);
+    hipLaunchKernelGGL(sweep_tiles_kernel, dim3(kSwR * kSwR, (unsigned)((kSwLook + 1) * nbk), 2), dim3(256), 0, s, n,
+                       b0, nbk, S, lds, Xp, tiles);
+    return hipGetLastError();
+}
+
 hipError_t launch_sweep_prepare(int n, const float* S, i64 lds, const float* Dinv, float* Xp, float* tiles,
                                 hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    const unsigned T = (unsigned)ceil_div(n, kSwNB);
-    hipLaunchKernelGGL(sweep_xinv_kernel, dim3(kSwR, T), dim3(256), 0, s, n, S, lds, Dinv, Xp);
-    hipLaunchKernelGGL(sweep_tiles_kernel, dim3(kSwR * kSwR, (kSwLook + 1) * T, 2), dim3(256), 0, s, n, S, lds, Xp,
-                       tiles);
-    return hipGetLastError();
+    return launch_sweep_prepare_blocks(n, 0, (int)ceil_div(n, kSwNB), S, lds, Dinv, Xp, tiles, s);
 }
 
 template <int NT>
