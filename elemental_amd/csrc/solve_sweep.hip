@@ -131,17 +131,37 @@ __global__ __launch_bounds__(256) void sweep_xinv_kernel(int n, int b0, const fl
         X[(64 * c + r) + (i64)(64 * c + q) * kSwNB] = dinv(c, r, q);
     }
     __syncthreads();
+    // The products (i, l), i = c+1 .. R-1, l = c .. i-1, in that order; the
+    // next one's operands load into registers while the current one
+    // multiplies.  X(l, c) for l < i is final before row i starts (stored,
+    // then a barrier), and the load of X(i-1, c) is issued in row i.
+    float ga[16], gb[16];
+    auto gload = [&](int i, int l) __attribute__((always_inline)) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int kk = w + 4 * q;
+            const i64 row = base + 64 * i + lane, col = base + 64 * l + kk;
+            ga[q] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // L(64i + lane, 64l + kk)
+            gb[q] = X[(64 * l + lane) + (i64)(64 * c + kk) * kSwNB];     // X(64l + lane, 64c + kk)
+        }
+    };
+    if (c + 1 < kSwR) gload(c + 1, c);
     for (int i = c + 1; i < kSwR; ++i) {
         f4_t acc[4] = {};
         for (int l = c; l < i; ++l) {
 #pragma unroll
             for (int q = 0; q < 16; ++q) {
                 const int kk = w + 4 * q;
-                const i64 row = base + 64 * i + lane, col = base + 64 * l + kk;
-                sA[lane][kk] = (row < n && col < n) ? S[row + col * lds] : 0.0f;  // L(64i + lane, 64l + kk)
-                sB[lane][kk] = X[(64 * l + lane) + (i64)(64 * c + kk) * kSwNB];     // X(64l + lane, 64c + kk)
+                sA[lane][kk] = ga[q];
+                sB[lane][kk] = gb[q];
             }
             __syncthreads();
+            // the next product: (i, l+1), else (i+1, c); after the last one
+            // the current operands again (in bounds, never stored)
+            int ni = i, nl = l + 1;
+            if (nl >= i) ni = i + 1, nl = c;
+            if (ni >= kSwR) ni = i, nl = l;
+            gload(ni, nl);
             lds_tile_product(acc, sA, sB, w, m, g);
             __syncthreads();
         }
