@@ -733,6 +733,16 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
         }
         int rc = cholesky_lower<float>(n, f.S, n, o.nb, o.trailing, f.dinfo, s, ready, &prep);
         if (rc == kOk && prepared < nT) rc = prep(n);
+        if (rc == kOk) {
+            // the mirror (upper triangle) on aux before the host reads the
+            // breakdown flag; it reads only the lower triangle, and after a
+            // breakdown the next attempt's mirror follows it on aux
+            rc = hipEventRecord(sp->ev_a, s) == hipSuccess && hipStreamWaitEvent(sp->aux, sp->ev_a, 0) == hipSuccess &&
+                         launch_sweep_mirror(n, f.S, n, sp->aux) == hipSuccess &&
+                         hipEventRecord(sp->ev_b, sp->aux) == hipSuccess
+                     ? kOk
+                     : kErrHip;
+        }
         if (rc) {  // the work on aux and aux3 ends before the caller's stream moves on
             if (ready != nullptr) (void)hipStreamWaitEvent(s, ready, 0);
             (void)join_prep();
@@ -746,18 +756,10 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
         sigma = (sigma == 0.0) ? 1e-5 : sigma * 10.0;
         f.shifted = 1;
     }
-    if (info != 0) {  // broke down even after shifting
-        (void)join_prep();
-        return kErrNotConverged;
-    }
-    // the mirror (upper triangle) on aux; the preparation on aux3 reads the
-    // lower triangle only
-    ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_a, s));
-    ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux, sp->ev_a, 0));
-    ELEMAMD_HIP_TRY(launch_sweep_mirror(n, f.S, n, sp->aux));
-    ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_b, sp->aux));
-    if (int rc = join_prep()) return rc;
+    // the caller's stream waits for the mirror (aux) and the preparation (aux3)
     ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
+    if (int rc = join_prep()) return rc;
+    if (info != 0) return kErrNotConverged;  // broke down even after shifting
     return kOk;
 }
 
