@@ -60,7 +60,7 @@ def parse():
                    help="refinement of the mixed solve: classic IR or preconditioned CG")
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline")
-    p.add_argument("--cpu-runs", type=int, default=3)
+    p.add_argument("--cpu-runs", type=int, default=5)
     p.add_argument("--mode", default="single", choices=["single", "batch", "sweep", "serve", "roofline"])
     p.add_argument("--count", type=int, default=8192, help="batch: number of matrices")
     p.add_argument("--requests", type=int, default=160, help="serve: requests per arrival rate")
@@ -206,20 +206,42 @@ def bench_roofline(args):
     return out
 
 
+def host_cpu_facts():
+    """CPU model, online CPUs and the affinity mask of this process."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, affinity
+
+
 def cpu_baseline(A_host, B_host, runs):
     """The oracle's LVar3 (nb=128, scalar LVar3Unb on diagonal blocks, host
     BLAS dtrsm/dsyrk) + SolveAfter, on the same matrix, median of `runs`
-    after one warm-up, with a residual self-check per run."""
+    after one warm-up, with a residual self-check per run.  Threads: every
+    core of the affinity mask, capped by OMP_NUM_THREADS when it is set (the
+    host's CPU share per GPU).  Returns (record, X of the last run)."""
     from oracle import oracle as O
 
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    blas = O.bind_host_blas(cores)
+    model, online, affinity = host_cpu_facts()
+    cores = affinity
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))
+    O.bind_host_blas(cores)
+    blas = O.host_blas_info()
     n, k = B_host.shape
     times = []
+    X = None
     for r in range(runs + 1):
         F, X = A_host.copy(order="F"), B_host.copy(order="F")
         t0 = time.perf_counter()
@@ -232,10 +254,44 @@ def cpu_baseline(A_host, B_host, runs):
         assert rel < 1e-14, rel
         if r > 0:
             times.append(dt)
+        del F
     med = float(np.median(times))
-    return {"value": 1.0 / med, "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"{runs} full factor+solve runs of the same n={n} k={k} matrix after 1 warm-up, median "
-                      f"{med:.3f} s; oracle LVar3(nb=128)+SolveAfter over {blas}"}
+    rec = {"value": 1.0 / med, "unit": "solves/s", "cores": cores, "kind": "port",
+           "sample": f"{runs} full factor+solve runs of the same n={n} k={k} matrix after 1 warm-up, median "
+                     f"{med:.3f} s (min {min(times):.3f}, max {max(times):.3f}); oracle LVar3(nb=128)+SolveAfter "
+                     f"over {blas.get('config', blas['blas'])} ({blas.get('threading', '?')} threading, "
+                     f"{cores} threads); CPU {model}, {online} online, {affinity} in the affinity mask"
+                     + (f", OMP_NUM_THREADS={omp}" if omp else ""),
+           "cpu_model": model, "cpus_online": online, "cpus_affinity": affinity,
+           "blas": blas}
+    return rec, X
+
+
+def vendor_fp64(A, B, steps, warmup, stream):
+    """torch.linalg.cholesky + torch.cholesky_solve (ROCm's rocSOLVER /
+    hipBLAS) on the same matrix: a reference point beside the plain FP64
+    path, not a target."""
+    import torch
+
+    At, Bt = A.tensor(), B.tensor()
+    ms = []
+    for it in range(warmup + max(1, min(steps, 5))):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        Lt = torch.linalg.cholesky(At)
+        Xt = torch.cholesky_solve(Bt, Lt)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        if it >= warmup:
+            ms.append(e0.elapsed_time(e1))
+        del Lt
+    m = float(np.median(ms))
+    n = At.shape[0]
+    del Xt
+    torch.cuda.empty_cache()
+    return {"what": "torch.linalg.cholesky + torch.cholesky_solve, FP64", "ms": round(m, 3),
+            "solves_per_s": round(1e3 / m, 4), "frac_fp64_peak": round(n ** 3 / 3 / (m * 1e-3) / PEAK["f64"], 4)}
 
 
 # ------------------------------------------------------------ other modes
@@ -520,6 +576,10 @@ def main():
     plain_ms = [run_plain() for _ in range(args.steps)]
     eta_plain = eta_inf(A, X, B, anorm)
     Xplain = X.tensor().clone()
+    try:
+        vendor = vendor_fp64(A, B, args.steps, args.warmup, stream)
+    except Exception as e:  # a reference point only: its failure is reported, not fatal
+        vendor = {"error": f"{type(e).__name__}: {e}"[:200]}
 
     # ---- mixed + IR (headline)
     for _ in range(args.warmup):
@@ -546,13 +606,31 @@ def main():
     achieved = flops_launch / (avg_ms * 1e-3)
     avg64, fl64, _ = syrk_roofline(n, args.nb, "f64", stream)
 
-    # ---- CPU baseline (oracle, host cores), bounded sample
+    # ---- CPU baseline (oracle, host cores), bounded sample; its X is the
+    # FP64 oracle solution the GPU results are compared with
     cpu = None
+    vs_oracle = None
     if args.cpu_baseline:
         Ah = np.asfortranarray(A.tensor().cpu().numpy())
         Bh = np.asfortranarray(B.tensor().cpu().numpy())
-        cpu = cpu_baseline(Ah, Bh, args.cpu_runs)
-        del Ah, Bh
+        cpu, Xo = cpu_baseline(Ah, Bh, args.cpu_runs)
+        Xm = X.tensor().cpu().numpy()
+        Xp = Xplain.cpu().numpy()
+        an = float(np.abs(Ah).sum(axis=1).max())
+
+        def eta_host(Xs):
+            R = Bh - Ah @ Xs
+            return float(max(np.abs(R[:, c]).max() / (an * np.abs(Xs[:, c]).max() + np.abs(Bh[:, c]).max())
+                             for c in range(k)))
+
+        no = float(np.linalg.norm(Xo))
+        vs_oracle = {"oracle": "CPU LVar3(nb=128)+SolveAfter in FP64 (the cpu_baseline run), same A and B",
+                     "forward_error_mixed": float(np.linalg.norm(Xm - Xo) / no),
+                     "forward_error_plain_fp64": float(np.linalg.norm(Xp - Xo) / no),
+                     "backward_error_mixed_host": eta_host(Xm),
+                     "backward_error_plain_fp64_host": eta_host(Xp),
+                     "backward_error_oracle_host": eta_host(Xo)}
+        del Ah, Bh, Xo
 
     out = {
         "metric": "HPD solves/s + p99 latency at FP64 backward err <=1e-14: mixed+IR vs plain FP64",
@@ -583,7 +661,9 @@ def main():
             "solves_per_s": round(1e3 / mean_plain, 4), "p50_ms": round(pct(plain_ms, 50), 3),
             "p99_ms": round(pct(plain_ms, 99), 3), "backward_error_inf": eta_plain,
             "gflops": round((flops_factor + 2 * n * n * k) / (mean_plain * 1e-3) / 1e9, 1),
+            "frac_fp64_peak": round(flops_factor / (mean_plain * 1e-3) / PEAK["f64"], 4),
         },
+        "vendor_fp64": vendor,
         "speedup_mixed_vs_fp64": round(mean_plain / mean_mixed, 3),
         "roofline": {
             "bound": "mfma", "kernel": f"syrk_ln_{args.trailing}", "achieved": achieved / 1e12,
@@ -596,6 +676,7 @@ def main():
                           "traffic": profiled_traffic(n, args.nb, "f64")},
         },
         "cpu_baseline": cpu,
+        "vs_fp64_oracle": vs_oracle,
         "input_generation_s": round(gen_s, 2),
     }
     print(json.dumps(out))

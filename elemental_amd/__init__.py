@@ -95,19 +95,29 @@ def _raise_info(info: int):
         raise RuntimeError(f"reading the factorization status failed ({info})")
 
 
+def _is_f32(A: Matrix) -> bool:
+    import torch
+
+    if A.dtype == torch.float32:
+        return True
+    if A.dtype != torch.float64:
+        raise TypeError(f"Matrix<{A.dtype}> is not on this path (float64 and float32 are)")
+    return False
+
+
 def Cholesky(uplo: UpperOrLower, A: Matrix, nb: int | None = None, stream=None) -> None:
-    """elem::Cholesky(uplo, Matrix<F>& A) (Cholesky.hpp:49-62): A's lower
-    triangle becomes L (upper triangle untouched).  Raises LogicError
-    ("A was not numerically HPD") like the reference."""
+    """elem::Cholesky(uplo, Matrix<F>& A) (Cholesky.hpp:49-62): LOWER runs
+    cholesky::LVar3 (A's lower triangle becomes L, the upper one is
+    untouched), UPPER cholesky::UVar3 (the upper triangle becomes U with
+    A = U^T U, the lower one is untouched).  F = float64 or float32.  Raises
+    LogicError ("A was not numerically HPD") like the reference."""
     if A.Height() != A.Width():
         raise LogicError("A must be square")
-    if uplo is not LOWER:
-        raise NotImplementedError("only LOWER is on this path")
     nb = nb or _DEFAULT_BLOCKSIZE
     d = _dinfo()
     s = _stream(stream)
-    _check(lib().elemamd_cholesky_f64(b"L", A.Height(), A.ptr(), A.LDim(), nb, ctypes.c_void_p(d.data_ptr()), s),
-           "elemamd_cholesky_f64")
+    fn = lib().elemamd_cholesky_f32 if _is_f32(A) else lib().elemamd_cholesky_f64
+    _check(fn(_c(uplo), A.Height(), A.ptr(), A.LDim(), nb, ctypes.c_void_p(d.data_ptr()), s), "elemamd_cholesky")
     _raise_info(lib().elemamd_info_read(ctypes.c_void_p(d.data_ptr()), s))
 
 
@@ -119,15 +129,17 @@ def LocalCholesky(uplo: UpperOrLower, A: Matrix, nb: int | None = None, stream=N
 
 def SolveAfter(uplo: UpperOrLower, orientation: Orientation, A: Matrix, B: Matrix, stream=None) -> None:
     """cholesky::SolveAfter (SolveAfter.hpp:20-75): B := A^{-1} B from the
-    factor stored in A."""
+    factor stored in A's uplo triangle."""
     if A.Height() != A.Width():
         raise LogicError("A must be square")
     if A.Height() != B.Height():
         raise LogicError("A and B must be the same height")
-    if uplo is not LOWER:
-        raise NotImplementedError("only LOWER is on this path")
-    _check(lib().elemamd_solve_after_f64(b"L", _c(orientation), A.Height(), B.Width(), A.ptr(), A.LDim(), B.ptr(),
-                                         B.LDim(), _stream(stream)), "elemamd_solve_after_f64")
+    f32 = _is_f32(A)
+    if f32 != _is_f32(B):
+        raise TypeError("A and B must have the same element type")
+    fn = lib().elemamd_solve_after_f32 if f32 else lib().elemamd_solve_after_f64
+    _check(fn(_c(uplo), _c(orientation), A.Height(), B.Width(), A.ptr(), A.LDim(), B.ptr(), B.LDim(),
+              _stream(stream)), "elemamd_solve_after")
 
 
 def HPDSolve(uplo: UpperOrLower, orientation: Orientation, A: Matrix, B: Matrix, nb: int | None = None,
@@ -138,13 +150,15 @@ def HPDSolve(uplo: UpperOrLower, orientation: Orientation, A: Matrix, B: Matrix,
         raise LogicError("A must be square")
     if A.Height() != B.Height():
         raise LogicError("A and B must be the same height")
-    if uplo is not LOWER:
-        raise NotImplementedError("only LOWER is on this path")
+    f32 = _is_f32(A)
+    if f32 != _is_f32(B):
+        raise TypeError("A and B must have the same element type")
     nb = nb or _DEFAULT_BLOCKSIZE
     d = _dinfo()
     s = _stream(stream)
-    _check(lib().elemamd_hpd_solve_f64(b"L", _c(orientation), A.Height(), B.Width(), A.ptr(), A.LDim(), B.ptr(),
-                                       B.LDim(), nb, ctypes.c_void_p(d.data_ptr()), s), "elemamd_hpd_solve_f64")
+    fn = lib().elemamd_hpd_solve_f32 if f32 else lib().elemamd_hpd_solve_f64
+    _check(fn(_c(uplo), _c(orientation), A.Height(), B.Width(), A.ptr(), A.LDim(), B.ptr(), B.LDim(), nb,
+              ctypes.c_void_p(d.data_ptr()), s), "elemamd_hpd_solve")
     _raise_info(lib().elemamd_info_read(ctypes.c_void_p(d.data_ptr()), s))
 
 
@@ -165,11 +179,13 @@ def Trsm(side: LeftOrRight, uplo: UpperOrLower, orientation: Orientation, diag: 
 
 def Herk(uplo: UpperOrLower, orientation: Orientation, alpha: float, A: Matrix, beta: float, C: Matrix,
          stream=None) -> None:
-    """elem::Herk(Matrix) (Herk.hpp:17-27): C := alpha A A^H + beta C on
-    the uplo triangle (real data: Syrk)."""
-    if A.Height() != C.Height() or C.Height() != C.Width():
+    """elem::Herk(Matrix) (Herk.hpp:17-27): C := alpha op(A) op(A)^H + beta C
+    on the uplo triangle (real data: Syrk); op(A) = A (NORMAL, A n x k) or
+    A^T (A k x n)."""
+    n, k = (A.Height(), A.Width()) if orientation is NORMAL else (A.Width(), A.Height())
+    if n != C.Height() or C.Height() != C.Width():
         raise LogicError("Nonconformal Syrk")
-    _check(lib().elemamd_herk_f64(_c(uplo), _c(orientation), C.Height(), A.Width(), float(alpha), A.ptr(), A.LDim(),
+    _check(lib().elemamd_herk_f64(_c(uplo), _c(orientation), n, k, float(alpha), A.ptr(), A.LDim(),
                                   float(beta), C.ptr(), C.LDim(), _stream(stream)), "elemamd_herk_f64")
 
 
@@ -178,9 +194,10 @@ Syrk = Herk  # identical for real data (Syrk.hpp:20-57 with conjugate=false)
 
 def Trrk(uplo: UpperOrLower, orientationOfA: Orientation, orientationOfB: Orientation, alpha: float, A: Matrix,
          B: Matrix, beta: float, C: Matrix, stream=None) -> None:
-    """elem::Trrk(Matrix) (src/blas-like/Trrk.cpp:19-38), TN case: the
-    uplo trapezoid of C := alpha op(A) op(B) + beta C with A, B k x n."""
-    _check(lib().elemamd_trrk_f64(_c(uplo), _c(orientationOfA), _c(orientationOfB), C.Height(), A.Height(),
+    """elem::Trrk(Matrix) (src/blas-like/Trrk.cpp:19-38): the uplo triangle
+    of C := alpha op(A) op(B) + beta C, op(A) n x k, op(B) k x n."""
+    k = A.Height() if orientationOfA is not NORMAL else A.Width()
+    _check(lib().elemamd_trrk_f64(_c(uplo), _c(orientationOfA), _c(orientationOfB), C.Height(), k,
                                   float(alpha), A.ptr(), A.LDim(), B.ptr(), B.LDim(), float(beta), C.ptr(), C.LDim(),
                                   _stream(stream)), "elemamd_trrk_f64")
 

@@ -88,6 +88,9 @@ SIGNATURES = {
     "elemamd_cholesky_f64": (_i, [_c, _i, _vp, _i64, _i, _vp, _vp]),
     "elemamd_solve_after_f64": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _vp]),
     "elemamd_hpd_solve_f64": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _i, _vp, _vp]),
+    "elemamd_cholesky_f32": (_i, [_c, _i, _vp, _i64, _i, _vp, _vp]),
+    "elemamd_solve_after_f32": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _vp]),
+    "elemamd_hpd_solve_f32": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _i, _vp, _vp]),
     "elemamd_hpd_solve_mixed_workspace": (_sz, [_i, _i, ctypes.POINTER(IROpts)]),
     "elemamd_hpd_solve_mixed_f64": (_i, [_i, _i, _vp, _i64, _vp, _i64, ctypes.POINTER(IROpts), _vp, _sz,
                                          ctypes.POINTER(IRReportC), _vp]),

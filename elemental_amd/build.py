@@ -64,15 +64,25 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
-    # the C++ drop-in API test program (host code only, links the library)
-    test_src = os.path.join(ROOT, "tests", "cpp", "elem_api_test.cpp")
-    test_bin = os.path.join(ROOT, "tests", "cpp", "elem_api_test")
-    if os.path.exists(test_src):
-        cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), test_src, "-o", test_bin,
-               "-L" + HERE, "-lelemental_amd", "-Wl,-rpath,$ORIGIN/../../elemental_amd"]
+    # host programs over the C++ drop-in (they link the library): the
+    # Cholesky test driver with the reference harness's command line, and the
+    # API / I/O test programs
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    programs = [
+        (os.path.join(CSRC, "drivers", "cholesky_driver.cpp"), os.path.join(HERE, "bin", "Cholesky"), "$ORIGIN/.."),
+        (os.path.join(ROOT, "tests", "cpp", "elem_api_test.cpp"), os.path.join(ROOT, "tests", "cpp", "elem_api_test"),
+         "$ORIGIN/../../elemental_amd"),
+        (os.path.join(ROOT, "tests", "cpp", "io_test.cpp"), os.path.join(ROOT, "tests", "cpp", "io_test"),
+         "$ORIGIN/../../elemental_amd"),
+    ]
+    for src, out, rpath in programs:
+        if not os.path.exists(src):
+            continue
+        cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o", out,
+               "-L" + HERE, "-lelemental_amd", "-Wl,-rpath," + rpath]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError(f"building {test_src} failed:\n{r.stderr}")
+            raise RuntimeError(f"building {src} failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -54,6 +54,84 @@ hipError_t launch_fill_normal_f64(double* out, i64 rows, i64 cols, i64 ld, uint6
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------ transposes
+// A := A^T for an n x n matrix in place: the workgroup of tile pair (I, J),
+// I >= J, 32 x 32 tiles, loads both tiles (coalesced along columns), then
+// writes each one's transpose over the other through LDS.  Every element is
+// read and written once; the diagonal tiles transpose within themselves.
+// The UPPER factorization runs as the LOWER one on A^T
+// (cholesky::UVar3Unb on A is cholesky::LVar3Unb on A^T operation by
+// operation for real data).
+constexpr int kTT = 32;
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_inplace_kernel(int n, T* A, i64 lda, int nt) {
+    __shared__ T s1[kTT][kTT + 1], s2[kTT][kTT + 1];
+    // tile pair from the linear index over the lower triangle of tiles
+    const i64 e = blockIdx.x;
+    int I = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+    while ((i64)I * (I + 1) / 2 > e) --I;
+    while ((i64)(I + 1) * (I + 2) / 2 <= e) ++I;
+    const int J = (int)(e - (i64)I * (I + 1) / 2);
+    if (I >= nt) return;
+    const int t = threadIdx.x, r = t & 31, c0 = t >> 5;  // 8 columns per pass
+    const i64 i0 = (i64)I * kTT, j0 = (i64)J * kTT;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 8 * q;
+        if (i0 + r < n && j0 + c < n) s1[c][r] = A[(i0 + r) + (j0 + c) * lda];  // tile (I, J): s1[c][r] = A(i0+r, j0+c)
+        if (j0 + r < n && i0 + c < n) s2[c][r] = A[(j0 + r) + (i0 + c) * lda];  // tile (J, I)
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 8 * q;
+        // new A(i0+r, j0+c) = old A(j0+c, i0+r) = s2[r][c]; new A(j0+r, i0+c) = old A(i0+c, j0+r) = s1[r][c]
+        if (i0 + r < n && j0 + c < n) A[(i0 + r) + (j0 + c) * lda] = s2[r][c];
+        if (I != J && j0 + r < n && i0 + c < n) A[(j0 + r) + (i0 + c) * lda] = s1[r][c];
+    }
+}
+
+template <typename T>
+hipError_t launch_transpose_inplace(int n, T* A, i64 lda, hipStream_t s) {
+    if (n <= 1) return hipSuccess;
+    const int nt = (int)ceil_div(n, kTT);
+    const i64 pairs = (i64)nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(transpose_inplace_kernel<T>, dim3((unsigned)pairs), dim3(256), 0, s, n, A, lda, nt);
+    return hipGetLastError();
+}
+template hipError_t launch_transpose_inplace<double>(int, double*, i64, hipStream_t);
+template hipError_t launch_transpose_inplace<float>(int, float*, i64, hipStream_t);
+hipError_t launch_transpose_inplace_f64(int n, double* A, i64 lda, hipStream_t s) {
+    return launch_transpose_inplace<double>(n, A, lda, s);
+}
+
+// B (n x m) := A^T, A m x n.
+__global__ __launch_bounds__(256) void transpose_kernel(int m, int n, const double* __restrict__ A, i64 lda,
+                                                        double* __restrict__ B, i64 ldb) {
+    __shared__ double sT[kTT][kTT + 1];
+    const i64 i0 = (i64)blockIdx.x * kTT, j0 = (i64)blockIdx.y * kTT;
+    const int t = threadIdx.x, r = t & 31, c0 = t >> 5;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 8 * q;
+        if (i0 + r < m && j0 + c < n) sT[c][r] = A[(i0 + r) + (j0 + c) * lda];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 8 * q;
+        // B(j0 + r, i0 + c) = A(i0 + c, j0 + r) = sT[r][c]
+        if (j0 + r < n && i0 + c < m) B[(j0 + r) + (i0 + c) * ldb] = sT[r][c];
+    }
+}
+
+hipError_t launch_transpose_f64(int m, int n, const double* A, i64 lda, double* B, i64 ldb, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)ceil_div(m, kTT), (unsigned)ceil_div(n, kTT)), dim3(256), 0,
+                       s, m, n, A, lda, B, ldb);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------- elementwise 2-D
 __global__ void scale_f64_kernel(int m, int n, double alpha, double* A, i64 lda) {
     const i64 i = (i64)blockIdx.x * 256 + threadIdx.x;

@@ -25,13 +25,23 @@ inline char up(char c) { return (c >= 'a' && c <= 'z') ? (char)(c - 32) : c; }
 inline bool is_trans(char c) { c = up(c); return c == 'T' || c == 'C'; }
 inline bool valid_orient(char c) { c = up(c); return c == 'N' || c == 'T' || c == 'C'; }
 
-// One triangular solve with a lower L (forward, or backward with L^T) on
-// the calling stream; the per-stream scratch holds the persistent-solve flags.
-int solve_left_lower(bool trans, int n, int k, const double* L, i64 ldl, double* B, i64 ldb, hipStream_t s) {
-    void* work = stream_scratch(solve_work_bytes<double>(n, k), s);
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// B := inv(op(T)) B for an n x n triangular T stored in the lower (upper =
+// false) or upper triangle, op = N or T, on the calling stream; `work` holds
+// solve_work_bytes<double>(n, k) bytes (null: the per-stream scratch).
+// Lower N and upper T run the forward chain, lower T and upper N the
+// backward one.  Synchronous: a persistent solve whose bounded wait gave up
+// is reported as ELEMAMD_ERR_HIP.
+int solve_left(bool upper, bool trans, bool unit, int n, int k, const double* L, i64 ldl, double* B, i64 ldb,
+               void* work, hipStream_t s) {
+    if (work == nullptr) work = stream_scratch(solve_work_bytes<double>(n, k), s);
     if (work == nullptr) return ELEMAMD_ERR_HIP;
-    int* ctrl = (int*)((char*)work + ((solve_dinv_count(n) * sizeof(double) + 255) & ~(size_t)255));
-    return solve_apply<double>(!trans, trans, n, k, L, ldl, B, ldb, nullptr, ctrl, nullptr, s);
+    int* ctrl = solve_ctrl<double>(n, work);
+    const bool backward = upper != trans;
+    const int rc = solve_apply<double>(!backward, backward, n, k, L, ldl, B, ldb, nullptr, ctrl, nullptr, s, upper,
+                                       unit);
+    return rc ? rc : solve_read_error(ctrl, s);
 }
 
 }  // namespace
@@ -58,29 +68,62 @@ int elemamd_trsm_f64(char side, char uplo, char trans, char diag, int m, int n, 
     side = up(side);
     uplo = up(uplo);
     diag = up(diag);
-    if (uplo != 'L' || diag != 'N' || !valid_orient(trans)) return ELEMAMD_ERR_ARG;
+    // every argument is validated before anything is enqueued, so an
+    // ELEMAMD_ERR_ARG return leaves B untouched
+    if ((uplo != 'L' && uplo != 'U') || (diag != 'N' && diag != 'U') || !valid_orient(trans)) return ELEMAMD_ERR_ARG;
+    if (side != 'L' && side != 'R') return ELEMAMD_ERR_ARG;
     if (m < 0 || n < 0 || ldb < std::max(1, m)) return ELEMAMD_ERR_ARG;
     const int na = (side == 'L') ? m : n;
     if (lda < std::max(1, na)) return ELEMAMD_ERR_ARG;
     if (m == 0 || n == 0) return ELEMAMD_OK;
-    if (alpha != 1.0 && launch_scale_f64(m, n, alpha, B, ldb, S(stream)) != hipSuccess) return ELEMAMD_ERR_HIP;
-    if (side == 'R') {
-        if (!is_trans(trans) || n > 128) return ELEMAMD_ERR_ARG;
-        return launch_trsm_rlt<double>(m, n, A, lda, B, ldb, nullptr, kNoLimit, S(stream)) == hipSuccess ? ELEMAMD_OK
-                                                                                               : ELEMAMD_ERR_HIP;
-    }
-    if (side != 'L') return ELEMAMD_ERR_ARG;
-    return solve_left_lower(is_trans(trans), m, n, A, lda, B, ldb, S(stream));
+    hipStream_t s = S(stream);
+    const bool upper = uplo == 'U', tr = is_trans(trans), unit = diag == 'U';
+    // the Cholesky panel form keeps the reference BLAS column order (bit-exact)
+    const bool panel = side == 'R' && !upper && tr && !unit && n <= 128;
+    if (alpha != 1.0 && launch_scale_f64(m, n, alpha, B, ldb, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    if (panel)
+        return launch_trsm_rlt<double>(m, n, A, lda, B, ldb, nullptr, kNoLimit, s) == hipSuccess ? ELEMAMD_OK
+                                                                                             : ELEMAMD_ERR_HIP;
+    if (side == 'L') return solve_left(upper, tr, unit, m, n, A, lda, B, ldb, nullptr, s);
+    // X op(T) = B  <=>  op(T)^T X^T = B^T: a left solve of B^T (n x m) with
+    // the other orientation, through a transposed copy
+    const size_t bt = al256(sizeof(double) * (size_t)n * m);
+    char* w = (char*)stream_scratch(bt + solve_work_bytes<double>(n, m), s);
+    if (w == nullptr) return ELEMAMD_ERR_HIP;
+    double* Bt = (double*)w;
+    if (launch_transpose_f64(m, n, B, ldb, Bt, n, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    const int rc = solve_left(upper, !tr, unit, n, m, A, lda, Bt, n, w + bt, s);
+    if (rc) return rc;
+    return launch_transpose_f64(n, m, Bt, n, B, ldb, s) == hipSuccess ? ELEMAMD_OK : ELEMAMD_ERR_HIP;
 }
 
 int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const double* A, int64_t lda, double beta,
                      double* C, int64_t ldc, void* stream) {
-    if (up(uplo) != 'L' || up(trans) != 'N') return ELEMAMD_ERR_ARG;
-    if (n < 0 || k < 0 || lda < std::max(1, n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
+    uplo = up(uplo);
+    if ((uplo != 'L' && uplo != 'U') || !valid_orient(trans)) return ELEMAMD_ERR_ARG;
+    const bool tr = is_trans(trans);
+    if (n < 0 || k < 0 || lda < std::max(1, tr ? k : n) || ldc < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
-    return launch_syrk_ln_f64(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr, 0, 1, S(stream)) == hipSuccess
+    hipStream_t s = S(stream);
+    if (uplo == 'L' && !tr)  // the Cholesky trailing update's form
+        return launch_syrk_ln_f64(n, n, k, alpha, A, lda, A, lda, beta, C, ldc, nullptr, 0, 1, s) == hipSuccess
+                   ? ELEMAMD_OK
+                   : ELEMAMD_ERR_HIP;
+    // op(A) op(A)^T = At^T At with At = op(A)^T (k x n): the TN product on
+    // the requested triangle; trans 'N' goes through a transposed copy
+    const double* At = A;
+    i64 ldat = lda;
+    if (!tr) {
+        double* w = (double*)stream_scratch(sizeof(double) * (size_t)std::max(1, k) * n, s);
+        if (w == nullptr) return ELEMAMD_ERR_HIP;
+        if (k > 0 && launch_transpose_f64(n, k, A, lda, w, std::max(1, k), s) != hipSuccess) return ELEMAMD_ERR_HIP;
+        At = w;
+        ldat = std::max(1, k);
+    }
+    return launch_gemm_tn_f64(n, n, k, alpha, At, ldat, At, ldat, beta, C, ldc, uplo == 'L' ? 1 : 2, s) ==
+                   hipSuccess
                ? ELEMAMD_OK
-                                                                                                  : ELEMAMD_ERR_HIP;
+               : ELEMAMD_ERR_HIP;
 }
 
 int elemamd_herk_f32(int mode, int n, int k, float alpha, const float* A, int64_t lda, float beta, float* C,
@@ -110,46 +153,167 @@ int elemamd_herk_f32(int mode, int n, int k, float alpha, const float* A, int64_
 
 int elemamd_trrk_f64(char uplo, char orientA, char orientB, int n, int k, double alpha, const double* A, int64_t lda,
                      const double* B, int64_t ldb, double beta, double* C, int64_t ldc, void* stream) {
-    if (up(uplo) != 'L' || !is_trans(orientA) || up(orientB) != 'N') return ELEMAMD_ERR_ARG;
-    if (n < 0 || k < 0 || lda < std::max(1, k) || ldb < std::max(1, k) || ldc < std::max(1, n))
+    uplo = up(uplo);
+    if ((uplo != 'L' && uplo != 'U') || !valid_orient(orientA) || !valid_orient(orientB)) return ELEMAMD_ERR_ARG;
+    const bool ta = is_trans(orientA), tb = is_trans(orientB);
+    // op(A) is n x k (A: k x n when transposed), op(B) is k x n (B: n x k when transposed)
+    if (n < 0 || k < 0 || lda < std::max(1, ta ? k : n) || ldb < std::max(1, tb ? n : k) || ldc < std::max(1, n))
         return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
-    return launch_gemm_tn_f64(n, n, k, alpha, A, lda, B, ldb, beta, C, ldc, true, S(stream)) == hipSuccess
+    hipStream_t s = S(stream);
+    // the TN product takes op(A)^T and op(B) as k x n column-major operands;
+    // the other orientations are transposed into scratch first
+    const size_t one = al256(sizeof(double) * (size_t)std::max(1, k) * n);
+    char* w = (!ta || tb) ? (char*)stream_scratch(2 * one, s) : nullptr;
+    if ((!ta || tb) && w == nullptr) return ELEMAMD_ERR_HIP;
+    const int ldk = std::max(1, k);
+    const double* Ao = A;
+    i64 ldao = lda;
+    if (!ta) {
+        if (k > 0 && launch_transpose_f64(n, k, A, lda, (double*)w, ldk, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+        Ao = (const double*)w;
+        ldao = ldk;
+    }
+    const double* Bo = B;
+    i64 ldbo = ldb;
+    if (tb) {
+        if (k > 0 && launch_transpose_f64(n, k, B, ldb, (double*)(w + one), ldk, s) != hipSuccess)
+            return ELEMAMD_ERR_HIP;
+        Bo = (const double*)(w + one);
+        ldbo = ldk;
+    }
+    return launch_gemm_tn_f64(n, n, k, alpha, Ao, ldao, Bo, ldbo, beta, C, ldc, uplo == 'L' ? 1 : 2, s) ==
+                   hipSuccess
                ? ELEMAMD_OK
                : ELEMAMD_ERR_HIP;
 }
 
 int elemamd_trsv_f64(char uplo, char trans, char diag, int n, const double* A, int64_t lda, double* x, int64_t incx,
                      void* stream) {
-    if (up(uplo) != 'L' || up(diag) != 'N' || !valid_orient(trans) || incx != 1) return ELEMAMD_ERR_ARG;
+    uplo = up(uplo);
+    diag = up(diag);
+    if ((uplo != 'L' && uplo != 'U') || (diag != 'N' && diag != 'U') || !valid_orient(trans) || incx != 1)
+        return ELEMAMD_ERR_ARG;
     if (n < 0 || lda < std::max(1, n)) return ELEMAMD_ERR_ARG;
     if (n == 0) return ELEMAMD_OK;
-    return solve_left_lower(is_trans(trans), n, 1, A, lda, x, std::max(1, n), S(stream));
+    return solve_left(uplo == 'U', is_trans(trans), diag == 'U', n, 1, A, lda, x, std::max(1, n), nullptr, S(stream));
 }
 
 int elemamd_cholesky_f64(char uplo, int n, double* A, int64_t lda, int nb, int* dinfo, void* stream) {
-    if (up(uplo) != 'L') return ELEMAMD_ERR_ARG;
-    return cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, S(stream));
+    uplo = up(uplo);
+    if (uplo != 'L' && uplo != 'U') return ELEMAMD_ERR_ARG;
+    if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr) return ELEMAMD_ERR_ARG;
+    hipStream_t s = S(stream);
+    if (uplo == 'L') return cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, s);
+    // cholesky::UVar3 on A is cholesky::LVar3 on A^T, operation by operation
+    // (real data): transpose in place, factor the lower triangle, transpose
+    // back.  The strictly lower triangle of A ends where it started.
+    if (launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    const int rc = cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, s);
+    if (launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    return rc;
 }
 
 int elemamd_solve_after_f64(char uplo, char orientation, int n, int k, const double* A, int64_t lda, double* B,
                             int64_t ldb, void* stream) {
-    if (up(uplo) != 'L' || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
+    uplo = up(uplo);
+    if ((uplo != 'L' && uplo != 'U') || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
     if (n < 0 || k < 0) return ELEMAMD_ERR_ARG;
     void* work = stream_scratch(solve_work_bytes<double>(std::max(n, 1), std::max(k, 1)), S(stream));
     if (work == nullptr) return ELEMAMD_ERR_HIP;
-    return solve_after_lower<double>(n, k, A, lda, B, ldb, nullptr, work, S(stream));
+    const int rc = solve_after_lower<double>(n, k, A, lda, B, ldb, nullptr, work, S(stream), uplo == 'U');
+    if (rc || n == 0 || k == 0) return rc;
+    return solve_read_error(solve_ctrl<double>(n, work), S(stream));
 }
 
 int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double* A, int64_t lda, double* B, int64_t ldb,
                           int nb, int* dinfo, void* stream) {
-    if (up(uplo) != 'L' || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
-    if (k < 0 || ldb < std::max(1, n)) return ELEMAMD_ERR_ARG;
-    void* work = stream_scratch(solve_work_bytes<double>(std::max(n, 1), std::max(k, 1)), S(stream));
+    uplo = up(uplo);
+    if ((uplo != 'L' && uplo != 'U') || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldb < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr)
+        return ELEMAMD_ERR_ARG;
+    hipStream_t s = S(stream);
+    void* work = stream_scratch(solve_work_bytes<double>(std::max(n, 1), std::max(k, 1)), s);
     if (work == nullptr) return ELEMAMD_ERR_HIP;
-    int rc = cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, S(stream));
-    if (rc) return rc;
-    return solve_after_lower<double>(n, k, A, lda, B, ldb, dinfo, work, S(stream));
+    const bool upper = uplo == 'U';
+    // UPPER: factor and solve on A^T (its lower triangle holds L = U^T), then
+    // transpose back
+    if (upper && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    int rc = cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, s);
+    if (rc == kOk) rc = solve_after_lower<double>(n, k, A, lda, B, ldb, dinfo, work, s);
+    if (upper && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    if (rc || n == 0 || k == 0) return rc;
+    return solve_read_error(solve_ctrl<double>(n, work), s);
+}
+
+// ------------------------------------------------------- Matrix<float>
+// Cholesky / SolveAfter / HPDSolve with F = float: the float factorization
+// (FP64 diagonal blocks, FP32 panels by products with the diagonal pieces'
+// inverses, FP32 MFMA trailing updates) and float solves with the 64 x 64
+// diagonal-block inverses.
+int elemamd_cholesky_f32(char uplo, int n, float* A, int64_t lda, int nb, int* dinfo, void* stream) {
+    uplo = up(uplo);
+    if (uplo != 'L' && uplo != 'U') return ELEMAMD_ERR_ARG;
+    if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr) return ELEMAMD_ERR_ARG;
+    hipStream_t s = S(stream);
+    if (uplo == 'U' && launch_transpose_inplace<float>(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    const int rc = cholesky_lower<float>(n, A, lda, nb, kTrailingF32, dinfo, s);
+    if (uplo == 'U' && launch_transpose_inplace<float>(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    return rc;
+}
+
+namespace {
+// float SolveAfter with the factor in the lower triangle of L; `extra` bytes
+// of the per-stream scratch precede the solve's own work (for a transposed copy)
+int solve_after_f32_lower(int n, int k, const float* L, i64 ldl, float* B, i64 ldb, const int* dinfo, char* work,
+                          hipStream_t s) {
+    const int rc = solve_after_lower<float>(n, k, L, ldl, B, ldb, dinfo, work, s);
+    if (rc || n == 0 || k == 0) return rc;
+    return solve_read_error(solve_ctrl<float>(n, work), s);
+}
+}  // namespace
+
+int elemamd_solve_after_f32(char uplo, char orientation, int n, int k, const float* A, int64_t lda, float* B,
+                            int64_t ldb, void* stream) {
+    uplo = up(uplo);
+    if ((uplo != 'L' && uplo != 'U') || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) return ELEMAMD_ERR_ARG;
+    if (n == 0 || k == 0) return ELEMAMD_OK;
+    hipStream_t s = S(stream);
+    // UPPER: the solves run on a transposed copy (L = U^T in its lower triangle)
+    const size_t tb = uplo == 'U' ? al256(sizeof(float) * (size_t)n * n) : 0;
+    char* w = (char*)stream_scratch(tb + solve_work_bytes<float>(n, k), s);
+    if (w == nullptr) return ELEMAMD_ERR_HIP;
+    const float* L = A;
+    i64 ldl = lda;
+    if (uplo == 'U') {
+        float* Lt = (float*)w;
+        if (hipMemcpy2DAsync(Lt, sizeof(float) * n, A, sizeof(float) * lda, sizeof(float) * n, n,
+                             hipMemcpyDeviceToDevice, s) != hipSuccess ||
+            launch_transpose_inplace<float>(n, Lt, n, s) != hipSuccess)
+            return ELEMAMD_ERR_HIP;
+        L = Lt;
+        ldl = n;
+    }
+    return solve_after_f32_lower(n, k, L, ldl, B, ldb, nullptr, w + tb, s);
+}
+
+int elemamd_hpd_solve_f32(char uplo, char orientation, int n, int k, float* A, int64_t lda, float* B, int64_t ldb,
+                          int nb, int* dinfo, void* stream) {
+    uplo = up(uplo);
+    if ((uplo != 'L' && uplo != 'U') || !valid_orient(orientation)) return ELEMAMD_ERR_ARG;
+    if (n < 0 || k < 0 || lda < std::max(1, n) || ldb < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr)
+        return ELEMAMD_ERR_ARG;
+    hipStream_t s = S(stream);
+    char* w = (char*)stream_scratch(solve_work_bytes<float>(std::max(n, 1), std::max(k, 1)), s);
+    if (w == nullptr) return ELEMAMD_ERR_HIP;
+    const bool upper = uplo == 'U';
+    if (upper && launch_transpose_inplace<float>(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    int rc = cholesky_lower<float>(n, A, lda, nb, kTrailingF32, dinfo, s);
+    if (rc == kOk && n > 0 && k > 0) rc = solve_after_lower<float>(n, k, A, lda, B, ldb, dinfo, w, s);
+    if (upper && launch_transpose_inplace<float>(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    if (rc || n == 0 || k == 0) return rc;
+    return solve_read_error(solve_ctrl<float>(n, w), s);
 }
 
 static MixedOpts to_opts(const elemamd_ir_opts* o) {

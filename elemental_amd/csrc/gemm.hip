@@ -1,5 +1,5 @@
 // C := beta*C + alpha * A^T * B with A k x m and B k x n (column-major),
-// optionally restricted to the lower triangle of a square C.
+// optionally restricted to the lower or the upper triangle of a square C.
 //
 // This is the product of Elemental's LocalTrrk / Trrk(LOWER, TRANSPOSE,
 // NORMAL, ...) (src/blas-like/Trrk/Local.hpp:733-775 and :931-976, leaf
@@ -34,12 +34,16 @@ __device__ inline void gemm_tri_tile(int t, int& ti, int& tj) {
 __global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k, double alpha,
                                                              const double* __restrict__ A, i64 lda,
                                                              const double* __restrict__ B, i64 ldb, double beta,
-                                                             double* __restrict__ C, i64 ldc, int lower_only) {
+                                                             double* __restrict__ C, i64 ldc, int tri) {
     __shared__ double sA[2][kGemmKC][kGemmLdp];  // sA[l][i] = A(l, i0+i)
     __shared__ double sB[2][kGemmKC][kGemmLdp];  // sB[l][j] = B(l, j0+j)
+    // tri: 0 every tile; 1 the lower triangle (tiles ti >= tj, j <= i);
+    // 2 the upper triangle (tiles ti <= tj, i <= j)
     int ti, tj;
-    if (lower_only) {
+    if (tri == 1) {
         gemm_tri_tile(blockIdx.x, ti, tj);
+    } else if (tri == 2) {
+        gemm_tri_tile(blockIdx.x, tj, ti);
     } else {
         ti = blockIdx.x;
         tj = blockIdx.y;
@@ -47,7 +51,7 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k
     const int i0 = ti * kGemmTile, j0 = tj * kGemmTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
-    const bool skip = lower_only && (ti == tj) && (wj > wi);
+    const bool skip = (ti == tj) && ((tri == 1 && wj > wi) || (tri == 2 && wi > wj));
 
     // loader: thread -> index column c = tid >> 1, l half (tid & 1) * 8 .. +8
     const int lc = tid >> 1, lh = (tid & 1) * 8;
@@ -112,7 +116,7 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int j = j0 + wj + 16 * mb + (lane >> 4) + 4 * v;  // f64 MFMA: row = lane/16 + 4*v
-                if (i < m && j < n && (!lower_only || j <= i)) {
+                if (i < m && j < n && (tri == 0 || (tri == 1 ? j <= i : i <= j))) {
                     double* p = C + (i64)i + (i64)j * ldc;
                     const double upd = alpha * acc[mb][nb][v];
                     *p = (beta == 0.0) ? upd : beta * (*p) + upd;
@@ -122,12 +126,12 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k
 }
 
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
-                              double beta, double* C, i64 ldc, bool lower_only, hipStream_t s) {
+                              double beta, double* C, i64 ldc, int tri, hipStream_t s) {
     if (m <= 0 || n <= 0) return hipSuccess;
-    if (lower_only) {
+    if (tri != 0) {  // square C
         const i64 T = ceil_div(m, kGemmTile);
         hipLaunchKernelGGL(gemm_tn_f64_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, m, n, k, alpha, A,
-                           lda, B, ldb, beta, C, ldc, 1);
+                           lda, B, ldb, beta, C, ldc, tri);
     } else {
         hipLaunchKernelGGL(gemm_tn_f64_kernel, dim3((unsigned)ceil_div(m, kGemmTile), (unsigned)ceil_div(n, kGemmTile)),
                            dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);

@@ -74,8 +74,9 @@ hipError_t launch_syrk_diag_bf16_planes(int np, int ncol, int k, const unsigned 
                                         float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
 
 // gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
+// tri: 0 all of C; 1 its lower triangle; 2 its upper triangle (1, 2: m == n)
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
-                              double beta, double* C, i64 ldc, bool lower_only, hipStream_t s);
+                              double beta, double* C, i64 ldc, int tri, hipStream_t s);
 
 // solve.hip -- persistent triangular solves with precomputed inverses of the
 // 64x64 diagonal blocks.  Dinv holds solve_dinv_count(n) elements, ctrl
@@ -84,9 +85,13 @@ hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A
 size_t solve_ctrl_count(int n, int kc);
 // solve_chain.hip -- chain + helpers triangular solve (forward, or backward with L^T)
 size_t chain_ctrl_count(int n, int kc);
+// trans = false: the forward solve B := inv(L) B; trans = true: the backward
+// solve B := inv(L^T) B (blocks in reverse order).  upper: the factor is
+// stored in the upper triangle as U = L^T, so the forward solve is with U^T
+// and the backward one with U.  unit: unit diagonal (never read).
 template <typename T>
 hipError_t launch_trsm_chain(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv, int* ctrl,
-                             T* part, const int* info, hipStream_t s);
+                             T* part, const int* info, hipStream_t s, bool upper = false, bool unit = false);
 size_t solve_dinv_count(int n);
 template <typename T>
 hipError_t launch_trtri_diag64(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s);
@@ -130,6 +135,11 @@ hipError_t sweep_read_error(int n, void* work, int* out, hipStream_t s);
 hipError_t launch_fill_int(int* p, int value, int count, hipStream_t s);
 hipError_t launch_fill_normal_f64(double* out, i64 rows, i64 cols, i64 ld, uint64_t seed, i64 offset, hipStream_t s);
 hipError_t launch_scale_f64(int m, int n, double alpha, double* A, i64 lda, hipStream_t s);
+// A := A^T in place (n x n); B (n x m) := A^T (A m x n).
+hipError_t launch_transpose_inplace_f64(int n, double* A, i64 lda, hipStream_t s);
+template <typename T>
+hipError_t launch_transpose_inplace(int n, T* A, i64 lda, hipStream_t s);
+hipError_t launch_transpose_f64(int m, int n, const double* A, i64 lda, double* B, i64 ldb, hipStream_t s);
 hipError_t launch_copy_lower_f64_to_f32(int n, const double* A, i64 lda, float* S, i64 lds, hipStream_t s);
 // columns j0 <= j < j1 only
 hipError_t launch_copy_lower_f64_to_f32_cols(int n, int j0, int j1, const double* A, i64 lda, float* S, i64 lds,

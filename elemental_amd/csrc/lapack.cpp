@@ -59,9 +59,27 @@ static int update_depth(int trailing) {
     return sizeof(T) == 8 ? kUpdateDepthF64 : (trailing == kTrailingF32 ? kUpdateDepthF32 : kUpdateDepthBf16x3);
 }
 
+// bf16 planes of the float factor's panels (two alternating sets) and the
+// inverses of the diagonal block's 32 x 32 pieces
+static int plane_count(int nb, int trailing) {
+    return nb % 32 == 0 ? (trailing == kTrailingBf16x3 ? 2 : trailing == kTrailingBf16x6 ? 3 : 0) : 0;
+}
+static size_t plane_region_bytes(int n, int nb, int trailing) {
+    const int np = plane_count(nb, trailing);
+    if (np == 0 || n <= 0) return 0;
+    const int G = std::max(1, update_depth<float>(trailing) / nb);
+    const i64 gw = std::min<i64>((i64)G * nb, n);
+    const i64 ldk = (gw + 31) / 32 * 32;
+    return (2 * (size_t)np * (size_t)n * (size_t)ldk * sizeof(unsigned short) + 255) & ~(size_t)255;
+}
+size_t lowprec_scratch_bytes(int n, int nb, int trailing) {
+    if (nb < 1 || nb > 128) return 0;
+    return plane_region_bytes(n, nb, trailing) + 4 * 1024 * sizeof(float);
+}
+
 template <typename T>
 int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s, hipEvent_t trailing_ready,
-                   const std::function<int(int)>* after_group) {
+                   const std::function<int(int)>* after_group, void* scratch_in) {
     if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr) return kErrArg;
     ELEMAMD_HIP_TRY(launch_fill_int(dinfo, 0, 1, s));
     if (n == 0) return kOk;
@@ -78,19 +96,18 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     // group's are written.
     // (plane chunks are 32 columns: block columns must start on chunk
     // boundaries; split-BF16x3 has an in-kernel-split form for other nb)
-    const int np = sizeof(T) == 4 && nb % 32 == 0
-                       ? (trailing == kTrailingBf16x3 ? 2 : trailing == kTrailingBf16x6 ? 3 : 0)
-                       : 0;
+    const int np = sizeof(T) == 4 ? plane_count(nb, trailing) : 0;
     if (sizeof(T) == 4 && trailing == kTrailingBf16x6 && np == 0) return kErrArg;
     const bool planes = np > 0;
     const i64 ldk = ((i64)gw + 31) / 32 * 32;
     const size_t plane_elems = (size_t)n * ldk;
     // float storage: the planes, then the diagonal pieces' inverses (4 x 32 x 32)
-    const size_t plane_bytes = planes ? (2 * np * plane_elems * sizeof(unsigned short) + 255) & ~(size_t)255 : 0;
+    const size_t plane_bytes = planes ? plane_region_bytes(n, nb, trailing) : 0;
     unsigned short* pbuf = nullptr;
     float* Xd = nullptr;
     if (sizeof(T) == 4) {
-        char* scratch = (char*)stream_scratch(plane_bytes + 4 * 1024 * sizeof(float), s);
+        char* scratch = scratch_in != nullptr ? (char*)scratch_in
+                                              : (char*)stream_scratch(lowprec_scratch_bytes(n, nb, trailing), s);
         if (scratch == nullptr) return kErrHip;
         pbuf = (unsigned short*)scratch;
         Xd = (float*)(scratch + plane_bytes);
@@ -180,9 +197,9 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
 }
 
 template int cholesky_lower<double>(int, double*, i64, int, int, int*, hipStream_t, hipEvent_t,
-                                    const std::function<int(int)>*);
+                                    const std::function<int(int)>*, void*);
 template int cholesky_lower<float>(int, float*, i64, int, int, int*, hipStream_t, hipEvent_t,
-                                    const std::function<int(int)>*);
+                                   const std::function<int(int)>*, void*);
 
 // ----------------------------------------------------------- SolveAfter
 // cholesky::SolveAfter(LOWER, NORMAL) (SolveAfter.hpp:20-75): forward then
@@ -208,26 +225,43 @@ int solve_prepare(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
 
 template <typename T>
 int solve_apply(bool forward, bool backward, int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
-                int* ctrl, const int* dinfo, hipStream_t s) {
-    // ctrl: solve_ctrl_count(n, k) ints = flags, then the helpers' partial sums
+                int* ctrl, const int* dinfo, hipStream_t s, bool upper, bool unit) {
+    // ctrl: solve_ctrl_count(n, k) ints = flags, then the helpers' partial sums;
+    // ctrl[1] is the error word of the whole solve (both directions)
     T* part = (T*)((char*)ctrl + ((chain_ctrl_count(n, k) * sizeof(int) + 255) & ~(size_t)255));
-    if (forward) ELEMAMD_HIP_TRY(launch_trsm_chain<T>(false, n, k, L, ldl, B, ldb, Dinv, ctrl, part, dinfo, s));
-    if (backward) ELEMAMD_HIP_TRY(launch_trsm_chain<T>(true, n, k, L, ldl, B, ldb, Dinv, ctrl, part, dinfo, s));
+    if (n > 0 && k > 0) ELEMAMD_HIP_TRY(launch_fill_int(ctrl + 1, 0, 1, s));
+    if (forward)
+        ELEMAMD_HIP_TRY(launch_trsm_chain<T>(false, n, k, L, ldl, B, ldb, Dinv, ctrl, part, dinfo, s, upper, unit));
+    if (backward)
+        ELEMAMD_HIP_TRY(launch_trsm_chain<T>(true, n, k, L, ldl, B, ldb, Dinv, ctrl, part, dinfo, s, upper, unit));
     return kOk;
 }
 
+int solve_read_error(const int* ctrl, hipStream_t s) {
+    int lost = 0;
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(&lost, ctrl + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
+    return lost != 0 ? kErrHip : kOk;
+}
+
 template <typename T>
-int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s) {
+int* solve_ctrl(int n, void* work) {
+    return (int*)((char*)work + align256(solve_dinv_count(n) * sizeof(T)));
+}
+
+template <typename T>
+int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s,
+                      bool upper) {
     if (n < 0 || k < 0 || ldl < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
     if (n == 0 || k == 0) return kOk;
     if (work == nullptr) return kErrWorkspace;
     T* Dinv = kSolveUsesInverse<T> ? (T*)work : nullptr;
-    int* ctrl = (int*)((char*)work + align256(solve_dinv_count(n) * sizeof(T)));
+    int* ctrl = solve_ctrl<T>(n, work);
     if (kSolveUsesInverse<T>) {
         int rc = solve_prepare<T>(n, L, ldl, Dinv, s);
         if (rc) return rc;
     }
-    return solve_apply<T>(true, true, n, k, L, ldl, B, ldb, Dinv, ctrl, dinfo, s);
+    return solve_apply<T>(true, true, n, k, L, ldl, B, ldb, Dinv, ctrl, dinfo, s, upper, false);
 }
 
 template size_t solve_work_bytes<double>(int, int);
@@ -235,11 +269,14 @@ template size_t solve_work_bytes<float>(int, int);
 template int solve_prepare<double>(int, const double*, i64, double*, hipStream_t);
 template int solve_prepare<float>(int, const float*, i64, float*, hipStream_t);
 template int solve_apply<double>(bool, bool, int, int, const double*, i64, double*, i64, const double*, int*,
-                                 const int*, hipStream_t);
+                                 const int*, hipStream_t, bool, bool);
 template int solve_apply<float>(bool, bool, int, int, const float*, i64, float*, i64, const float*, int*, const int*,
-                                hipStream_t);
-template int solve_after_lower<double>(int, int, const double*, i64, double*, i64, const int*, void*, hipStream_t);
-template int solve_after_lower<float>(int, int, const float*, i64, float*, i64, const int*, void*, hipStream_t);
+                                hipStream_t, bool, bool);
+template int* solve_ctrl<double>(int, void*);
+template int* solve_ctrl<float>(int, void*);
+template int solve_after_lower<double>(int, int, const double*, i64, double*, i64, const int*, void*, hipStream_t,
+                                       bool);
+template int solve_after_lower<float>(int, int, const float*, i64, float*, i64, const int*, void*, hipStream_t, bool);
 
 // ------------------------------------------------------ mixed precision
 namespace {
@@ -661,14 +698,15 @@ size_t mixed_refine_bytes(int n, int k, const MixedOpts& o) {
 }
 
 size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o) {
-    return mixed_factor_bytes(n) + mixed_refine_bytes(n, k, o);
+    return align256(mixed_factor_bytes(n)) + align256(lowprec_scratch_bytes(n, o.nb, o.trailing)) +
+           mixed_refine_bytes(n, k, o);
 }
 
 // Low-precision factor of A: S = float(A) (lower), FP64 diagonal potrf,
 // float panels and trailing updates; then the operands of the block-inverse
 // correction sweeps (diagonal-block inverses, look-ahead products, the
 // mirror of L in the upper triangle of S); ||A||_inf.  On breakdown the factor of A + sigma*diag(A) is retried.
-int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFactor& f, hipStream_t s) {
+int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFactor& f, hipStream_t s, void* scratch) {
     if (n < 0 || lda < std::max(1, n) || o.nb < 1 || o.nb > 128) return kErrArg;
     f.lowprec_info = 0;
     f.shifted = 0;
@@ -719,6 +757,12 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     };
     for (int attempt = 0; attempt < 4; ++attempt) {
         hipEvent_t ready = nullptr;
+        if (attempt > 0) {
+            // the failed attempt's mirror (aux) and preparation (aux3) still
+            // read S: the next attempt's copy into S waits for both
+            ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
+            if (int rc = join_prep()) return rc;
+        }
         prepared = 0;
         if (sigma == 0.0 && gw < n && o.nb > 0) {
             ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32_cols(n, 0, gw, A, lda, f.S, n, s));
@@ -731,7 +775,7 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
             ELEMAMD_HIP_TRY(launch_copy_lower_f64_to_f32(n, A, lda, f.S, n, s));
             if (sigma > 0.0) ELEMAMD_HIP_TRY(launch_shift_diag(n, A, lda, f.S, n, sigma, s));
         }
-        int rc = cholesky_lower<float>(n, f.S, n, o.nb, o.trailing, f.dinfo, s, ready, &prep);
+        int rc = cholesky_lower<float>(n, f.S, n, o.nb, o.trailing, f.dinfo, s, ready, &prep, scratch);
         if (rc == kOk && prepared < nT) rc = prep(n);
         if (rc == kOk) {
             // the mirror (upper triangle) on aux before the host reads the
@@ -774,10 +818,12 @@ int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, 
         }
         return kOk;
     }
-    const size_t fb = mixed_factor_bytes(n);
+    // the caller's workspace: the factor, the factorization's scratch, the refinement's vectors
+    const size_t fb = align256(mixed_factor_bytes(n));
+    const size_t sb = align256(lowprec_scratch_bytes(n, o.nb, o.trailing));
     if (work == nullptr || work_bytes < mixed_workspace_bytes(n, k, o)) return kErrWorkspace;
     MixedFactor f = carve_factor(work, n);
-    int rc = mixed_factor(n, A, lda, o, f, s);
+    int rc = mixed_factor(n, A, lda, o, f, s, (char*)work + fb);
     if (rc == kErrNotConverged) {
         if (rep) {
             *rep = MixedReport{};
@@ -788,7 +834,7 @@ int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, 
         return rc;
     }
     if (rc) return rc;
-    return mixed_refine(n, k, A, lda, B, ldb, o, f, (char*)work + fb, work_bytes - fb, rep, s);
+    return mixed_refine(n, k, A, lda, B, ldb, o, f, (char*)work + fb + sb, work_bytes - fb - sb, rep, s);
 }
 
 // FP64-residual refinement (classic, then GMRES-IR on stagnation) of A X = B

@@ -29,23 +29,38 @@ constexpr int kUpdateDepthBf16x3 = 512;
 // after_group (may be null): called on the host once the launches of each
 // group of block columns are enqueued, with the group's end column c1; work
 // ordered after that point of s sees columns [0, c1) of the factor final.
+// scratch (may be null): lowprec_scratch_bytes(n, nb, trailing) bytes of
+// device memory for the float factor's bf16 planes and diagonal-piece
+// inverses; null takes them from the per-stream scratch buffer.
 template <typename T>
 int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipStream_t s,
-                   hipEvent_t trailing_ready = nullptr, const std::function<int(int)>* after_group = nullptr);
+                   hipEvent_t trailing_ready = nullptr, const std::function<int(int)>* after_group = nullptr,
+                   void* scratch = nullptr);
+size_t lowprec_scratch_bytes(int n, int nb, int trailing);
 
 // cholesky::SolveAfter(LOWER, NORMAL): B := inv(L^T) inv(L) B.
 // `work` holds solve_work_bytes<T>(n, k) bytes of device memory.
 template <typename T>
 size_t solve_work_bytes(int n, int k);
+// upper: the factor is stored in the upper triangle as U = L^T
+// (cholesky::SolveAfter(UPPER, ...): the forward solve with U^T, the backward
+// one with U).
 template <typename T>
-int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s);
+int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s,
+                      bool upper = false);
+// The control words of solve_after_lower inside its `work` buffer.
+template <typename T>
+int* solve_ctrl(int n, void* work);
+// Synchronize `s` and report a persistent solve whose bounded wait gave up
+// (kErrHip; the solution is then wrong) since solve_apply cleared ctrl[1].
+int solve_read_error(const int* ctrl, hipStream_t s);
 // The two halves of it for repeated solves with one factor: the diagonal
 // block inverses are computed once (solve_prepare) and reused (solve_apply).
 template <typename T>
 int solve_prepare(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s);
 template <typename T>
 int solve_apply(bool forward, bool backward, int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv,
-                int* ctrl, const int* dinfo, hipStream_t s);
+                int* ctrl, const int* dinfo, hipStream_t s, bool upper = false, bool unit = false);
 
 struct MixedOpts {
     int nb = 128;
@@ -85,7 +100,10 @@ struct MixedFactor {
 };
 size_t mixed_factor_bytes(int n);
 MixedFactor carve_factor(void* mem, int n);
-int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFactor& f, hipStream_t s);
+// scratch: lowprec_scratch_bytes(n, o.nb, o.trailing) bytes (null: the
+// per-stream scratch buffer).
+int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFactor& f, hipStream_t s,
+                 void* scratch = nullptr);
 size_t mixed_refine_bytes(int n, int k, const MixedOpts& o);
 int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o,
                  const MixedFactor& f, void* work, size_t work_bytes, MixedReport* rep, hipStream_t s);

@@ -1,5 +1,6 @@
 #include "runtime.hpp"
 
+#include <cstdlib>
 #include <initializer_list>
 #include <map>
 #include <utility>
@@ -49,6 +50,15 @@ StreamPair* stream_pair(hipStream_t stream) {
         }
     }
     return &p;
+}
+
+long solve_spin_limit() {
+    const char* v = std::getenv("ELEMAMD_SPIN_LIMIT");
+    if (v != nullptr && v[0] != 0) {
+        const long l = std::strtol(v, nullptr, 10);
+        if (l > 0) return l;
+    }
+    return 1L << 26;
 }
 
 void* stream_scratch(size_t bytes, hipStream_t stream) {

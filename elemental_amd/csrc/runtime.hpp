@@ -24,4 +24,9 @@ struct StreamPair {
 // Owned by the calling thread; nullptr members on failure.
 StreamPair* stream_pair(hipStream_t stream);
 
+// Polls a persistent triangular solve makes before it gives up a wait and
+// records an error: 2^26, or ELEMAMD_SPIN_LIMIT from the environment (read at
+// every launch; tests set a tiny limit to exercise the error path).
+long solve_spin_limit();
+
 }  // namespace elemamd

@@ -61,6 +61,7 @@ struct Server::Factor {
 struct Server::Req {
     ServeRequest q;
     ServeResult res{};
+    size_t wbytes = 0;           // transient workspace reserved at admission
     hipEvent_t ready = nullptr;  // recorded on the submitter's stream
     bool done = false;
 };
@@ -107,18 +108,19 @@ size_t Server::factor_bytes(int n) const {
     return cfg_.precision == 0 ? al256(sizeof(double) * (size_t)n * n) + 256 : mixed_factor_bytes(n);
 }
 
-size_t Server::work_bytes(int n, int k) const {
+size_t Server::work_bytes(int n, int k, bool factor) const {
     if (cfg_.precision == 0) return al256(solve_work_bytes<double>(n, k)) + 256;
     MixedOpts o;
     o.nb = cfg_.nb;
     o.trailing = cfg_.precision == 2 ? kTrailingBf16x3 : kTrailingF32;
     o.tol = cfg_.tol;
-    return mixed_refine_bytes(n, k, o);
+    // a request that factors also needs the factorization's plane scratch
+    return al256(mixed_refine_bytes(n, k, o)) + (factor ? al256(lowprec_scratch_bytes(n, o.nb, o.trailing)) : 0);
 }
 
 int Server::submit(const ServeRequest& q, int64_t* ticket) {
     if (q.n < 1 || q.k < 1 || q.lda < q.n || q.ldb < q.n || q.A == nullptr || q.B == nullptr) return kErrArg;
-    if (factor_bytes(q.n) + work_bytes(q.n, q.k) > budget_) return kErrWorkspace;  // could never be admitted
+    if (factor_bytes(q.n) + work_bytes(q.n, q.k, true) > budget_) return kErrWorkspace;  // could never be admitted
     auto r = std::make_shared<Req>();
     r->q = q;
     r->res.t_submit = now_s(t0_);
@@ -187,7 +189,7 @@ std::shared_ptr<Server::Req> Server::admit(std::shared_ptr<Factor>& fac, bool& b
         if (need_factor && ci != cache_.end() && (ci->second->pins > 0 || !ci->second->ready))
             continue;  // the key's current factor is in use: do not replace it now
         const size_t fbytes = need_factor ? factor_bytes(q.n) : 0;
-        const size_t wbytes = work_bytes(q.n, q.k);
+        const size_t wbytes = work_bytes(q.n, q.k, need_factor);
         size_t freed = (need_factor && ci != cache_.end()) ? ci->second->bytes : 0;  // replaced entry
         size_t need = fbytes + wbytes;
         if (used_ - freed + need > budget_) {
@@ -231,6 +233,7 @@ std::shared_ptr<Server::Req> Server::admit(std::shared_ptr<Factor>& fac, bool& b
         fac->pins++;
         fac->last_use = ++use_clock_;
         used_ += fbytes + wbytes;
+        r->wbytes = wbytes;
         stats_.peak_used = std::max(stats_.peak_used, used_);
         queue_.erase(qi);
         return r;
@@ -246,8 +249,6 @@ void Server::worker(int slot) {
     o.nb = cfg_.nb;
     o.trailing = cfg_.precision == 2 ? kTrailingBf16x3 : kTrailingF32;
     o.tol = cfg_.tol;
-    void* work = nullptr;
-    size_t work_cap = 0;
     for (;;) {
         std::shared_ptr<Req> r;
         std::shared_ptr<Factor> F;
@@ -269,13 +270,12 @@ void Server::worker(int slot) {
         res.factored = build ? 1 : 0;
         int rc = s ? kOk : kErrHip;
         if (rc == kOk && r->ready != nullptr && hipStreamWaitEvent(s, r->ready, 0) != hipSuccess) rc = kErrHip;
-        const size_t wbytes = work_bytes(q.n, q.k);
-        if (rc == kOk && wbytes > work_cap) {  // grow-only per-slot workspace, stream ordered
-            if (work) (void)hipFreeAsync(work, s);
+        // exactly the reserved workspace, stream ordered, released when the request ends
+        const size_t wbytes = r->wbytes;
+        void* work = nullptr;
+        if (rc == kOk && hipMallocAsync(&work, wbytes, s) != hipSuccess) {
             work = nullptr;
-            work_cap = 0;
-            if (hipMallocAsync(&work, wbytes, s) == hipSuccess) work_cap = wbytes;
-            else rc = kErrHip;
+            rc = kErrHip;
         }
         if (rc == kOk && build && hipMallocAsync(&F->mem, F->bytes, s) != hipSuccess) {
             F->mem = nullptr;
@@ -298,19 +298,25 @@ void Server::worker(int slot) {
                 if (rc == kOk && info != 0) rc = kServeNotHpd;
             }
             if (rc == kOk) rc = solve_after_lower<double>(q.n, q.k, L, q.n, q.B, q.ldb, nullptr, work, s);
+            // synchronizes; a solve whose bounded wait gave up is an error
+            if (rc == kOk && q.n > 0 && q.k > 0) rc = solve_read_error(solve_ctrl<double>(q.n, work), s);
             if (rc == kOk && hipStreamSynchronize(s) != hipSuccess) rc = kErrHip;
         } else {
             if (rc == kOk && build) {
                 F->mf = carve_factor(F->mem, q.n);
-                rc = mixed_factor(q.n, q.A, q.lda, o, F->mf, s);
+                // the factorization's plane scratch follows the refinement's vectors
+                rc = mixed_factor(q.n, q.A, q.lda, o, F->mf, s, (char*)work + al256(mixed_refine_bytes(q.n, q.k, o)));
                 res.info = F->mf.lowprec_info;
             }
             MixedReport rep{};
-            if (rc == kOk) rc = mixed_refine(q.n, q.k, q.A, q.lda, q.B, q.ldb, o, F->mf, work, wbytes, &rep, s);
+            if (rc == kOk)
+                rc = mixed_refine(q.n, q.k, q.A, q.lda, q.B, q.ldb, o, F->mf, work, mixed_refine_bytes(q.n, q.k, o),
+                                  &rep, s);
             res.iterations = rep.iterations;
             res.gmres_iterations = rep.gmres_iterations;
             res.backward_error = rep.backward_error;
         }
+        if (work) (void)hipFreeAsync(work, s);
         (void)hipStreamSynchronize(s);
         if (r->ready != nullptr) (void)hipEventDestroy(r->ready);
         res.status = rc;
@@ -338,7 +344,6 @@ void Server::worker(int slot) {
         done_cv_.notify_all();
         cv_.notify_all();  // memory and a slot were released
     }
-    if (work) (void)hipFreeAsync(work, s);
     if (s) {
         (void)hipStreamSynchronize(s);
         (void)hipStreamDestroy(s);

@@ -70,7 +70,7 @@ private:
     struct Factor;
     struct Req;
     size_t factor_bytes(int n) const;
-    size_t work_bytes(int n, int k) const;
+    size_t work_bytes(int n, int k, bool factor) const;
     std::shared_ptr<Req> admit(std::shared_ptr<Factor>& fac, bool& build, hipStream_t s);
     void worker(int slot);
 

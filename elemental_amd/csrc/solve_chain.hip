@@ -21,7 +21,9 @@
 // acknowledgement of its stores before it raises a flag (solve_dev.hpp).
 // Workgroups take tickets in launch order: the chains first, then helpers by
 // increasing position, so every wait is on a workgroup that is running or
-// done.  Every wait is bounded (kSpinLimit polls, then an error in ctrl[1]).
+// done.  Every wait is bounded (spin_limit polls, then an error in ctrl[1];
+// ctrl[1] is cleared once per solve by the caller, not per launch, so a wait
+// that gave up in the forward solve is still reported after the backward one).
 //
 // Diagonal blocks: USE_INV multiplies by the precomputed inverse (float
 // solves of iterative refinement), otherwise substitution with reciprocal
@@ -29,6 +31,7 @@
 // diagonals).
 #include "common.hpp"
 #include "kernels.hpp"
+#include "runtime.hpp"
 #include "solve_dev.hpp"
 
 namespace elemamd {
@@ -39,11 +42,17 @@ struct ChainDepth {
     static constexpr int value = sizeof(T) == 8 ? 3 : 4;
 };
 
-template <typename T, bool TRANS, bool USE_INV>
+// TRANS: the backward direction (blocks in reverse order).  UPPER: the
+// factor is stored in the upper triangle (L = U^T, element (i, j) of L at
+// (j, i)), so the tiles are read transposed: READ_T = TRANS xor UPPER.
+// unit: the diagonal is taken as one and never used (Trsm's UNIT diag).
+template <typename T, bool TRANS, bool UPPER, bool USE_INV>
 __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T* __restrict__ L, i64 ldl,
                                                          T* __restrict__ B, i64 ldb, const T* __restrict__ Dinv,
                                                          int* __restrict__ ctrl, T* __restrict__ part,
-                                                         const int* __restrict__ info) {
+                                                         const int* __restrict__ info, long spin_limit, int unit) {
+    static_assert(!(UPPER && USE_INV), "upper-stored factors take the substitution path");
+    constexpr bool READ_T = TRANS != UPPER;
     constexpr int D = ChainDepth<T>::value;
     typedef typename Acc4<T>::type acc_t;
     __shared__ T sM[D][kSB][kSLd];      // [q][p] tiles: chain band (then diagonal in sM[0]); helpers: double buffer
@@ -66,22 +75,23 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
     T gm[16];
     auto gload = [&](int I, int J) {
         const i64 r0 = (i64)I * kSB, q0 = (i64)J * kSB;
-        if (!TRANS) {  // M[q][p] = L(r0+p, q0+q), lanes along p
+        // both indices are checked: either block may be the partial last one
+        if (!READ_T) {  // M[q][p] = L(r0+p, q0+q), lanes along p
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
-                const i64 p = r0 + lane;
-                gm[s] = (p < n) ? L[p + (q0 + w + 4 * s) * ldl] : T(0);
+                const i64 p = r0 + lane, q = q0 + w + 4 * s;
+                gm[s] = (p < n && q < n) ? L[p + q * ldl] : T(0);
             }
         } else {  // M[q][p] = L(q0+q, r0+p), lanes along q
 #pragma unroll
             for (int s = 0; s < 16; ++s) {
-                const i64 q = q0 + lane;
-                gm[s] = (q < n) ? L[q + (r0 + w + 4 * s) * ldl] : T(0);
+                const i64 q = q0 + lane, p = r0 + w + 4 * s;
+                gm[s] = (q < n && p < n) ? L[q + p * ldl] : T(0);
             }
         }
     };
     auto sstore = [&](int buf) {
-        if (!TRANS) {
+        if (!READ_T) {
 #pragma unroll
             for (int s = 0; s < 16; ++s) sM[buf][w + 4 * s][lane] = gm[s];
         } else {
@@ -94,7 +104,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
             long spins = 0;
             while (coherent_load(f) == 0) {
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
+                if (++spins > spin_limit) {
                     atomicOr(&ctrl[1], 1);
                     break;
                 }
@@ -133,7 +143,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
                     const unsigned long long msk = __ballot(f != 0);
                     cnt = (~msk == 0ull) ? 64 : (int)__builtin_ctzll(~msk);
                     if (cnt > 0) break;
-                    if (++spins > kSpinLimit) {
+                    if (++spins > spin_limit) {
                         if (lane == 0) atomicOr(&ctrl[1], 1);
                         cnt = nJ - done;
                         break;
@@ -223,8 +233,12 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
 #pragma unroll
             for (int s2 = 0; s2 < 16; ++s2) {
                 const int kk = w + 4 * s2;
-                const int gi = TRANS ? (kSB - 1 - kk) : lane;
-                const int gk = TRANS ? (kSB - 1 - lane) : kk;
+                // stored (row, column) of M(lane, kk) within the block: an
+                // upper-stored factor swaps them
+                const int li = TRANS ? (kSB - 1 - kk) : lane;
+                const int lk = TRANS ? (kSB - 1 - lane) : kk;
+                const int gi = UPPER ? lk : li;
+                const int gk = UPPER ? li : lk;
                 T v = T(0);
                 if (gi < bs && gk < bs) {
                     if (lane >= kk) v = L[r0 + gi + (r0 + gk) * ldl];
@@ -236,7 +250,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
         }
     };
     auto stage = [&](int slot, const T (&g)[16]) {  // a band tile into sM[slot] ([q][p] layout)
-        if (!TRANS) {
+        if (!READ_T) {
 #pragma unroll
             for (int s2 = 0; s2 < 16; ++s2) sM[slot][w + 4 * s2][lane] = g[s2];
         } else {
@@ -312,7 +326,7 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
                 if (row < n && c < kcl) coherent_store(&B[row + (i64)(c_base + c) * ldb], y[v]);
             }
         } else {
-            if (t < kSB) sRinv[t] = T(1) / sM[0][t][t];
+            if (t < kSB) sRinv[t] = unit ? T(1) : T(1) / sM[0][t][t];
             __syncthreads();
             const int myrow = TRANS ? (kSB - 1 - lane) : lane;
             T v[4];
@@ -345,37 +359,45 @@ __global__ __launch_bounds__(256) void trsm_chain_kernel(int n, int kc, const T*
 
 size_t chain_ctrl_count(int n, int kc) { return 2 + 2 * (size_t)ceil_div(n, kSB) * ceil_div(kc, kSC); }
 
+template <typename T, bool TRANS, bool UPPER, bool INV>
+static void chain_launch(unsigned grid, hipStream_t s, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb,
+                         const T* Dinv, int* ctrl, T* part, const int* info, long spin, int unit) {
+    hipLaunchKernelGGL((trsm_chain_kernel<T, TRANS, UPPER, INV>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B, ldb,
+                       Dinv, ctrl, part, info, spin, unit);
+}
+
 template <typename T>
 hipError_t launch_trsm_chain(bool trans, int n, int kc, const T* L, i64 ldl, T* B, i64 ldb, const T* Dinv, int* ctrl,
-                             T* part, const int* info, hipStream_t s) {
+                             T* part, const int* info, hipStream_t s, bool upper, bool unit) {
     if (n <= 0 || kc <= 0) return hipSuccess;
     constexpr int D = ChainDepth<T>::value;
     const int nT = (int)ceil_div(n, kSB), CY = (int)ceil_div(kc, kSC);
-    hipError_t e = launch_fill_int(ctrl, 0, (int)chain_ctrl_count(n, kc), s);
-    if (e != hipSuccess) return e;
-    const unsigned grid = (unsigned)(CY + CY * std::max(0, nT - D - 1));
     const bool inv = Dinv != nullptr;
-    if (trans) {
-        if (inv)
-            hipLaunchKernelGGL((trsm_chain_kernel<T, true, true>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B, ldb,
-                               Dinv, ctrl, part, info);
-        else
-            hipLaunchKernelGGL((trsm_chain_kernel<T, true, false>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B,
-                               ldb, Dinv, ctrl, part, info);
+    if (inv && (upper || unit)) return hipErrorInvalidValue;
+    // ctrl[0] (tickets) and the flags from ctrl[2] on restart at zero; the
+    // error word ctrl[1] keeps what earlier launches of the solve recorded
+    hipError_t e = launch_fill_int(ctrl, 0, 1, s);
+    if (e == hipSuccess) e = launch_fill_int(ctrl + 2, 0, (int)chain_ctrl_count(n, kc) - 2, s);
+    if (e != hipSuccess) return e;
+    const long spin = solve_spin_limit();
+    const unsigned grid = (unsigned)(CY + CY * std::max(0, nT - D - 1));
+    const int u = unit ? 1 : 0;
+    if (inv) {
+        if (trans) chain_launch<T, true, false, true>(grid, s, n, kc, L, ldl, B, ldb, Dinv, ctrl, part, info, spin, u);
+        else chain_launch<T, false, false, true>(grid, s, n, kc, L, ldl, B, ldb, Dinv, ctrl, part, info, spin, u);
+    } else if (!upper) {
+        if (trans) chain_launch<T, true, false, false>(grid, s, n, kc, L, ldl, B, ldb, Dinv, ctrl, part, info, spin, u);
+        else chain_launch<T, false, false, false>(grid, s, n, kc, L, ldl, B, ldb, Dinv, ctrl, part, info, spin, u);
     } else {
-        if (inv)
-            hipLaunchKernelGGL((trsm_chain_kernel<T, false, true>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B,
-                               ldb, Dinv, ctrl, part, info);
-        else
-            hipLaunchKernelGGL((trsm_chain_kernel<T, false, false>), dim3(grid), dim3(256), 0, s, n, kc, L, ldl, B,
-                               ldb, Dinv, ctrl, part, info);
+        if (trans) chain_launch<T, true, true, false>(grid, s, n, kc, L, ldl, B, ldb, Dinv, ctrl, part, info, spin, u);
+        else chain_launch<T, false, true, false>(grid, s, n, kc, L, ldl, B, ldb, Dinv, ctrl, part, info, spin, u);
     }
     return hipGetLastError();
 }
 
 template hipError_t launch_trsm_chain<double>(bool, int, int, const double*, i64, double*, i64, const double*, int*,
-                                              double*, const int*, hipStream_t);
+                                              double*, const int*, hipStream_t, bool, bool);
 template hipError_t launch_trsm_chain<float>(bool, int, int, const float*, i64, float*, i64, const float*, int*,
-                                             float*, const int*, hipStream_t);
+                                             float*, const int*, hipStream_t, bool, bool);
 
 }  // namespace elemamd

@@ -64,20 +64,27 @@ int elemamd_info_read(const int *dinfo, void *stream);
 int elemamd_potrf_unb_f64(int n, double *A, int64_t lda, int *dinfo, void *stream);
 
 /* B := alpha * op(A)^{-1} B (side 'L') or alpha * B op(A)^{-1} (side 'R'),
- * A lower triangular, non-unit diagonal.  Replaces elem::Trsm(Matrix)
+ * A lower (uplo 'L') or upper ('U') triangular, non-unit ('N') or unit ('U')
+ * diagonal, op = N or T ('C' == 'T').  Replaces elem::Trsm(Matrix)
  * (include/elemental/blas-like/level3/Trsm.hpp:49-86) -> blas::Trsm ->
- * dtrsm_ (src/core/imports/blas.cpp:1203-1210).
- * Supported: side 'R', trans 'T'/'C' with n <= 128 (the Cholesky panel
- * solve, bit-identical to the column-oriented reference BLAS loop); side 'L',
- * trans 'N'/'T'/'C', any size (the SolveAfter solves).  Other combinations
- * return ELEMAMD_ERR_ARG.  alpha != 1 pre-scales B. */
+ * dtrsm_ (src/core/imports/blas.cpp:1203-1210); every combination the
+ * reference accepts.  side 'R', lower, trans 'T', non-unit with n <= 128 is
+ * the Cholesky panel solve, bit-identical to the column-oriented reference
+ * BLAS loop; side 'L' runs the persistent chain solves; the other side 'R'
+ * forms solve B^T on the left through a transposed copy.  alpha != 1
+ * pre-scales B.  Every argument is checked before anything is enqueued (an
+ * ELEMAMD_ERR_ARG return leaves B untouched).  Synchronous when it runs a
+ * persistent solve (a bounded wait that gave up returns ELEMAMD_ERR_HIP). */
 int elemamd_trsm_f64(char side, char uplo, char trans, char diag, int m, int n, double alpha,
                      const double *A, int64_t lda, double *B, int64_t ldb, void *stream);
 
-/* Lower C := alpha*A*A^T + beta*C, A n x k (trans 'N').  Replaces
+/* C := alpha*op(A)*op(A)^T + beta*C on the uplo triangle of C ('L' or 'U'),
+ * op(A) = A (trans 'N', A n x k) or A^T (trans 'T'/'C', A k x n).  Replaces
  * elem::Herk / Syrk(Matrix) (include/elemental/blas-like/level3/Herk.hpp:17-27,
  * Syrk.hpp:20-57) -> blas::Herk -> dsyrk_ (src/core/imports/blas.cpp:1043-1050).
- * The strictly upper triangle of C is neither read nor written. */
+ * The other triangle of C is neither read nor written.  ('L', 'N') is the
+ * Cholesky trailing update's kernel; the other forms run the TN product on
+ * the requested triangle (trans 'N' through a transposed copy of A). */
 int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const double *A, int64_t lda,
                      double beta, double *C, int64_t ldc, void *stream);
 
@@ -92,43 +99,65 @@ int elemamd_herk_f64(char uplo, char trans, int n, int k, double alpha, const do
 int elemamd_herk_f32(int mode, int n, int k, float alpha, const float *A, int64_t lda, float beta, float *C,
                      int64_t ldc, void *stream);
 
-/* Lower trapezoid of C := alpha * op(A) * op(B) + beta*C with A and B k x n
- * (orientA 'T'/'C', orientB 'N').  Replaces elem::Trrk(Matrix) /
- * LocalTrrk's TN path (src/blas-like/Trrk.cpp:19-38,
- * src/blas-like/Trrk/Local.hpp:733-775 and :931-976). */
+/* The uplo triangle ('L' or 'U') of C := alpha * op(A) * op(B) + beta*C,
+ * op(A) n x k, op(B) k x n, orientations 'N' or 'T'/'C'.  Replaces
+ * elem::Trrk(Matrix) / LocalTrrk (src/blas-like/Trrk.cpp:19-38,
+ * src/blas-like/Trrk/Local.hpp:733-775 and :931-976); the TN form is the
+ * direct product, the other orientations go through transposed copies. */
 int elemamd_trrk_f64(char uplo, char orientA, char orientB, int n, int k, double alpha, const double *A,
                      int64_t lda, const double *B, int64_t ldb, double beta, double *C, int64_t ldc, void *stream);
 
-/* x := op(L)^{-1} x for one vector (incx must be 1).  Replaces elem::Trsv
+/* x := op(A)^{-1} x for one vector (incx must be 1), A lower or upper,
+ * non-unit or unit diagonal.  Replaces elem::Trsv
  * (include/elemental/blas-like/level2/Trsv.hpp:20-44) -> dtrsv_
- * (src/core/imports/blas.cpp:901-904). */
+ * (src/core/imports/blas.cpp:901-904).  Synchronous. */
 int elemamd_trsv_f64(char uplo, char trans, char diag, int n, const double *A, int64_t lda, double *x,
                      int64_t incx, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* The path.                                                          */
 
-/* Blocked right-looking lower Cholesky (variant 3) with block size nb
- * (1..128; Elemental's default Blocksize() is 128, src/core/global.cpp:194).
- * Replaces elem::Cholesky(LOWER, Matrix<F>&)
- * (include/elemental/lapack-like/Cholesky.hpp:49-62) -> cholesky::LVar3
- * (include/elemental/lapack-like/Cholesky/LVar3.hpp:83-123).  The lower
- * triangle of A becomes L; the strictly upper triangle is untouched.
- * Asynchronous: the outcome is written to *dinfo. */
+/* Blocked right-looking Cholesky (variant 3) with block size nb (1..128;
+ * Elemental's default Blocksize() is 128, src/core/global.cpp:194).
+ * Replaces elem::Cholesky(uplo, Matrix<F>&)
+ * (include/elemental/lapack-like/Cholesky.hpp:49-62).  uplo 'L' ->
+ * cholesky::LVar3 (include/elemental/lapack-like/Cholesky/LVar3.hpp:83-123):
+ * the lower triangle of A becomes L, the strictly upper triangle is
+ * untouched.  uplo 'U' -> cholesky::UVar3 (Cholesky/UVar3.hpp:85-125): the
+ * upper triangle becomes U with A = U^T U, the strictly lower triangle is
+ * untouched; it runs as the lower factorization of A^T (UVar3Unb on A is
+ * LVar3Unb on A^T operation by operation), with A transposed in place
+ * before and after.  Asynchronous: the outcome is written to *dinfo. */
 int elemamd_cholesky_f64(char uplo, int n, double *A, int64_t lda, int nb, int *dinfo, void *stream);
 
-/* B := A^{-1} B given the factor L in the lower triangle of A.  Replaces
+/* B := A^{-1} B given the factor in A's uplo triangle.  Replaces
  * cholesky::SolveAfter(uplo, orientation, A, B)
  * (include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75): two Trsv
- * for one column, two Trsm otherwise.  uplo 'L'; orientation 'N', 'T' or
- * 'C' (identical for real data). */
+ * for one column, two Trsm otherwise; LOWER solves with L then L^T, UPPER
+ * with U^T then U.  orientation 'N', 'T' or 'C' (identical for real data).
+ * Synchronous: returns once the solves are done, ELEMAMD_ERR_HIP if a
+ * persistent solve's bounded wait gave up (the result is then wrong). */
 int elemamd_solve_after_f64(char uplo, char orientation, int n, int k, const double *A, int64_t lda, double *B,
                             int64_t ldb, void *stream);
 
 /* Cholesky then SolveAfter.  Replaces elem::HPDSolve(uplo, orientation,
  * Matrix<F>& A, Matrix<F>& B) (include/elemental/lapack-like/HPDSolve.hpp:18-28).
- * Asynchronous; the solve is skipped when *dinfo becomes nonzero. */
+ * The solve is skipped when *dinfo becomes nonzero.  Synchronous, as
+ * SolveAfter. */
 int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double *A, int64_t lda, double *B,
+                          int64_t ldb, int nb, int *dinfo, void *stream);
+
+/* The same three entry points for Matrix<float> (the reference templates
+ * Cholesky<F>, cholesky::SolveAfter<F> and HPDSolve<F> on the field,
+ * Cholesky.hpp:47-62, SolveAfter.hpp:20-75, HPDSolve.hpp:18-28): the float
+ * factorization keeps each 128 x 128 diagonal block's factor in FP64 before
+ * storing it as float, solves the panels by products with the inverses of
+ * its 32 x 32 diagonal pieces and updates the trailing matrix with FP32
+ * MFMA; the float solves use the inverses of the 64 x 64 diagonal blocks. */
+int elemamd_cholesky_f32(char uplo, int n, float *A, int64_t lda, int nb, int *dinfo, void *stream);
+int elemamd_solve_after_f32(char uplo, char orientation, int n, int k, const float *A, int64_t lda, float *B,
+                            int64_t ldb, void *stream);
+int elemamd_hpd_solve_f32(char uplo, char orientation, int n, int k, float *A, int64_t lda, float *B,
                           int64_t ldb, int nb, int *dinfo, void *stream);
 
 /* ------------------------------------------------------------------ */

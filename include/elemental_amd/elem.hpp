@@ -1,10 +1,10 @@
-// elem.hpp -- C++ drop-in for Elemental's lower HPD Cholesky / HPDSolve path
+// elem.hpp -- C++ drop-in for Elemental's HPD Cholesky / HPDSolve path
 // on one MI355X.  Same call signatures and column-major Matrix<F> semantics as
 // the reference (names below cite aterrel/Elemental); the work runs on the GPU
 // through the C ABI in elemental_amd.h.
 //
 //   elem::Matrix<T>                  include/elemental/core/matrix.hpp:16-183
-//   elem::Cholesky(uplo, A)          include/elemental/lapack-like/Cholesky.hpp:49-62
+//   elem::Cholesky(uplo, A)          include/elemental/lapack-like/Cholesky.hpp:49-62 (LVar3 / UVar3)
 //   elem::LocalCholesky(uplo, A)     include/elemental/lapack-like/Cholesky.hpp:29-37
 //   elem::HPDSolve(uplo, orient, A, B)  include/elemental/lapack-like/HPDSolve.hpp:18-28
 //   elem::cholesky::SolveAfter(...)  include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75
@@ -12,6 +12,7 @@
 //   elem::Herk / elem::Syrk(...)     include/elemental/blas-like/level3/Herk.hpp:17-27, Syrk.hpp:20-57
 //   elem::Trrk(...)                  src/blas-like/Trrk.cpp:19-38
 //   elem::Trsv(...)                  include/elemental/blas-like/level2/Trsv.hpp:20-44
+//   elem::Print / elem::Write        include/elemental/io/Print.hpp:15-35, io/Write.hpp:17-28
 //
 // A host Matrix is staged to device memory for the call and copied back, so
 // existing callers keep their buffers.  Callers that keep data on the GPU use
@@ -21,6 +22,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <fstream>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -108,6 +111,29 @@ inline void View(Matrix<T>& B, Matrix<T>& A, int i, int j, int height, int width
     B.Attach(height, width, A.Buffer(i, j), A.LDim());
 }
 
+// Print(A, title, os) (io/Print.hpp:15-35): the title line, then each row
+// as its entries with the stream's current format, each followed by a
+// space, then an empty line.
+template <typename T>
+inline void Print(const Matrix<T>& A, std::string title = "", std::ostream& os = std::cout) {
+    if (title != "") os << title << std::endl;
+    for (int i = 0; i < A.Height(); ++i) {
+        for (int j = 0; j < A.Width(); ++j) os << A.Get(i, j) << " ";
+        os << std::endl;
+    }
+    os << std::endl;
+}
+
+// Write(A, title, filename) (io/Write.hpp:17-28): Print into the file
+// `filename` in scientific notation.
+template <typename T>
+inline void Write(const Matrix<T>& A, std::string title = "", std::string filename = "Matrix") {
+    std::ofstream file(filename.c_str());
+    file.setf(std::ios::scientific);
+    Print(A, title, file);
+    file.close();
+}
+
 namespace detail {
 
 inline void check_hip(hipError_t e, const char* what) {
@@ -120,20 +146,21 @@ inline char uplo_char(UpperOrLower u) { return u == LOWER ? 'L' : 'U'; }
 inline char orient_char(Orientation o) { return o == NORMAL ? 'N' : (o == TRANSPOSE ? 'T' : 'C'); }
 
 // Device copy of a host Matrix (packed, ld = max(height, 1)); copies back on demand.
+template <typename T>
 struct Dev {
-    double* p = nullptr;
+    T* p = nullptr;
     int h = 0, w = 0, ld = 1;
-    explicit Dev(const Matrix<double>& A) : h(A.Height()), w(A.Width()), ld(A.Height() > 1 ? A.Height() : 1) {
-        size_t bytes = sizeof(double) * (size_t)ld * (w > 0 ? w : 1);
+    explicit Dev(const Matrix<T>& A) : h(A.Height()), w(A.Width()), ld(A.Height() > 1 ? A.Height() : 1) {
+        size_t bytes = sizeof(T) * (size_t)ld * (w > 0 ? w : 1);
         check_hip(hipMalloc(&p, bytes), "hipMalloc");
         if (h > 0 && w > 0)
-            check_hip(hipMemcpy2D(p, sizeof(double) * ld, A.LockedBuffer(), sizeof(double) * A.LDim(),
-                                  sizeof(double) * h, w, hipMemcpyHostToDevice),
+            check_hip(hipMemcpy2D(p, sizeof(T) * ld, A.LockedBuffer(), sizeof(T) * A.LDim(), sizeof(T) * h, w,
+                                  hipMemcpyHostToDevice),
                       "hipMemcpy2D");
     }
-    void back(Matrix<double>& A) const {
+    void back(Matrix<T>& A) const {
         if (h > 0 && w > 0)
-            check_hip(hipMemcpy2D(A.Buffer(), sizeof(double) * A.LDim(), p, sizeof(double) * ld, sizeof(double) * h, w,
+            check_hip(hipMemcpy2D(A.Buffer(), sizeof(T) * A.LDim(), p, sizeof(T) * ld, sizeof(T) * h, w,
                                   hipMemcpyDeviceToHost),
                       "hipMemcpy2D");
     }
@@ -146,11 +173,15 @@ struct Dev {
 
 struct DevInfo {
     int* p = nullptr;
-    DevInfo() { check_hip(hipMalloc(&p, sizeof(int)), "hipMalloc"); }
+    DevInfo() {
+        check_hip(hipMalloc(&p, sizeof(int)), "hipMalloc");
+        check_hip(hipMemset(p, 0, sizeof(int)), "hipMemset");
+    }
     ~DevInfo() {
         if (p) (void)hipFree(p);
     }
-    // cholesky::LVar3Unb throws this at the failing column (LVar3.hpp:37-38)
+    // cholesky::LVar3Unb / UVar3Unb throw this at the failing column
+    // (LVar3.hpp:37-38, UVar3.hpp:38-39)
     void raise_if_failed() const {
         int info = elemamd_info_read(p, nullptr);
         if (info > 0) throw std::logic_error("A was not numerically HPD");
@@ -158,45 +189,71 @@ struct DevInfo {
     }
 };
 
+// the C ABI entry points by field
+inline int cholesky_abi(char u, int n, double* A, int64_t lda, int nb, int* info) {
+    return elemamd_cholesky_f64(u, n, A, lda, nb, info, nullptr);
+}
+inline int cholesky_abi(char u, int n, float* A, int64_t lda, int nb, int* info) {
+    return elemamd_cholesky_f32(u, n, A, lda, nb, info, nullptr);
+}
+inline int solve_after_abi(char u, char o, int n, int k, const double* A, int64_t lda, double* B, int64_t ldb) {
+    return elemamd_solve_after_f64(u, o, n, k, A, lda, B, ldb, nullptr);
+}
+inline int solve_after_abi(char u, char o, int n, int k, const float* A, int64_t lda, float* B, int64_t ldb) {
+    return elemamd_solve_after_f32(u, o, n, k, A, lda, B, ldb, nullptr);
+}
+inline int hpd_solve_abi(char u, char o, int n, int k, double* A, int64_t lda, double* B, int64_t ldb, int nb,
+                         int* info) {
+    return elemamd_hpd_solve_f64(u, o, n, k, A, lda, B, ldb, nb, info, nullptr);
+}
+inline int hpd_solve_abi(char u, char o, int n, int k, float* A, int64_t lda, float* B, int64_t ldb, int nb,
+                         int* info) {
+    return elemamd_hpd_solve_f32(u, o, n, k, A, lda, B, ldb, nb, info, nullptr);
+}
+
 }  // namespace detail
 
 inline int Blocksize() { return 128; }  // src/core/global.cpp:194
 
-inline void Cholesky(UpperOrLower uplo, Matrix<double>& A, int nb = Blocksize()) {
+// F = double or float (Cholesky.hpp:47-62 templates on the field).
+template <typename F>
+inline void Cholesky(UpperOrLower uplo, Matrix<F>& A, int nb = Blocksize()) {
     if (A.Height() != A.Width()) throw std::logic_error("A must be square");
-    if (uplo != LOWER) throw std::logic_error("only LOWER is supported on this path");
-    detail::Dev d(A);
+    detail::Dev<F> d(A);
     detail::DevInfo info;
-    detail::check(elemamd_cholesky_f64('L', A.Height(), d.p, d.ld, nb, info.p, nullptr), "elemamd_cholesky_f64");
+    detail::check(detail::cholesky_abi(detail::uplo_char(uplo), A.Height(), d.p, d.ld, nb, info.p),
+                  "elemamd_cholesky");
     d.back(A);
     info.raise_if_failed();
 }
 
-inline void LocalCholesky(UpperOrLower uplo, Matrix<double>& A) { Cholesky(uplo, A); }
+template <typename F>
+inline void LocalCholesky(UpperOrLower uplo, Matrix<F>& A) {
+    Cholesky(uplo, A);
+}
 
 namespace cholesky {
-inline void SolveAfter(UpperOrLower uplo, Orientation orientation, const Matrix<double>& A, Matrix<double>& B) {
+template <typename F>
+inline void SolveAfter(UpperOrLower uplo, Orientation orientation, const Matrix<F>& A, Matrix<F>& B) {
     if (A.Height() != A.Width()) throw std::logic_error("A must be square");
     if (A.Height() != B.Height()) throw std::logic_error("A and B must be the same height");
-    if (uplo != LOWER) throw std::logic_error("only LOWER is supported on this path");
-    detail::Dev dA(A), dB(B);
-    detail::check(elemamd_solve_after_f64('L', detail::orient_char(orientation), A.Height(), B.Width(), dA.p, dA.ld,
-                                          dB.p, dB.ld, nullptr),
-                  "elemamd_solve_after_f64");
+    detail::Dev<F> dA(A), dB(B);
+    detail::check(detail::solve_after_abi(detail::uplo_char(uplo), detail::orient_char(orientation), A.Height(),
+                                          B.Width(), dA.p, dA.ld, dB.p, dB.ld),
+                  "elemamd_solve_after");
     dB.back(B);
 }
 }  // namespace cholesky
 
-inline void HPDSolve(UpperOrLower uplo, Orientation orientation, Matrix<double>& A, Matrix<double>& B,
-                     int nb = Blocksize()) {
+template <typename F>
+inline void HPDSolve(UpperOrLower uplo, Orientation orientation, Matrix<F>& A, Matrix<F>& B, int nb = Blocksize()) {
     if (A.Height() != A.Width()) throw std::logic_error("A must be square");
     if (A.Height() != B.Height()) throw std::logic_error("A and B must be the same height");
-    if (uplo != LOWER) throw std::logic_error("only LOWER is supported on this path");
-    detail::Dev dA(A), dB(B);
+    detail::Dev<F> dA(A), dB(B);
     detail::DevInfo info;
-    detail::check(elemamd_hpd_solve_f64('L', detail::orient_char(orientation), A.Height(), B.Width(), dA.p, dA.ld,
-                                        dB.p, dB.ld, nb, info.p, nullptr),
-                  "elemamd_hpd_solve_f64");
+    detail::check(detail::hpd_solve_abi(detail::uplo_char(uplo), detail::orient_char(orientation), A.Height(),
+                                        B.Width(), dA.p, dA.ld, dB.p, dB.ld, nb, info.p),
+                  "elemamd_hpd_solve");
     dA.back(A);
     dB.back(B);
     info.raise_if_failed();
@@ -210,7 +267,7 @@ inline void Trsm(LeftOrRight side, UpperOrLower uplo, Orientation orientation, U
     if (checkIfSingular && diag != UNIT)
         for (int j = 0; j < A.Height(); ++j)
             if (A.Get(j, j) == 0.0) throw SingularMatrixException();
-    detail::Dev dA(A), dB(B);
+    detail::Dev<double> dA(A), dB(B);
     detail::check(elemamd_trsm_f64(side == LEFT ? 'L' : 'R', detail::uplo_char(uplo), detail::orient_char(orientation),
                                    diag == NON_UNIT ? 'N' : 'U', B.Height(), B.Width(), alpha, dA.p, dA.ld, dB.p, dB.ld,
                                    nullptr),
@@ -218,12 +275,16 @@ inline void Trsm(LeftOrRight side, UpperOrLower uplo, Orientation orientation, U
     dB.back(B);
 }
 
+// C := alpha op(A) op(A)^T + beta C on the uplo triangle; op(A) = A (NORMAL,
+// A n x k) or A^T (A k x n).
 inline void Herk(UpperOrLower uplo, Orientation orientation, double alpha, const Matrix<double>& A, double beta,
                  Matrix<double>& C) {
-    if (A.Height() != C.Height() || C.Height() != C.Width()) throw std::logic_error("Nonconformal Syrk");
-    detail::Dev dA(A), dC(C);
-    detail::check(elemamd_herk_f64(detail::uplo_char(uplo), detail::orient_char(orientation), C.Height(), A.Width(),
-                                   alpha, dA.p, dA.ld, beta, dC.p, dC.ld, nullptr),
+    const int n = orientation == NORMAL ? A.Height() : A.Width();
+    const int k = orientation == NORMAL ? A.Width() : A.Height();
+    if (n != C.Height() || C.Height() != C.Width()) throw std::logic_error("Nonconformal Syrk");
+    detail::Dev<double> dA(A), dC(C);
+    detail::check(elemamd_herk_f64(detail::uplo_char(uplo), detail::orient_char(orientation), n, k, alpha, dA.p,
+                                   dA.ld, beta, dC.p, dC.ld, nullptr),
                   "elemamd_herk_f64");
     dC.back(C);
 }
@@ -233,12 +294,16 @@ inline void Syrk(UpperOrLower uplo, Orientation orientation, double alpha, const
     Herk(uplo, orientation, alpha, A, beta, C);
 }
 
+// The uplo triangle of C := alpha op(A) op(B) + beta C, op(A) n x k, op(B) k x n.
 inline void Trrk(UpperOrLower uplo, Orientation orientationOfA, Orientation orientationOfB, double alpha,
                  const Matrix<double>& A, const Matrix<double>& B, double beta, Matrix<double>& C) {
-    detail::Dev dA(A), dB(B), dC(C);
+    const int n = C.Height();
+    const int k = orientationOfA == NORMAL ? A.Width() : A.Height();
+    if (C.Width() != n) throw std::logic_error("C must be square");
+    detail::Dev<double> dA(A), dB(B), dC(C);
     detail::check(elemamd_trrk_f64(detail::uplo_char(uplo), detail::orient_char(orientationOfA),
-                                   detail::orient_char(orientationOfB), C.Height(), A.Height(), alpha, dA.p, dA.ld,
-                                   dB.p, dB.ld, beta, dC.p, dC.ld, nullptr),
+                                   detail::orient_char(orientationOfB), n, k, alpha, dA.p, dA.ld, dB.p, dB.ld, beta,
+                                   dC.p, dC.ld, nullptr),
                   "elemamd_trrk_f64");
     dC.back(C);
 }
@@ -246,7 +311,7 @@ inline void Trrk(UpperOrLower uplo, Orientation orientationOfA, Orientation orie
 inline void Trsv(UpperOrLower uplo, Orientation orientation, UnitOrNonUnit diag, const Matrix<double>& A,
                  Matrix<double>& x) {
     if (x.Width() != 1) throw std::logic_error("x must be a column vector here");
-    detail::Dev dA(A), dx(x);
+    detail::Dev<double> dA(A), dx(x);
     detail::check(elemamd_trsv_f64(detail::uplo_char(uplo), detail::orient_char(orientation),
                                    diag == NON_UNIT ? 'N' : 'U', A.Height(), dA.p, dA.ld, dx.p, 1, nullptr),
                   "elemamd_trsv_f64");

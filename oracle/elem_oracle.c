@@ -11,6 +11,9 @@
  *                      block clamp            include/elemental/core/partition_impl.hpp:403-424
  *   eo_solve_after     cholesky::SolveAfter   include/elemental/lapack-like/Cholesky/SolveAfter.hpp:20-75
  *   eo_hpd_solve       HPDSolve               include/elemental/lapack-like/HPDSolve.hpp:18-28
+ *   eo_uvar3unb        cholesky::UVar3Unb     include/elemental/lapack-like/Cholesky/UVar3.hpp:21-49
+ *   eo_uvar3           cholesky::UVar3        include/elemental/lapack-like/Cholesky/UVar3.hpp:85-125
+ *   eo_solve_after_upper  SolveAfter(UPPER)   include/elemental/lapack-like/Cholesky/SolveAfter.hpp:33-43,56-72
  *   eo_trrk_tn         internal::TrrkTN       src/blas-like/Trrk/Local.hpp:733-775 (leaf :452-490)
  *   eo_local_trrk_tn   LocalTrrk (1x1 grid)   src/blas-like/Trrk/Local.hpp:931-976 (leaf :492-536)
  *   eo_scale_trapezoid ScaleTrapezoid         include/elemental/blas-like/level1/ScaleTrapezoid.hpp:15-84
@@ -143,12 +146,35 @@ void eo_dgemm(char ta, char tb, int m, int n, int k, double alpha,
     }
 }
 
-/* Lower-triangle SYRK: C := alpha*A*A^T + beta*C (trans N, A n x k) or
- * C := alpha*A^T*A + beta*C (trans T, A k x n).  Upper triangle untouched. */
+/* SYRK on one triangle of C (uplo 'L' or 'U'): C := alpha*A*A^T + beta*C
+ * (trans N, A n x k) or C := alpha*A^T*A + beta*C (trans T, A k x n).  The
+ * other triangle is untouched. */
 void eo_dsyrk(char uplo, char trans, int n, int k, double alpha,
               const double *A, int lda, double beta, double *C, int ldc) {
-    (void)uplo; /* lower only; the only uplo on this path */
     if (n == 0 || ((alpha == 0.0 || k == 0) && beta == 1.0)) return;
+    if (lsame(uplo, 'U')) {  /* upper triangle (Netlib dsyrk's UPLO = 'U' loops) */
+        if (lsame(trans, 'N')) {
+            for (int j = 0; j < n; ++j) {
+                if (beta == 0.0) for (int i = 0; i <= j; ++i) AT(C, i, j, ldc) = 0.0;
+                else if (beta != 1.0) for (int i = 0; i <= j; ++i) AT(C, i, j, ldc) = beta * AT(C, i, j, ldc);
+                if (alpha == 0.0) continue;
+                for (int l = 0; l < k; ++l) {
+                    if (AT(A, j, l, lda) != 0.0) {
+                        double temp = alpha * AT(A, j, l, lda);
+                        for (int i = 0; i <= j; ++i) AT(C, i, j, ldc) += temp * AT(A, i, l, lda);
+                    }
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i <= j; ++i) {
+                    double temp = 0.0;
+                    for (int l = 0; l < k; ++l) temp += AT(A, l, i, lda) * AT(A, l, j, lda);
+                    AT(C, i, j, ldc) = (beta == 0.0) ? alpha * temp : alpha * temp + beta * AT(C, i, j, ldc);
+                }
+        }
+        return;
+    }
     if (lsame(trans, 'N')) {
         for (int j = 0; j < n; ++j) {
             if (beta == 0.0) for (int i = j; i < n; ++i) AT(C, i, j, ldc) = 0.0;
@@ -171,19 +197,76 @@ void eo_dsyrk(char uplo, char trans, int n, int k, double alpha,
     }
 }
 
-/* Lower-triangular TRSM, non-unit or unit diagonal.
+/* Triangular TRSM (Netlib dtrsm loops), lower or upper, non-unit or unit
+ * diagonal.
  *   side L, trans N : B := alpha*inv(A)*B
  *   side L, trans T : B := alpha*inv(A^T)*B
  *   side R, trans T : B := alpha*B*inv(A^T)
  *   side R, trans N : B := alpha*B*inv(A)
- * A is m x m (side L) or n x n (side R), lower. */
+ * A is m x m (side L) or n x n (side R). */
+static void dtrsm_upper(int left, int notrans, int nounit, int m, int n, double alpha,
+                        const double *A, int lda, double *B, int ldb) {
+    if (left) {
+        if (notrans) { /* B := alpha*inv(A)*B */
+            for (int j = 0; j < n; ++j) {
+                if (alpha != 1.0) for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = alpha * AT(B, i, j, ldb);
+                for (int k = m - 1; k >= 0; --k) {
+                    if (AT(B, k, j, ldb) != 0.0) {
+                        if (nounit) AT(B, k, j, ldb) = AT(B, k, j, ldb) / AT(A, k, k, lda);
+                        for (int i = 0; i < k; ++i) AT(B, i, j, ldb) -= AT(B, k, j, ldb) * AT(A, i, k, lda);
+                    }
+                }
+            }
+        } else { /* B := alpha*inv(A^T)*B */
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < m; ++i) {
+                    double temp = alpha * AT(B, i, j, ldb);
+                    for (int k = 0; k < i; ++k) temp -= AT(A, k, i, lda) * AT(B, k, j, ldb);
+                    if (nounit) temp = temp / AT(A, i, i, lda);
+                    AT(B, i, j, ldb) = temp;
+                }
+        }
+    } else {
+        if (notrans) { /* B := alpha*B*inv(A) */
+            for (int j = 0; j < n; ++j) {
+                if (alpha != 1.0) for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = alpha * AT(B, i, j, ldb);
+                for (int k = 0; k < j; ++k)
+                    if (AT(A, k, j, lda) != 0.0)
+                        for (int i = 0; i < m; ++i) AT(B, i, j, ldb) -= AT(A, k, j, lda) * AT(B, i, k, ldb);
+                if (nounit) {
+                    double temp = 1.0 / AT(A, j, j, lda);
+                    for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = temp * AT(B, i, j, ldb);
+                }
+            }
+        } else { /* B := alpha*B*inv(A^T) */
+            for (int k = n - 1; k >= 0; --k) {
+                if (nounit) {
+                    double temp = 1.0 / AT(A, k, k, lda);
+                    for (int i = 0; i < m; ++i) AT(B, i, k, ldb) = temp * AT(B, i, k, ldb);
+                }
+                for (int j = 0; j < k; ++j) {
+                    if (AT(A, j, k, lda) != 0.0) {
+                        double temp = AT(A, j, k, lda);
+                        for (int i = 0; i < m; ++i) AT(B, i, j, ldb) -= temp * AT(B, i, k, ldb);
+                    }
+                }
+                if (alpha != 1.0)
+                    for (int i = 0; i < m; ++i) AT(B, i, k, ldb) = alpha * AT(B, i, k, ldb);
+            }
+        }
+    }
+}
+
 void eo_dtrsm(char side, char uplo, char transa, char diag, int m, int n,
               double alpha, const double *A, int lda, double *B, int ldb) {
-    (void)uplo; /* lower only */
     int nounit = lsame(diag, 'N');
     if (m == 0 || n == 0) return;
     if (alpha == 0.0) {
         for (int j = 0; j < n; ++j) for (int i = 0; i < m; ++i) AT(B, i, j, ldb) = 0.0;
+        return;
+    }
+    if (lsame(uplo, 'U')) {
+        dtrsm_upper(lsame(side, 'L'), lsame(transa, 'N'), nounit, m, n, alpha, A, lda, B, ldb);
         return;
     }
     if (lsame(side, 'L')) {
@@ -238,10 +321,29 @@ void eo_dtrsm(char side, char uplo, char transa, char diag, int m, int n,
     }
 }
 
-/* Lower-triangular TRSV with unit stride, trans N or T. */
+/* Triangular TRSV (Netlib dtrsv loops) with unit stride, lower or upper,
+ * trans N or T. */
 void eo_dtrsv(char uplo, char trans, char diag, int n, const double *A, int lda, double *x) {
-    (void)uplo;
     int nounit = lsame(diag, 'N');
+    if (lsame(uplo, 'U')) {
+        if (lsame(trans, 'N')) {
+            for (int j = n - 1; j >= 0; --j) {
+                if (x[j] != 0.0) {
+                    if (nounit) x[j] = x[j] / AT(A, j, j, lda);
+                    double temp = x[j];
+                    for (int i = j - 1; i >= 0; --i) x[i] -= temp * AT(A, i, j, lda);
+                }
+            }
+        } else {
+            for (int j = 0; j < n; ++j) {
+                double temp = x[j];
+                for (int i = 0; i < j; ++i) temp -= AT(A, i, j, lda) * x[i];
+                if (nounit) temp = temp / AT(A, j, j, lda);
+                x[j] = temp;
+            }
+        }
+        return;
+    }
     if (lsame(trans, 'N')) {
         for (int j = 0; j < n; ++j) {
             if (x[j] != 0.0) {
@@ -397,6 +499,63 @@ int eo_hpd_solve_hostblas(int n, int k, double *A, int lda, double *B, int ldb, 
     return 0;
 }
 
+/* UVar3Unb (Cholesky/UVar3.hpp:21-49): the upper factor U with A = U^T U
+ * overwrites the upper triangle row by row.  Returns 0 or the failing
+ * column + 1 (the reference throws logic_error there). */
+int eo_uvar3unb(int n, double *A, int lda) {
+    for (int j = 0; j < n; ++j) {
+        double alpha = AT(A, j, j, lda);
+        if (alpha <= 0.0) return j + 1;
+        alpha = sqrt(alpha);
+        AT(A, j, j, lda) = alpha;
+        for (int k = j + 1; k < n; ++k) AT(A, j, k, lda) /= alpha;
+        for (int k = j + 1; k < n; ++k)
+            for (int i = j + 1; i <= k; ++i)
+                AT(A, i, k, lda) -= AT(A, j, i, lda) * AT(A, j, k, lda);
+    }
+    return 0;
+}
+
+/* UVar3 (Cholesky/UVar3.hpp:85-125): per step UVar3Unb(A11);
+ * Trsm(LEFT, UPPER, ADJOINT, NON_UNIT, 1, A11, A12);
+ * Herk(UPPER, ADJOINT, -1, A12, 1, A22). */
+int eo_uvar3(int n, double *A, int lda, int nb) {
+    if (nb < 1) nb = 1;
+    for (int k = 0; k < n; k += nb) {
+        int b = (n - k < nb) ? n - k : nb;
+        int m2 = n - k - b;
+        double *A11 = &AT(A, k, k, lda);
+        int info = eo_uvar3unb(b, A11, lda);
+        if (info) return k + info;
+        if (m2 > 0) {
+            double *A12 = &AT(A, k, k + b, lda);
+            double *A22 = &AT(A, k + b, k + b, lda);
+            eo_dtrsm('L', 'U', 'T', 'N', b, m2, 1.0, A11, lda, A12, lda);
+            eo_dsyrk('U', 'T', m2, b, -1.0, A12, lda, 1.0, A22, lda);
+        }
+    }
+    return 0;
+}
+
+/* SolveAfter(UPPER, NORMAL, U, B) (SolveAfter.hpp:33-43, 56-72):
+ * B := inv(U) inv(U^T) B; two Trsv for one column, two Trsm otherwise. */
+void eo_solve_after_upper(int n, int k, const double *U, int ldu, double *B, int ldb) {
+    if (k == 1) {
+        eo_dtrsv('U', 'T', 'N', n, U, ldu, B);
+        eo_dtrsv('U', 'N', 'N', n, U, ldu, B);
+    } else {
+        eo_dtrsm('L', 'U', 'T', 'N', n, k, 1.0, U, ldu, B, ldb);
+        eo_dtrsm('L', 'U', 'N', 'N', n, k, 1.0, U, ldu, B, ldb);
+    }
+}
+
+int eo_hpd_solve_upper(int n, int k, double *A, int lda, double *B, int ldb, int nb) {
+    int info = eo_uvar3(n, A, lda, nb);
+    if (info) return info;
+    eo_solve_after_upper(n, k, A, lda, B, ldb);
+    return 0;
+}
+
 /* ScaleTrapezoid(alpha, LOWER, A, offset=0, side=LEFT). */
 void eo_scale_trapezoid_lower(double alpha, int m, int n, double *A, int lda) {
     for (int j = 0; j < n; ++j) {
@@ -460,17 +619,6 @@ void eo_trrk_tn(int k, int n, double alpha, const double *A, int lda,
 
 /* ------------------------------------------------------------------ */
 /* Norms used by the checks.                                             */
-
-/* Frobenius norm of the symmetric matrix whose lower triangle is stored. */
-double eo_sym_lower_fro(int n, const double *A, int lda) {
-    double s = 0.0;
-    for (int j = 0; j < n; ++j) {
-        double d = AT(A, j, j, lda);
-        s += d * d;
-        for (int i = j + 1; i < n; ++i) { double v = AT(A, i, j, lda); s += 2.0 * v * v; }
-    }
-    return sqrt(s);
-}
 
 /* R := B - A*X with A symmetric from its lower triangle (n x n), X, B n x k. */
 void eo_sym_lower_residual(int n, int k, const double *A, int lda, const double *X, int ldx,

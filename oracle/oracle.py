@@ -64,9 +64,14 @@ def lib():
         L.eo_scale_trapezoid_lower.argtypes = [d, i, i, vp, i]
         L.eo_axpy_triangle_lower.argtypes = [i, d, vp, i, vp, i]
         L.eo_trrk_tn.argtypes = [i, i, d, vp, i, vp, i, d, vp, i, i]
-        L.eo_sym_lower_fro.argtypes = [i, vp, i]
-        L.eo_sym_lower_fro.restype = d
         L.eo_sym_lower_residual.argtypes = [i, i, vp, i, vp, i, vp, i, vp, i]
+        L.eo_uvar3unb.argtypes = [i, vp, i]
+        L.eo_uvar3unb.restype = i
+        L.eo_uvar3.argtypes = [i, vp, i, i]
+        L.eo_uvar3.restype = i
+        L.eo_solve_after_upper.argtypes = [i, i, vp, i, vp, i]
+        L.eo_hpd_solve_upper.argtypes = [i, i, vp, i, vp, i, i]
+        L.eo_hpd_solve_upper.restype = i
         L.eo_bind_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i]
         L.eo_bind_blas.restype = i
         L.eo_blas_bound.restype = i
@@ -113,25 +118,25 @@ def dgemm(ta, tb, alpha, A, B, beta, C):
     return C
 
 
-def dsyrk(trans, alpha, A, beta, C):
+def dsyrk(trans, alpha, A, beta, C, uplo="L"):
     _f(A), _f(C)
     n = C.shape[0]
     k = A.shape[1] if trans.upper() == "N" else A.shape[0]
-    lib().eo_dsyrk(b"L", trans.encode(), n, k, alpha, _p(A), _ld(A), beta, _p(C), _ld(C))
+    lib().eo_dsyrk(uplo.encode(), trans.encode(), n, k, alpha, _p(A), _ld(A), beta, _p(C), _ld(C))
     return C
 
 
-def dtrsm(side, transa, alpha, L, B, diag="N"):
+def dtrsm(side, transa, alpha, L, B, diag="N", uplo="L"):
     _f(L), _f(B)
     m, n = B.shape
-    lib().eo_dtrsm(side.encode(), b"L", transa.encode(), diag.encode(), m, n, alpha,
+    lib().eo_dtrsm(side.encode(), uplo.encode(), transa.encode(), diag.encode(), m, n, alpha,
                    _p(L), _ld(L), _p(B), _ld(B))
     return B
 
 
-def dtrsv(trans, L, x, diag="N"):
+def dtrsv(trans, L, x, diag="N", uplo="L"):
     _f(L), _f(x)
-    lib().eo_dtrsv(b"L", trans.encode(), diag.encode(), L.shape[0], _p(L), _ld(L), _p(x))
+    lib().eo_dtrsv(uplo.encode(), trans.encode(), diag.encode(), L.shape[0], _p(L), _ld(L), _p(x))
     return x
 
 
@@ -164,6 +169,29 @@ def hpd_solve(A, B, nb: int = 128, host_blas: bool = False) -> int:
     return fn(A.shape[0], k, _p(A), _ld(A), _p(B), _ld(B), nb)
 
 
+def uvar3unb(A) -> int:
+    _f(A)
+    return lib().eo_uvar3unb(A.shape[0], _p(A), _ld(A))
+
+
+def uvar3(A, nb: int = 128) -> int:
+    _f(A)
+    return lib().eo_uvar3(A.shape[0], _p(A), _ld(A), nb)
+
+
+def solve_after_upper(U, B):
+    _f(U), _f(B)
+    k = B.shape[1] if B.ndim == 2 else 1
+    lib().eo_solve_after_upper(U.shape[0], k, _p(U), _ld(U), _p(B), _ld(B))
+    return B
+
+
+def hpd_solve_upper(A, B, nb: int = 128) -> int:
+    _f(A), _f(B)
+    k = B.shape[1] if B.ndim == 2 else 1
+    return lib().eo_hpd_solve_upper(A.shape[0], k, _p(A), _ld(A), _p(B), _ld(B), nb)
+
+
 def scale_trapezoid_lower(alpha, A):
     _f(A)
     lib().eo_scale_trapezoid_lower(alpha, A.shape[0], A.shape[1], _p(A), _ld(A))
@@ -183,11 +211,6 @@ def trrk_tn(alpha, A, B, beta, C, bsize: int = 64):
     k, n = A.shape
     lib().eo_trrk_tn(k, n, alpha, _p(A), _ld(A), _p(B), _ld(B), beta, _p(C), _ld(C), bsize)
     return C
-
-
-def sym_lower_fro(A) -> float:
-    _f(A)
-    return lib().eo_sym_lower_fro(A.shape[0], _p(A), _ld(A))
 
 
 def sym_lower_residual(A, X, B) -> np.ndarray:
@@ -218,3 +241,20 @@ def bind_host_blas(nthreads: int) -> str:
     if rc != 0:
         raise RuntimeError(f"binding host BLAS {path} failed (rc={rc})")
     return f"{name}: {os.path.basename(path)}"
+
+
+def host_blas_info() -> dict:
+    """Version string and threading layer of the bound host BLAS (OpenBLAS's
+    own get_config / get_parallel), for the CPU baseline's record."""
+    path, prefix, _, name = find_host_blas()
+    L = ctypes.CDLL(path)
+    info = {"blas": name, "library": os.path.basename(path)}
+    try:
+        f = getattr(L, prefix + "openblas_get_config")
+        f.restype = ctypes.c_char_p
+        info["config"] = f().decode()
+        g = getattr(L, prefix + "openblas_get_parallel")
+        info["threading"] = {0: "sequential", 1: "pthreads", 2: "OpenMP"}.get(int(g()), "unknown")
+    except AttributeError:
+        pass
+    return info

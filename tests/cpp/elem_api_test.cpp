@@ -104,6 +104,54 @@ int main() {
     std::printf("Trsm(R,L,T): max |X L^T - B| = %.3e\n", e2);
     fails += !(e2 < 1e-13);
 
+    // HPDSolve(UPPER): U in the upper triangle with U^T U = A, lower untouched
+    {
+        elem::Matrix<double> Au(A0), Bu(B0);
+        elem::HPDSolve(elem::UPPER, elem::NORMAL, Au, Bu);
+        double r2 = 0, utu = 0;
+        for (int c = 0; c < k; ++c)
+            for (int i = 0; i < n; ++i) {
+                double s = B0.Get(i, c);
+                for (int j = 0; j < n; ++j) s -= A0.Get(i, j) * Bu.Get(j, c);
+                r2 += s * s;
+            }
+        bool lower_ok = true;
+        for (int j = 0; j < n; ++j)
+            for (int i = j + 1; i < n; ++i) lower_ok &= (Au.Get(i, j) == A0.Get(i, j));
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i <= j; ++i) {
+                double s = 0;
+                for (int l = 0; l <= i; ++l) s += Au.Get(l, i) * Au.Get(l, j);
+                utu += (s - A0.Get(i, j)) * (s - A0.Get(i, j));
+            }
+        const double relu = std::sqrt(r2 / bn), du = std::sqrt(utu / an);
+        std::printf("HPDSolve(UPPER): rel residual %.3e, ||U^T U-A||/||A|| %.3e, lower untouched %d\n", relu, du,
+                    (int)lower_ok);
+        fails += !(relu < 1e-13 && du < 1e-14 && lower_ok);
+    }
+
+    // Matrix<float>: HPDSolve<float>
+    {
+        elem::Matrix<float> Af(n, n), Bf(n, k);
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) Af.Set(i, j, (float)A0.Get(i, j));
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < n; ++i) Bf.Set(i, j, (float)B0.Get(i, j));
+        elem::Matrix<float> Af0(Af), Bf0(Bf);
+        elem::HPDSolve(elem::LOWER, elem::NORMAL, Af, Bf);
+        double r2 = 0, b2 = 0;
+        for (int c = 0; c < k; ++c)
+            for (int i = 0; i < n; ++i) {
+                double s = Bf0.Get(i, c);
+                for (int j = 0; j < n; ++j) s -= (double)Af0.Get(i, j) * Bf.Get(j, c);
+                r2 += s * s;
+                b2 += (double)Bf0.Get(i, c) * Bf0.Get(i, c);
+            }
+        const double relf = std::sqrt(r2 / b2);
+        std::printf("HPDSolve<float>: rel residual %.3e\n", relf);
+        fails += !(relf < 1e-5);
+    }
+
     std::printf("%s\n", fails ? "FAILED" : "ALL OK");
     return fails ? 1 : 0;
 }

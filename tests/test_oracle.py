@@ -200,3 +200,154 @@ def test_spectrum_construction_condition():
     A = inputs.spectrum_host(200, 1e6, seed=1)
     w = np.linalg.eigvalsh(A)
     assert abs(w.max() - 1.0) < 1e-10 and abs(w.min() - 1e-6) < 1e-12
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (300, 1), (700, 16), (513, 33)])
+def test_host_blas_solve_vs_lapack_and_restated_loops(oracle, n, k):
+    """The host-BLAS legs (LVar3 + SolveAfter over OpenBLAS, the CPU baseline
+    and the reference X of the mixed tests) against LAPACK dpotrf/dpotrs and
+    against the restated Netlib loops, to 1e-14 relative; k = 1 takes the
+    dtrsv route of SolveAfter."""
+    oracle.bind_host_blas(2)
+    A = inputs.gram_host(n, seed=3 * n + k)
+    B = oracle.normal((n, k), seed=n + 7 * k)
+    F1, X1 = A.copy(order="F"), B.copy(order="F")
+    assert oracle.hpd_solve(F1, X1, 128, host_blas=True) == 0
+    F2, X2 = A.copy(order="F"), B.copy(order="F")
+    assert oracle.hpd_solve(F2, X2, 128) == 0
+    Xr = sl.cho_solve(sl.cho_factor(A, lower=True), B)
+    assert np.linalg.norm(X1 - Xr) / np.linalg.norm(Xr) <= 1e-14
+    assert np.linalg.norm(X1 - X2) / np.linalg.norm(X2) <= 1e-14
+    assert np.linalg.norm(np.tril(F1 - F2)) / np.linalg.norm(np.tril(F2)) <= 1e-14
+    # SolveAfter alone over the host BLAS, with LAPACK's factor
+    X3 = B.copy(order="F")
+    oracle.solve_after(np.asfortranarray(sl.cholesky(A, lower=True)), X3, host_blas=True)
+    assert np.linalg.norm(X3 - Xr) / np.linalg.norm(Xr) <= 1e-14
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (64, 3), (300, 16)])
+def test_host_blas_solve_integer_exact(oracle, n, k):
+    """Integer known answer through the host-BLAS legs: power-of-two pivots
+    keep every quotient exact, so X must be X0 bit for bit."""
+    oracle.bind_host_blas(2)
+    A, L0 = inputs.integer_factor(n, seed=5 * n + k)
+    rng = np.random.default_rng(n + k)
+    X0 = np.asfortranarray(rng.integers(-4, 5, (n, k)).astype(np.float64))
+    F, X = A.copy(order="F"), np.asfortranarray(A @ X0)
+    assert oracle.hpd_solve(F, X, 128, host_blas=True) == 0
+    assert np.array_equal(np.tril(F), L0)
+    assert np.array_equal(X, X0)
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (37, 5), (300, 16)])
+def test_sym_lower_residual_exact_at_random_integer_x(oracle, n, k):
+    """R = B - A X at an X that is NOT the solution, on integers, bit for bit
+    against numpy's product with the full symmetric matrix; the strictly upper
+    triangle holds garbage that must not be read."""
+    rng = np.random.default_rng(n * 31 + k)
+    Lw = np.tril(rng.integers(-9, 10, (n, n)).astype(np.float64))
+    Asym = Lw + np.tril(Lw, -1).T
+    Astored = np.asfortranarray(Lw + np.triu(rng.integers(1000, 2000, (n, n)).astype(np.float64), 1))
+    X = np.asfortranarray(rng.integers(-7, 8, (n, k)).astype(np.float64))
+    B = np.asfortranarray(rng.integers(-50, 51, (n, k)).astype(np.float64))
+    R = oracle.sym_lower_residual(Astored, X, B)
+    assert np.array_equal(R, B - Asym @ X)
+    # a mistake planted in the check's input (the upper triangle read as the
+    # matrix) is caught by the same comparison
+    assert not np.array_equal(oracle.sym_lower_residual(np.asfortranarray(np.triu(Astored)), X, B),
+                              B - Asym @ X) or n == 1
+
+
+# ------------------------------------------------------------------ UPPER
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 129, 300])
+def test_uvar3unb_integer_exact_and_transpose_identity(oracle, n):
+    """A = U0^T U0 with U0 = L0^T: UVar3Unb must give U0 exactly and leave the
+    strictly lower triangle untouched; and operation by operation it is
+    LVar3Unb on A^T (real data), so on a non-integer matrix the two agree bit
+    for bit."""
+    A, L0 = inputs.integer_factor(n, seed=3 * n + 1)
+    F = A.copy(order="F")
+    assert oracle.uvar3unb(F) == 0
+    assert np.array_equal(np.triu(F), L0.T)
+    assert np.array_equal(np.tril(F, -1), np.tril(A, -1))
+    G = inputs.gram_host(n, seed=n + 5)
+    G[np.triu_indices(n, 1)] += 0.0  # symmetric already
+    Fu, Fl = G.copy(order="F"), np.asfortranarray(G.T)
+    assert oracle.uvar3unb(Fu) == 0 and oracle.lvar3unb(Fl) == 0
+    assert np.array_equal(np.triu(Fu), np.tril(Fl).T)
+
+
+@pytest.mark.parametrize("n,nb", [(1, 128), (7, 3), (129, 128), (300, 64), (257, 1)])
+def test_uvar3_integer_exact(oracle, n, nb):
+    A, L0 = inputs.integer_factor(n, seed=11 * n + nb)
+    F = A.copy(order="F")
+    assert oracle.uvar3(F, nb) == 0
+    assert np.array_equal(np.triu(F), L0.T)
+    assert np.array_equal(np.tril(F, -1), np.tril(A, -1))
+
+
+@pytest.mark.parametrize("n,nb", [(100, 32), (300, 128), (513, 128)])
+def test_uvar3_vs_lapack_upper(oracle, n, nb):
+    A = inputs.gram_host(n, seed=2 * n)
+    F = A.copy(order="F")
+    assert oracle.uvar3(F, nb) == 0
+    Ur = sl.cholesky(A, lower=False)
+    assert np.linalg.norm(np.triu(F) - Ur) / np.linalg.norm(Ur) < 1e-14
+    U = np.triu(F)
+    assert np.linalg.norm(U.T @ U - A) / np.linalg.norm(A) < 1e-15
+
+
+@pytest.mark.parametrize("col", [0, 5, 128, 200])
+def test_uvar3_non_hpd_reports_column(oracle, col):
+    n = 260
+    A = inputs.gram_host(n, seed=13)
+    L = sl.cholesky(A, lower=True)
+    A[col, col] -= L[col, col] ** 2 * 1.5
+    F = A.copy(order="F")
+    assert oracle.uvar3(F, 128) == col + 1
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (9, 1), (64, 3), (300, 16)])
+def test_solve_after_upper_integer_exact_and_lapack(oracle, n, k):
+    A, L0 = inputs.integer_factor(n, seed=n + 2 * k)
+    rng = np.random.default_rng(n + k)
+    X0 = np.asfortranarray(rng.integers(-4, 5, (n, k)).astype(np.float64))
+    X = np.asfortranarray(A @ X0)
+    oracle.solve_after_upper(np.asfortranarray(L0.T), X)
+    assert np.array_equal(X, X0)
+    G = inputs.gram_host(n, seed=n)
+    B = oracle.normal((n, k), seed=n)
+    F, Xg = G.copy(order="F"), B.copy(order="F")
+    assert oracle.hpd_solve_upper(F, Xg, 128) == 0
+    Xr = sl.cho_solve(sl.cho_factor(G, lower=False), B)
+    assert np.linalg.norm(Xg - Xr) / np.linalg.norm(Xr) < 1e-13
+
+
+def test_upper_blas_restatements_vs_host_blas(oracle):
+    rng = np.random.default_rng(5)
+    m, n, k = 23, 31, 9
+    U = np.asfortranarray(np.triu(rng.standard_normal((n, n))) + 4 * np.eye(n))
+    for side, trans in (("L", "N"), ("L", "T"), ("R", "N"), ("R", "T")):
+        for diag in ("N", "U"):
+            B = np.asfortranarray(rng.standard_normal((n, m) if side == "L" else (m, n)))
+            X = B.copy(order="F")
+            oracle.dtrsm(side, trans, 0.5, U, X, diag=diag, uplo="U")
+            T = U.copy() if diag == "N" else np.triu(U, 1) + np.eye(n)
+            op = T if trans == "N" else T.T
+            lhs = op @ X if side == "L" else X @ op
+            assert np.allclose(lhs, 0.5 * B, atol=1e-12), (side, trans, diag)
+    for trans in ("N", "T"):
+        x0 = rng.standard_normal(n)
+        x = np.asfortranarray(x0.copy())
+        oracle.dtrsv(trans, U, x, uplo="U")
+        op = U if trans == "N" else U.T
+        assert np.allclose(op @ x, x0, atol=1e-12)
+        A = np.asfortranarray(rng.standard_normal((n, k) if trans == "N" else (k, n)))
+        C = np.asfortranarray(rng.standard_normal((n, n)))
+        C1 = C.copy(order="F")
+        oracle.dsyrk(trans, -0.75, A, 2.0, C1, uplo="U")
+        ref = -0.75 * (A @ A.T if trans == "N" else A.T @ A) + 2.0 * C
+        assert np.allclose(np.triu(C1), np.triu(ref), atol=1e-12)
+        assert np.array_equal(np.tril(C1, -1), np.tril(C, -1))
+        C2 = sb.dsyrk(-0.75, A, beta=2.0, c=C.copy(order="F"), trans=0 if trans == "N" else 1, lower=0)
+        assert np.allclose(np.triu(C1), np.triu(C2), atol=1e-12)
