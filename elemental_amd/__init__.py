@@ -26,7 +26,8 @@ __all__ = [
     "UpperOrLower", "Orientation", "LeftOrRight", "UnitOrNonUnit", "LOWER", "UPPER", "NORMAL",
     "TRANSPOSE", "ADJOINT", "LEFT", "RIGHT", "NON_UNIT", "UNIT", "Matrix", "View", "Blocksize",
     "Cholesky", "LocalCholesky", "HPDSolve", "SolveAfter", "Trsm", "Herk", "Syrk", "Trrk", "Trsv",
-    "HPDSolveMixed", "LogicError", "SingularMatrixError",
+    "HPDSolveMixed", "LogicError", "SingularMatrixError", "Gemm", "HPDInverse", "TriangularInverse",
+    "HermitianGenDefiniteEig", "HermitianGenDefiniteEigType", "AXBX", "ABX", "BAX",
 ]
 
 
@@ -209,6 +210,95 @@ def Trsv(uplo: UpperOrLower, orientation: Orientation, diag: UnitOrNonUnit, A: M
         raise NotImplementedError("x must be a column vector")
     _check(lib().elemamd_trsv_f64(_c(uplo), _c(orientation), _c(diag), A.Height(), A.ptr(), A.LDim(), x.ptr(), 1,
                                   _stream(stream)), "elemamd_trsv_f64")
+
+
+def _orient(o) -> bytes:
+    return ctypes.c_char((o.value if isinstance(o, Orientation) else str(o)).encode())
+
+
+def Gemm(orientationOfA, orientationOfB, alpha: float, A: Matrix, B: Matrix, beta: float, C: Matrix,
+         stream=None) -> None:
+    """elem::Gemm(Matrix) (blas-like/level3/Gemm.hpp): C := alpha op(A) op(B)
+    + beta C (Orientation values or 'N' / 'T' / 'C')."""
+    ta = _orient(orientationOfA).value.decode() != "N"
+    k = A.Height() if ta else A.Width()
+    _check(lib().elemamd_gemm_f64(_orient(orientationOfA), _orient(orientationOfB), C.Height(), C.Width(), k,
+                                  float(alpha), A.ptr(), A.LDim(), B.ptr(), B.LDim(), float(beta), C.ptr(),
+                                  C.LDim(), _stream(stream)), "elemamd_gemm_f64")
+
+
+def HPDInverse(uplo: UpperOrLower, A: Matrix, nb: int | None = None, stream=None) -> None:
+    """elem::HPDInverse(uplo, Matrix) (lapack-like/Inverse.hpp:29-40 ->
+    hpd_inverse::CholeskyLVar2, Inverse/CholeskyLVar2.hpp:29-75): the uplo
+    triangle of A becomes that of inv(A), the other triangle is untouched.
+    Raises LogicError ("A was not numerically HPD") like the reference."""
+    if A.Height() != A.Width():
+        raise LogicError("Nonsquare matrices cannot be triangular")
+    nb = nb or _DEFAULT_BLOCKSIZE
+    d = _dinfo()
+    s = _stream(stream)
+    _check(lib().elemamd_hpd_inverse_f64(_c(uplo), A.Height(), A.ptr(), A.LDim(), nb, ctypes.c_void_p(d.data_ptr()),
+                                         s), "elemamd_hpd_inverse_f64")
+    _raise_info(lib().elemamd_info_read(ctypes.c_void_p(d.data_ptr()), s))
+
+
+def TriangularInverse(uplo: UpperOrLower, diag: UnitOrNonUnit, A: Matrix, nb: int | None = None,
+                      stream=None) -> None:
+    """elem::TriangularInverse(uplo, diag, Matrix) (lapack-like/
+    TriangularInverse.hpp -> triangular_inverse::LVar3 / UVar3): A := inv(A)
+    on the uplo triangle."""
+    if A.Height() != A.Width():
+        raise LogicError("Nonsquare matrices cannot be triangular")
+    _check(lib().elemamd_triangular_inverse_f64(_c(uplo), _c(diag), A.Height(), A.ptr(), A.LDim(),
+                                                nb or _DEFAULT_BLOCKSIZE, _stream(stream)),
+           "elemamd_triangular_inverse_f64")
+
+
+class HermitianGenDefiniteEigType(enum.Enum):  # lapack-like_decl.hpp: AXBX, ABX, BAX
+    AXBX = 1
+    ABX = 2
+    BAX = 3
+
+
+AXBX, ABX, BAX = HermitianGenDefiniteEigType.AXBX, HermitianGenDefiniteEigType.ABX, HermitianGenDefiniteEigType.BAX
+
+
+def HermitianGenDefiniteEig(type: HermitianGenDefiniteEigType, uplo: UpperOrLower, A: Matrix, B: Matrix,
+                            vectors: bool = False, lowerBound=None, upperBound=None, stream=None):
+    """elem::HermitianGenDefiniteEig(type, uplo, A, B, w[, X][, a, b])
+    (lapack-like/HermitianGenDefiniteEig.hpp): Cholesky(uplo, B), the
+    two-sided reduction of A, the standard eigenproblem and the
+    back-transformation.  Integer bounds select eigenvalues a..b (0-based,
+    inclusive), float bounds the eigenvalues in (a, b].  Returns (w, X): w a
+    nev x 1 Matrix of ascending eigenvalues, X the n x nev eigenvectors (None
+    unless ``vectors``).  B not numerically HPD raises LogicError."""
+    import torch
+
+    n = A.Height()
+    if A.Width() != n or B.Height() != B.Width() or B.Height() != n:
+        raise LogicError("Hermitian matrices must be square.")
+    rng, il, iu, vl, vu = 0, 0, 0, 0.0, 0.0
+    if lowerBound is not None:
+        if isinstance(lowerBound, int) and isinstance(upperBound, int):
+            rng, il, iu = 1, lowerBound, upperBound
+        else:
+            rng, vl, vu = 2, float(lowerBound), float(upperBound)
+    wbuf = torch.empty(max(n, 1), dtype=torch.float64, device="cuda")
+    X = Matrix(n, n) if vectors else None
+    nev, info = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().elemamd_hermitian_gen_definite_eig_f64(
+        type.value, _c(uplo), n, A.ptr(), A.LDim(), B.ptr(), B.LDim(), ctypes.c_void_p(wbuf.data_ptr()),
+        X.ptr() if X is not None else None, X.LDim() if X is not None else 1, rng, il, iu, vl, vu,
+        ctypes.byref(nev), ctypes.byref(info), _stream(stream)), "elemamd_hermitian_gen_definite_eig_f64")
+    if info.value > 0:
+        raise LogicError("A was not numerically HPD", column=info.value - 1)
+    k = nev.value
+    w = Matrix(k, 1)
+    if k > 0:
+        w.tensor()[:, 0].copy_(wbuf[:k])
+    if X is not None:
+        X = View(X, 0, 0, n, k)
+    return w, X
 
 
 from .mixed import HPDSolveMixed, IROptions, IRReport  # noqa: E402,F401

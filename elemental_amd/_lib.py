@@ -38,6 +38,7 @@ class IROpts(ctypes.Structure):
         ("allow_shift", ctypes.c_int),
         ("tol", ctypes.c_double),
         ("refine", ctypes.c_int),
+        ("fallback", ctypes.c_int),
     ]
 
 
@@ -52,6 +53,7 @@ class IRReportC(ctypes.Structure):
         ("converged", ctypes.c_int),
         ("backward_error", ctypes.c_double),
         ("anorm_inf", ctypes.c_double),
+        ("fallback", ctypes.c_int),
     ]
 
 
@@ -88,6 +90,11 @@ SIGNATURES = {
     "elemamd_cholesky_f64": (_i, [_c, _i, _vp, _i64, _i, _vp, _vp]),
     "elemamd_solve_after_f64": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _vp]),
     "elemamd_hpd_solve_f64": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _i, _vp, _vp]),
+    "elemamd_gemm_f64": (_i, [_c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
+    "elemamd_hpd_inverse_f64": (_i, [_c, _i, _vp, _i64, _i, _vp, _vp]),
+    "elemamd_triangular_inverse_f64": (_i, [_c, _c, _i, _vp, _i64, _i, _vp]),
+    "elemamd_hermitian_gen_definite_eig_f64": (_i, [_i, _c, _i, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i, _i, _i,
+                                                    _d, _d, _vp, _vp, _vp]),
     "elemamd_cholesky_f32": (_i, [_c, _i, _vp, _i64, _i, _vp, _vp]),
     "elemamd_solve_after_f32": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _vp]),
     "elemamd_hpd_solve_f32": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _i, _vp, _vp]),

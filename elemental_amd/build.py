@@ -24,6 +24,7 @@ SOURCES = [
     "trsm_panel.hip",
     "syrk.hip",
     "gemm.hip",
+    "inverse.hip", "hpd_inverse.cpp", "gen_eig.cpp",
     "solve.hip",
     "solve_chain.hip", "solve_sweep.hip",
     "blas1.hip",
@@ -59,7 +60,10 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    # rocSOLVER runs the standard symmetric eigenproblem of the generalized
+    # eigensolver (gen_eig.cpp)
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

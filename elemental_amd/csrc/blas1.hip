@@ -105,6 +105,56 @@ hipError_t launch_transpose_inplace_f64(int n, double* A, i64 lda, hipStream_t s
     return launch_transpose_inplace<double>(n, A, lda, s);
 }
 
+// F (n x n, ldf) := the symmetric matrix stored in A's lower triangle
+// (mirror = 1) or A's lower triangle with zeros above (mirror = 0); A's
+// strictly upper triangle is never read.  One workgroup per lower tile pair.
+__global__ __launch_bounds__(256) void from_lower_kernel(int n, const double* __restrict__ A, i64 lda,
+                                                         double* __restrict__ F, i64 ldf, int nt, int mirror) {
+    __shared__ double s1[kTT][kTT + 1];
+    const i64 e = blockIdx.x;
+    int I = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+    while ((i64)I * (I + 1) / 2 > e) --I;
+    while ((i64)(I + 1) * (I + 2) / 2 <= e) ++I;
+    const int J = (int)(e - (i64)I * (I + 1) / 2);
+    if (I >= nt) return;
+    const int t = threadIdx.x, r = t & 31, c0 = t >> 5;
+    const i64 i0 = (i64)I * kTT, j0 = (i64)J * kTT;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 8 * q;  // s1[c][r] = A(i0+r, j0+c), lower part only
+        const bool in = i0 + r < n && j0 + c < n && (I != J || r >= c);
+        s1[c][r] = in ? A[(i0 + r) + (j0 + c) * lda] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c0 + 8 * q;
+        if (i0 + r < n && j0 + c < n) {  // F(i0+r, j0+c)
+            double v = s1[c][r];
+            if (I == J && r < c) v = mirror ? s1[r][c] : 0.0;
+            F[(i0 + r) + (j0 + c) * ldf] = v;
+        }
+        if (I != J && j0 + r < n && i0 + c < n)  // F(j0+r, i0+c) = A(i0+c, j0+r) = s1[r][c]
+            F[(j0 + r) + (i0 + c) * ldf] = mirror ? s1[r][c] : 0.0;
+    }
+}
+
+hipError_t launch_symmetrize_lower_f64(int n, const double* A, i64 lda, double* F, i64 ldf, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int nt = (int)ceil_div(n, kTT);
+    hipLaunchKernelGGL(from_lower_kernel, dim3((unsigned)((i64)nt * (nt + 1) / 2)), dim3(256), 0, s, n, A, lda, F,
+                       ldf, nt, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_lower_zero_upper_f64(int n, const double* A, i64 lda, double* F, i64 ldf, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int nt = (int)ceil_div(n, kTT);
+    hipLaunchKernelGGL(from_lower_kernel, dim3((unsigned)((i64)nt * (nt + 1) / 2)), dim3(256), 0, s, n, A, lda, F,
+                       ldf, nt, 0);
+    return hipGetLastError();
+}
+
 // B (n x m) := A^T, A m x n.
 __global__ __launch_bounds__(256) void transpose_kernel(int m, int n, const double* __restrict__ A, i64 lda,
                                                         double* __restrict__ B, i64 ldb) {

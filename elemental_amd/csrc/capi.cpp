@@ -246,6 +246,86 @@ int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double* A, 
     return solve_read_error(solve_ctrl<double>(n, work), s);
 }
 
+// ------------------------------------------------- inverses and Gemm
+int elemamd_gemm_f64(char ta, char tb, int m, int n, int k, double alpha, const double* A, int64_t lda,
+                     const double* B, int64_t ldb, double beta, double* C, int64_t ldc, void* stream) {
+    if (!valid_orient(ta) || !valid_orient(tb) || m < 0 || n < 0 || k < 0) return ELEMAMD_ERR_ARG;
+    const bool TA = is_trans(ta), TB = is_trans(tb);
+    if (lda < std::max(1, TA ? k : m) || ldb < std::max(1, TB ? n : k) || ldc < std::max(1, m))
+        return ELEMAMD_ERR_ARG;
+    return launch_gemm_f64(TA, TB, m, n, k, alpha, A, lda, kGemmShapeFull, B, ldb, kGemmShapeFull, beta, C, ldc, 0,
+                           nullptr, S(stream)) == hipSuccess
+               ? ELEMAMD_OK
+               : ELEMAMD_ERR_HIP;
+}
+
+int elemamd_hpd_inverse_f64(char uplo, int n, double* A, int64_t lda, int nb, int* dinfo, void* stream) {
+    uplo = up(uplo);
+    if (uplo != 'L' && uplo != 'U') return ELEMAMD_ERR_ARG;
+    if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128 || dinfo == nullptr) return ELEMAMD_ERR_ARG;
+    hipStream_t s = S(stream);
+    void* w = stream_scratch(inverse_scratch_bytes(n, nb), s);
+    if (w == nullptr) return ELEMAMD_ERR_HIP;
+    // UPPER (hpd_inverse::CholeskyUVar2) on real data is the lower algorithm
+    // on A^T: transpose in place, invert, transpose back
+    if (uplo == 'U' && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    const int rc = hpd_inverse_lower(n, A, lda, nb, dinfo, w, s);
+    if (uplo == 'U' && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    return rc;
+}
+
+int elemamd_triangular_inverse_f64(char uplo, char diag, int n, double* A, int64_t lda, int nb, void* stream) {
+    uplo = up(uplo);
+    diag = up(diag);
+    if ((uplo != 'L' && uplo != 'U') || (diag != 'N' && diag != 'U')) return ELEMAMD_ERR_ARG;
+    if (n < 0 || lda < std::max(1, n) || nb < 1 || nb > 128) return ELEMAMD_ERR_ARG;
+    hipStream_t s = S(stream);
+    void* w = stream_scratch(inverse_scratch_bytes(n, nb), s);
+    if (w == nullptr) return ELEMAMD_ERR_HIP;
+    // UPPER: inv(U) = inv(U^T)^T, the lower algorithm on the transpose
+    if (uplo == 'U' && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    const int rc = triangular_inverse_lower(n, A, lda, diag == 'U', nb, w, s);
+    if (uplo == 'U' && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    return rc;
+}
+
+int elemamd_hermitian_gen_definite_eig_f64(int type, char uplo, int n, double* A, int64_t lda, double* B,
+                                           int64_t ldb, double* w, double* X, int64_t ldx, int range, int il, int iu,
+                                           double vl, double vu, int* nev, int* info, void* stream) {
+    uplo = up(uplo);
+    if (type < ELEMAMD_AXBX || type > ELEMAMD_BAX || (uplo != 'L' && uplo != 'U')) return ELEMAMD_ERR_ARG;
+    if (n < 0 || lda < std::max(1, n) || ldb < std::max(1, n) || nev == nullptr || info == nullptr)
+        return ELEMAMD_ERR_ARG;
+    if ((n > 0 && w == nullptr) || (X != nullptr && ldx < std::max(1, n))) return ELEMAMD_ERR_ARG;
+    if (range == ELEMAMD_EIG_INDEX) {
+        if (n > 0 && (il < 0 || il > iu || iu >= n)) return ELEMAMD_ERR_ARG;
+    } else if (range == ELEMAMD_EIG_VALUE) {
+        if (!(vl < vu)) return ELEMAMD_ERR_ARG;
+    } else if (range != ELEMAMD_EIG_ALL) {
+        return ELEMAMD_ERR_ARG;
+    }
+    *nev = 0;
+    *info = 0;
+    if (n == 0) return ELEMAMD_OK;
+    hipStream_t s = S(stream);
+    void* work = stream_scratch(gen_eig_scratch_bytes(n), s);
+    if (work == nullptr) return ELEMAMD_ERR_HIP;
+    // UPPER on real data: the same reduction with U = L^T, run on the
+    // transposes of A and B and transposed back (the factor and the reduced
+    // matrix end in the upper triangles)
+    const bool upper = uplo == 'U';
+    if (upper && (launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess ||
+                  launch_transpose_inplace_f64(n, B, ldb, s) != hipSuccess))
+        return ELEMAMD_ERR_HIP;
+    const int rc =
+        hermitian_gen_definite_eig_lower(type, n, A, lda, B, ldb, w, X, ldx, range, il, iu, vl, vu, nev, info, work, s);
+    if (upper && (launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess ||
+                  launch_transpose_inplace_f64(n, B, ldb, s) != hipSuccess))
+        return ELEMAMD_ERR_HIP;
+    if (upper && hipStreamSynchronize(s) != hipSuccess) return ELEMAMD_ERR_HIP;
+    return rc;
+}
+
 // ------------------------------------------------------- Matrix<float>
 // Cholesky / SolveAfter / HPDSolve with F = float: the float factorization
 // (FP64 diagonal blocks, FP32 panels by products with the diagonal pieces'
@@ -327,6 +407,7 @@ static MixedOpts to_opts(const elemamd_ir_opts* o) {
         m.allow_shift = o->allow_shift;
         m.tol = o->tol > 0.0 ? o->tol : 1e-14;
         m.refine = o->refine;
+        m.fallback = o->fallback;
     }
     return m;
 }
@@ -355,6 +436,7 @@ int elemamd_hpd_solve_mixed_f64(int n, int k, const double* A, int64_t lda, doub
         report->converged = r.converged;
         report->backward_error = r.backward_error;
         report->anorm_inf = r.anorm_inf;
+        report->fallback = r.fallback;
     }
     return rc;
 }

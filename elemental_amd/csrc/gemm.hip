@@ -1,19 +1,30 @@
-// C := beta*C + alpha * A^T * B with A k x m and B k x n (column-major),
-// optionally restricted to the lower or the upper triangle of a square C.
+// C := beta*C + alpha * op(A) * op(B), FP64, op(A) m x k, op(B) k x n
+// (column-major), optionally restricted to the lower or the upper triangle
+// of a square C, with triangular operand masks.
 //
-// This is the product of Elemental's LocalTrrk / Trrk(LOWER, TRANSPOSE,
-// NORMAL, ...) (src/blas-like/Trrk/Local.hpp:733-775 and :931-976, leaf
-// kernels :452-536), the triangular rank-k update the DistMatrix route of the
-// blocked Cholesky uses for its trailing matrix
-// (include/elemental/lapack-like/Cholesky/LVar3Square.hpp).  The reference
-// recurses by halves down to 64x64 leaves and then does full square GEMMs
-// plus AxpyTriangle; on the GPU the lower tiles are simply enumerated and
-// diagonal tiles mask their upper part, which gives the same lower trapezoid.
+// Uses:
+//  * Elemental's LocalTrrk / Trrk(LOWER, TRANSPOSE, NORMAL, ...)
+//    (src/blas-like/Trrk/Local.hpp:733-775 and :931-976, leaf kernels
+//    :452-536), the triangular rank-k update of the DistMatrix route of the
+//    blocked Cholesky (include/elemental/lapack-like/Cholesky/LVar3Square.hpp).
+//    The reference recurses by halves down to 64x64 leaves and then does full
+//    square GEMMs plus AxpyTriangle; on the GPU the lower tiles are simply
+//    enumerated and diagonal tiles mask their upper part, which gives the same
+//    lower trapezoid.
+//  * elem::Gemm (include/elemental/blas-like/level3/Gemm.hpp) in every
+//    orientation, and the products with a triangular factor or its inverse
+//    (Trmm-like) of the HPD inverse, the triangular inverse and the
+//    two-sided reductions: a masked operand reads as zero outside its
+//    triangle (and as one on a unit diagonal), so the other triangle of the
+//    stored matrix is never used.
 //
-// Tiling as in the SYRK kernel: 128x128 tiles, 4 waves of 64x64,
-// v_mfma_f64_16x16x4_f64, double-buffered LDS chunks of 16 along k.
-// Operand columns are contiguous in k here, so each 16 lanes of a load
-// instruction fetch one 128-byte column segment.
+// Tiling: 128x128 tiles, 4 waves of 64x64, v_mfma_f64_16x16x4_f64,
+// double-buffered LDS chunks of 16 along k.  Global loads follow the stored
+// layout: an operand whose k index runs down its columns (A transposed, B not)
+// is read as 8 consecutive k values of one column per thread; one whose m or
+// n index runs down the columns (A not transposed, B transposed) as 8
+// consecutive rows of one column, 16 threads covering one 128-element column
+// segment.  Either way a wave's 64 lanes read contiguous 64-byte pieces.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -31,12 +42,73 @@ __device__ inline void gemm_tri_tile(int t, int& ti, int& tj) {
     tj = t - (int)((i64)r * (r + 1) / 2);
 }
 
-__global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k, double alpha,
-                                                             const double* __restrict__ A, i64 lda,
-                                                             const double* __restrict__ B, i64 ldb, double beta,
-                                                             double* __restrict__ C, i64 ldc, int tri) {
-    __shared__ double sA[2][kGemmKC][kGemmLdp];  // sA[l][i] = A(l, i0+i)
-    __shared__ double sB[2][kGemmKC][kGemmLdp];  // sB[l][j] = B(l, j0+j)
+// Stored element (r, c) of a masked operand (kGemmShape*).
+__device__ __forceinline__ double gemm_mask(double v, int r, int c, int shape) {
+    if (shape == kGemmShapeFull) return v;
+    const bool lower = shape == kGemmShapeLower || shape == kGemmShapeLowerUnit;
+    const bool unit = shape == kGemmShapeLowerUnit || shape == kGemmShapeUpperUnit;
+    if (r == c) return unit ? 1.0 : v;
+    return (lower ? r > c : r < c) ? v : 0.0;
+}
+
+// One operand tile of 128 (m or n) x 16 (k) into registers, then LDS s[l][x]
+// = op(M)(x0 + x, l) with x the m (A) or n (B) index.  KCONTIG: the stored
+// matrix holds k down its columns (element (l, x) at M[l + x*ld]); otherwise
+// element (x, l) at M[x + l*ld].
+template <bool KCONTIG>
+struct GemmLoader {
+    const double* M;
+    i64 ld;
+    int ext, k, x0, shape;
+    int tid;
+    double g[8];
+    __device__ __forceinline__ void load(int kc0) {
+        if constexpr (KCONTIG) {
+            const int x = tid >> 1, lh = (tid & 1) * 8;
+            const bool xin = (x0 + x) < ext;
+            const double* p = M + (i64)(x0 + x) * ld;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int l = kc0 + lh + q;
+                const double v = (xin && l < k) ? p[l] : 0.0;
+                g[q] = gemm_mask(v, l, x0 + x, shape);  // stored (row l, column x)
+            }
+        } else {
+            const int l = kc0 + (tid >> 4), xb = (tid & 15) * 8;
+            const bool lin = l < k;
+            const double* p = M + (i64)l * ld + x0 + xb;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int x = x0 + xb + q;
+                const double v = (lin && x < ext) ? p[q] : 0.0;
+                g[q] = gemm_mask(v, x, l, shape);  // stored (row x, column l)
+            }
+        }
+    }
+    __device__ __forceinline__ void store(double (*s)[kGemmLdp]) const {
+        if constexpr (KCONTIG) {
+            const int x = tid >> 1, lh = (tid & 1) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s[lh + q][x] = g[q];
+        } else {
+            const int l = tid >> 4, xb = (tid & 15) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s[l][xb + q] = g[q];
+        }
+    }
+};
+
+template <bool TA, bool TB>
+// C may be the same memory as A when op(B) has at most 128 columns (one
+// tile column): each workgroup reads all of its rows of A before it writes
+// them, and rows are disjoint between workgroups (hence no __restrict__).
+__global__ __launch_bounds__(256, 2) void gemm_f64_kernel(int m, int n, int k, double alpha, const double* A,
+                                                          i64 lda, int ashape, const double* B, i64 ldb, int bshape,
+                                                          double beta, double* C, i64 ldc, int tri,
+                                                          const int* __restrict__ info) {
+    __shared__ double sA[2][kGemmKC][kGemmLdp];  // sA[l][i] = op(A)(i0+i, l)
+    __shared__ double sB[2][kGemmKC][kGemmLdp];  // sB[l][j] = op(B)(l, j0+j)
+    if (info != nullptr && *info != 0) return;   // an earlier step failed
     // tri: 0 every tile; 1 the lower triangle (tiles ti >= tj, j <= i);
     // 2 the upper triangle (tiles ti <= tj, i <= j)
     int ti, tj;
@@ -53,41 +125,27 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
     const bool skip = (ti == tj) && ((tri == 1 && wj > wi) || (tri == 2 && wi > wj));
 
-    // loader: thread -> index column c = tid >> 1, l half (tid & 1) * 8 .. +8
-    const int lc = tid >> 1, lh = (tid & 1) * 8;
-    const bool colA = (i0 + lc) < m, colB = (j0 + lc) < n;
-    const double* pA = A + (i64)(i0 + lc) * lda;
-    const double* pB = B + (i64)(j0 + lc) * ldb;
-    double gA[8], gB[8];
-    auto gload = [&](int kc0) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int l = kc0 + lh + q;
-            const bool lin = l < k;
-            gA[q] = (colA && lin) ? pA[l] : 0.0;
-            gB[q] = (colB && lin) ? pB[l] : 0.0;
-        }
-    };
-    auto sstore = [&](int buf) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            sA[buf][lh + q][lc] = gA[q];
-            sB[buf][lh + q][lc] = gB[q];
-        }
-    };
+    // op(A) = A^T: A stored k x m (k down the columns); op(B) = B: B stored k x n
+    GemmLoader<TA> la{A, lda, m, k, i0, ashape, tid, {}};
+    GemmLoader<!TB> lb{B, ldb, n, k, j0, bshape, tid, {}};
     d4_t acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = (d4_t){0.0, 0.0, 0.0, 0.0};
-    gload(0);
-    sstore(0);
+    la.load(0);
+    lb.load(0);
+    la.store(sA[0]);
+    lb.store(sB[0]);
     __syncthreads();
     const int nchunks = (int)ceil_div(k, kGemmKC);
     for (int c = 0; c < nchunks; ++c) {
         const int buf = c & 1;
         const bool more = (c + 1) < nchunks;
-        if (more) gload((c + 1) * kGemmKC);
+        if (more) {
+            la.load((c + 1) * kGemmKC);
+            lb.load((c + 1) * kGemmKC);
+        }
         if (!skip) {
 #pragma unroll
             for (int ks = 0; ks < kGemmKC / 4; ++ks) {
@@ -104,10 +162,20 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k
                         acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[mb], fb[nb], acc[mb][nb], 0, 0, 0);
             }
         }
-        if (more) sstore(buf ^ 1);
+        if (more) {
+            la.store(sA[buf ^ 1]);
+            lb.store(sB[buf ^ 1]);
+        }
         __syncthreads();
     }
     if (skip) return;
+    // dependent zero-operand MFMAs before the accumulators are read (the
+    // compiler's wait states after a loop back edge are short for FP64 MFMA)
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+            acc[mb][nb] = __builtin_amdgcn_mfma_f64_16x16x4f64(0.0, 0.0, acc[mb][nb], 0, 0, 0);
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
@@ -125,18 +193,30 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_f64_kernel(int m, int n, int k
         }
 }
 
+hipError_t launch_gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, i64 lda, int ashape,
+                           const double* B, i64 ldb, int bshape, double beta, double* C, i64 ldc, int tri,
+                           const int* info, hipStream_t s) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (tri != 0 && m != n) return hipErrorInvalidValue;
+    const i64 T = ceil_div(m, kGemmTile);
+    const dim3 grid = tri != 0 ? dim3((unsigned)(T * (T + 1) / 2))
+                               : dim3((unsigned)T, (unsigned)ceil_div(n, kGemmTile));
+    k = k < 0 ? 0 : k;
+#define ELEMAMD_GEMM_CASE(TA_, TB_)                                                                              \
+    hipLaunchKernelGGL((gemm_f64_kernel<TA_, TB_>), grid, dim3(256), 0, s, m, n, k, alpha, A, lda, ashape, B, ldb, \
+                       bshape, beta, C, ldc, tri, info)
+    if (ta && tb) ELEMAMD_GEMM_CASE(true, true);
+    else if (ta) ELEMAMD_GEMM_CASE(true, false);
+    else if (tb) ELEMAMD_GEMM_CASE(false, true);
+    else ELEMAMD_GEMM_CASE(false, false);
+#undef ELEMAMD_GEMM_CASE
+    return hipGetLastError();
+}
+
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
                               double beta, double* C, i64 ldc, int tri, hipStream_t s) {
-    if (m <= 0 || n <= 0) return hipSuccess;
-    if (tri != 0) {  // square C
-        const i64 T = ceil_div(m, kGemmTile);
-        hipLaunchKernelGGL(gemm_tn_f64_kernel, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, s, m, n, k, alpha, A,
-                           lda, B, ldb, beta, C, ldc, tri);
-    } else {
-        hipLaunchKernelGGL(gemm_tn_f64_kernel, dim3((unsigned)ceil_div(m, kGemmTile), (unsigned)ceil_div(n, kGemmTile)),
-                           dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 0);
-    }
-    return hipGetLastError();
+    return launch_gemm_f64(true, false, m, n, k, alpha, A, lda, kGemmShapeFull, B, ldb, kGemmShapeFull, beta, C, ldc,
+                           tri, nullptr, s);
 }
 
 }  // namespace elemamd

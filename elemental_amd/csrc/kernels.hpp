@@ -73,10 +73,37 @@ hipError_t launch_syrk_col_bf16_planes(int np, int m, int ncol, int rbeg, int re
 hipError_t launch_syrk_diag_bf16_planes(int np, int ncol, int k, const unsigned short* planes, i64 ps, i64 ldk,
                                         float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
 
-// gemm.hip -- C := beta*C + alpha*A^T*B (TN), general, for Trrk.
-// tri: 0 all of C; 1 its lower triangle; 2 its upper triangle (1, 2: m == n)
+// gemm.hip -- C := beta*C + alpha*op(A)*op(B), op(A) m x k, op(B) k x n;
+// ta: A stored k x m (op = T), tb: B stored n x k (op = T).
+// tri: 0 all of C; 1 its lower triangle; 2 its upper triangle (1, 2: m == n).
+// Operand shapes mask the stored matrix: outside the triangle reads as zero,
+// a unit diagonal as one.  info (may be null): the launch does nothing when
+// *info != 0.  beta == 0: C is not read.
+enum : int {
+    kGemmShapeFull = 0,
+    kGemmShapeLower = 1,
+    kGemmShapeLowerUnit = 2,
+    kGemmShapeUpper = 3,
+    kGemmShapeUpperUnit = 4,
+};
+hipError_t launch_gemm_f64(bool ta, bool tb, int m, int n, int k, double alpha, const double* A, i64 lda, int ashape,
+                           const double* B, i64 ldb, int bshape, double beta, double* C, i64 ldc, int tri,
+                           const int* info, hipStream_t s);
+// the TN form without masks (Trrk)
 hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
                               double beta, double* C, i64 ldc, int tri, hipStream_t s);
+
+// inverse.hip -- n x n (n <= 128) blocks by one workgroup, bit-identical to
+// the reference's unblocked loops in IEEE double (no contraction).
+// triangular_inverse::LVar3Unb on the lower triangle of L: X (ldx, full n x n:
+// zeros above the diagonal, ones on it for a unit diagonal) and, when Lout
+// is not null, the lower triangle of Lout (strictly lower for unit) receive
+// inv(L).  L and Lout may be the same block.
+hipError_t launch_trtri_unb_f64(int n, const double* L, i64 ldl, bool unit, double* X, i64 ldx, double* Lout,
+                                i64 ldo, const int* info, hipStream_t s);
+// internal::TrtrmmLUnblocked(TRANSPOSE) on the lower triangle of X: the lower
+// triangle of C := X^T X.
+hipError_t launch_trtrmm_unb_f64(int n, const double* X, i64 ldx, double* C, i64 ldc, const int* info, hipStream_t s);
 
 // solve.hip -- persistent triangular solves with precomputed inverses of the
 // 64x64 diagonal blocks.  Dinv holds solve_dinv_count(n) elements, ctrl
@@ -140,6 +167,10 @@ hipError_t launch_transpose_inplace_f64(int n, double* A, i64 lda, hipStream_t s
 template <typename T>
 hipError_t launch_transpose_inplace(int n, T* A, i64 lda, hipStream_t s);
 hipError_t launch_transpose_f64(int m, int n, const double* A, i64 lda, double* B, i64 ldb, hipStream_t s);
+// F (n x n) := the symmetric matrix stored in A's lower triangle; or A's
+// lower triangle with zeros above.  A's strictly upper triangle is not read.
+hipError_t launch_symmetrize_lower_f64(int n, const double* A, i64 lda, double* F, i64 ldf, hipStream_t s);
+hipError_t launch_copy_lower_zero_upper_f64(int n, const double* A, i64 lda, double* F, i64 ldf, hipStream_t s);
 hipError_t launch_copy_lower_f64_to_f32(int n, const double* A, i64 lda, float* S, i64 lds, hipStream_t s);
 // columns j0 <= j < j1 only
 hipError_t launch_copy_lower_f64_to_f32_cols(int n, int j0, int j1, const double* A, i64 lda, float* S, i64 lds,

@@ -105,6 +105,11 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     const size_t plane_bytes = planes ? plane_region_bytes(n, nb, trailing) : 0;
     unsigned short* pbuf = nullptr;
     float* Xd = nullptr;
+    double* Xinv = nullptr;  // FP64: the diagonal block's inverse (128 x 128)
+    if (sizeof(T) == 8 && n > nb) {
+        Xinv = (double*)stream_scratch(128 * 128 * sizeof(double), s, kScratchFactor);
+        if (Xinv == nullptr) return kErrHip;
+    }
     if (sizeof(T) == 4) {
         char* scratch = scratch_in != nullptr ? (char*)scratch_in
                                               : (char*)stream_scratch(lowprec_scratch_bytes(n, nb, trailing), s);
@@ -146,8 +151,16 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
             }
             if constexpr (sizeof(T) == 8) {
                 ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
-                if (n - k - b > 0)
-                    ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
+                // the panel A21 := A21 inv(L11)^T as a product with the
+                // block's inverse (in place: one tile column, every
+                // workgroup reads its rows before it writes them)
+                if (n - k - b > 0) {
+                    ELEMAMD_HIP_TRY(launch_trtri_unb_f64(b, (const double*)Akk, lda, false, Xinv, 128, nullptr, 0,
+                                                         dinfo, s));
+                    ELEMAMD_HIP_TRY(launch_gemm_f64(false, true, n - k - b, b, b, 1.0, (const double*)Akk + b, lda,
+                                                    kGemmShapeFull, Xinv, 128, kGemmShapeLower, 0.0,
+                                                    (double*)Akk + b, lda, 0, dinfo, s));
+                }
             } else {  // the low-precision factor: diagonal inverse, panel as a product with it
                 ELEMAMD_HIP_TRY(launch_potrf_f32(b, (float*)Akk, lda, Xd, dinfo, k, s));
                 if (below_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_d, 0));
@@ -697,9 +710,35 @@ size_t mixed_refine_bytes(int n, int k, const MixedOpts& o) {
     return carve_refine(nullptr, 0, n, k, o.gmres_restart, o.refine == kRefineCG).bytes;
 }
 
+// The plain FP64 finish of a failed low-precision solve: an n x n copy of A
+// to factor, the original B, the residual, the solve's control words and the
+// residual's partial sums.
+struct FallbackWork {
+    double* Af;    // n x n, ld n
+    double* B0;    // n x k, ld n: the right-hand sides as given
+    double* R;     // n x k residual
+    double* nrm;   // 3k column maxima
+    double* part;  // symm_part_count(n, k)
+    void* solve;   // solve_work_bytes<double>(n, k)
+    size_t bytes;
+};
+static FallbackWork carve_fallback(void* mem, int n, int k) {
+    Carve c(mem, 0);
+    FallbackWork w;
+    const size_t nk = (size_t)n * k;
+    w.Af = c.take<double>((size_t)n * n);
+    w.B0 = c.take<double>(nk);
+    w.R = c.take<double>(nk);
+    w.nrm = c.take<double>((size_t)3 * k);
+    w.part = c.take<double>(symm_part_count(n, k));
+    w.solve = c.take<char>(solve_work_bytes<double>(n, k));
+    w.bytes = c.off + 256;
+    return w;
+}
+
 size_t mixed_workspace_bytes(int n, int k, const MixedOpts& o) {
     return align256(mixed_factor_bytes(n)) + align256(lowprec_scratch_bytes(n, o.nb, o.trailing)) +
-           mixed_refine_bytes(n, k, o);
+           align256(mixed_refine_bytes(n, k, o)) + (o.fallback ? carve_fallback(nullptr, n, k).bytes : 0);
 }
 
 // Low-precision factor of A: S = float(A) (lower), FP64 diagonal potrf,
@@ -807,6 +846,47 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     return kOk;
 }
 
+// The dispatch rule's plain FP64 route: Cholesky + SolveAfter on a copy of
+// A (A itself stays read-only), from the right-hand sides as given, and the
+// backward error of the result.  kErrNotConverged if A is not numerically
+// HPD in FP64 either (r.info = the failing column) or the result misses the
+// target (NaN or Inf in the input).
+static int fallback_fp64(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o,
+                         const MixedFactor& f, const FallbackWork& fw, MixedReport& r, hipStream_t s) {
+    r.fallback = 1;
+    r.converged = 0;
+    ELEMAMD_HIP_TRY(launch_copy_f64(n, n, A, lda, fw.Af, n, s));
+    int rc = cholesky_lower<double>(n, fw.Af, n, o.nb, kTrailingF64, f.dinfo, s);
+    if (rc) return rc;
+    int info = 0;
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(&info, f.dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
+    r.info = info;
+    if (info != 0) return kErrNotConverged;
+    ELEMAMD_HIP_TRY(launch_copy_f64(n, k, fw.B0, n, B, ldb, s));
+    rc = solve_after_lower<double>(n, k, fw.Af, n, B, ldb, nullptr, fw.solve, s);
+    if (rc) return rc;
+    rc = solve_read_error(solve_ctrl<double>(n, fw.solve), s);
+    if (rc) return rc;
+    // eta = max_c ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)
+    ELEMAMD_HIP_TRY(launch_symm_lower_residual(n, k, A, lda, B, ldb, fw.B0, n, fw.R, n, fw.part, s));
+    ELEMAMD_HIP_TRY(launch_col_absmax(n, k, fw.R, n, fw.nrm, s));
+    ELEMAMD_HIP_TRY(launch_col_absmax(n, k, B, ldb, fw.nrm + k, s));
+    ELEMAMD_HIP_TRY(launch_col_absmax(n, k, fw.B0, n, fw.nrm + 2 * k, s));
+    std::vector<double> h((size_t)3 * k);
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(h.data(), fw.nrm, sizeof(double) * 3 * k, hipMemcpyDeviceToHost, s));
+    ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
+    double eta = 0.0;
+    for (int c = 0; c < k; ++c) {
+        const double den = f.anorm * h[k + c] + h[2 * k + c];
+        const double e = den > 0.0 ? h[c] / den : (h[c] > 0.0 ? INFINITY : 0.0);
+        eta = std::max(eta, std::isnan(e) ? INFINITY : e);
+    }
+    r.backward_error = eta;
+    r.converged = eta <= o.tol ? 1 : 0;
+    return r.converged ? kOk : kErrNotConverged;
+}
+
 int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, const MixedOpts& o, void* work,
                     size_t work_bytes, MixedReport* rep, hipStream_t s) {
     if (n < 0 || k < 0 || lda < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
@@ -818,23 +898,33 @@ int hpd_solve_mixed(int n, int k, const double* A, i64 lda, double* B, i64 ldb, 
         }
         return kOk;
     }
-    // the caller's workspace: the factor, the factorization's scratch, the refinement's vectors
+    // the caller's workspace: the factor, the factorization's scratch, the
+    // refinement's vectors, the FP64 fallback's buffers
     const size_t fb = align256(mixed_factor_bytes(n));
     const size_t sb = align256(lowprec_scratch_bytes(n, o.nb, o.trailing));
+    const size_t rb = align256(mixed_refine_bytes(n, k, o));
     if (work == nullptr || work_bytes < mixed_workspace_bytes(n, k, o)) return kErrWorkspace;
     MixedFactor f = carve_factor(work, n);
+    FallbackWork fw{};
+    if (o.fallback) {
+        fw = carve_fallback((char*)work + fb + sb + rb, n, k);
+        ELEMAMD_HIP_TRY(launch_copy_f64(n, k, B, ldb, fw.B0, n, s));
+    }
+    MixedReport r{};
     int rc = mixed_factor(n, A, lda, o, f, s, (char*)work + fb);
     if (rc == kErrNotConverged) {
-        if (rep) {
-            *rep = MixedReport{};
-            rep->lowprec_info = f.lowprec_info;
-            rep->shifted = f.shifted;
-            rep->anorm_inf = f.anorm;
-        }
+        r.lowprec_info = f.lowprec_info;
+        r.shifted = f.shifted;
+        r.anorm_inf = f.anorm;
+    } else if (rc) {
         return rc;
+    } else {
+        rc = mixed_refine(n, k, A, lda, B, ldb, o, f, (char*)work + fb + sb, rb, &r, s);
+        if (rc != kOk && rc != kErrNotConverged) return rc;
     }
-    if (rc) return rc;
-    return mixed_refine(n, k, A, lda, B, ldb, o, f, (char*)work + fb + sb, work_bytes - fb - sb, rep, s);
+    if (rc == kErrNotConverged && o.fallback) rc = fallback_fp64(n, k, A, lda, B, ldb, o, f, fw, r, s);
+    if (rep) *rep = r;
+    return rc;
 }
 
 // FP64-residual refinement (classic, then GMRES-IR on stagnation) of A X = B
@@ -868,6 +958,7 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
     double prev = INFINITY;
     int stall = 0;
     double eta = INFINITY;
+    double gmres_prev = INFINITY;  // backward error before the last GMRES-IR cycle
     int it0 = 0, cg_conv = 0;
     if (o.refine == kRefineCG && !gmres) {
         int rc = cg_refine(c, o, bn, r, eta, &cg_conv);
@@ -898,6 +989,10 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
         }
         if (it > 0 && eta <= o.tol) break;
         if (it == o.max_iter) break;
+        // with a fallback, a GMRES-IR cycle that does not halve the backward
+        // error ends the refinement (the plain FP64 solve finishes the job)
+        if (o.fallback && gmres_prev < INFINITY && !(eta <= 0.5 * gmres_prev)) break;
+        gmres_prev = INFINITY;
         if (it > 1) {
             if (eta > 0.5 * prev) ++stall;
             else stall = 0;
@@ -911,6 +1006,7 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
             ELEMAMD_HIP_TRY(launch_axpy_f32_to_f64_scaled(n, k, w.Rf, n, w.nrm + k, w.X, n, s));
         } else {
             r.used_gmres = 1;
+            gmres_prev = eta;
             int gi = 0;
             rc = gmres_cycle(c, m, 1e-8, &gi);
             if (rc) return rc;

@@ -38,6 +38,23 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
                    void* scratch = nullptr);
 size_t lowprec_scratch_bytes(int n, int nb, int trailing);
 
+// hpd_inverse::CholeskyLVar2 / triangular_inverse::LVar3 on the lower
+// triangle (hpd_inverse.cpp); scratch: inverse_scratch_bytes(n, nb) bytes.
+size_t inverse_scratch_bytes(int n, int nb);
+int hpd_inverse_lower(int n, double* A, i64 lda, int nb, int* dinfo, void* scratch, hipStream_t s);
+int triangular_inverse_lower(int n, double* L, i64 ldl, bool unit, int nb, void* scratch, hipStream_t s);
+
+// elem::HermitianGenDefiniteEig on the lower triangles of A and B
+// (gen_eig.cpp).  type 1 AXBX, 2 ABX, 3 BAX; range 0 all, 1 indices
+// [il, iu] (0-based), 2 values in (vl, vu].  w: n doubles (device); X
+// (device, may be null): n x nev eigenvectors.  *nev: eigenvalues found;
+// *info: B's failing column (1-based, 0 = none; nothing else is done).
+// scratch: gen_eig_scratch_bytes(n).  Synchronous.
+size_t gen_eig_scratch_bytes(int n);
+int hermitian_gen_definite_eig_lower(int type, int n, double* A, i64 lda, double* B, i64 ldb, double* w, double* X,
+                                     i64 ldx, int range, int il, int iu, double vl, double vu, int* nev, int* info,
+                                     void* scratch, hipStream_t s);
+
 // cholesky::SolveAfter(LOWER, NORMAL): B := inv(L^T) inv(L) B.
 // `work` holds solve_work_bytes<T>(n, k) bytes of device memory.
 template <typename T>
@@ -71,6 +88,9 @@ struct MixedOpts {
     int allow_shift = 1;
     double tol = 1e-14;
     int refine = kRefineIR;
+    // finish in plain FP64 (Cholesky + SolveAfter on a copy of A) when the
+    // low-precision route fails
+    int fallback = 1;
 };
 
 struct MixedReport {
@@ -83,6 +103,7 @@ struct MixedReport {
     int converged;
     double backward_error;
     double anorm_inf;
+    int fallback;
 };
 
 // Low-precision factor kept for repeated solves (device memory of

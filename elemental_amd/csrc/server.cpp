@@ -114,6 +114,7 @@ size_t Server::work_bytes(int n, int k, bool factor) const {
     o.nb = cfg_.nb;
     o.trailing = cfg_.precision == 2 ? kTrailingBf16x3 : kTrailingF32;
     o.tol = cfg_.tol;
+    o.fallback = 0;
     // a request that factors also needs the factorization's plane scratch
     return al256(mixed_refine_bytes(n, k, o)) + (factor ? al256(lowprec_scratch_bytes(n, o.nb, o.trailing)) : 0);
 }
@@ -249,6 +250,7 @@ void Server::worker(int slot) {
     o.nb = cfg_.nb;
     o.trailing = cfg_.precision == 2 ? kTrailingBf16x3 : kTrailingF32;
     o.tol = cfg_.tol;
+    o.fallback = 0;  // a served request reports its own status
     for (;;) {
         std::shared_ptr<Req> r;
         std::shared_ptr<Factor> F;

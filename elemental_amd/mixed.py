@@ -20,13 +20,17 @@ class IROptions:
     max_iter: int = 30
     gmres_restart: int = 20
     force_gmres: bool = False
-    allow_shift: bool = True
+    allow_shift: bool = False
     tol: float = 1e-14
-    refine: str = "ir"  # "ir": classic refinement (GMRES-IR fallback), "cg": preconditioned CG
+    refine: str = "ir"  # "ir": classic refinement (GMRES-IR on stagnation), "cg": preconditioned CG
+    # finish in plain FP64 when the low-precision route fails (breakdown, a
+    # GMRES-IR cycle that does not halve the backward error, max_iter)
+    fallback: bool = True
 
     def c(self) -> IROpts:
         return IROpts(self.nb, TRAILING[self.trailing], self.max_iter, self.gmres_restart if self.gmres_restart else -1,
-                      int(self.force_gmres), int(self.allow_shift), self.tol, REFINE[self.refine])
+                      int(self.force_gmres), int(self.allow_shift), self.tol, REFINE[self.refine],
+                      int(self.fallback))
 
 
 @dataclasses.dataclass
@@ -40,6 +44,8 @@ class IRReport:
     converged: bool
     backward_error: float
     anorm_inf: float
+    fallback: bool = False
+    info: int = 0
 
 
 def workspace_bytes(n: int, k: int, opts: IROptions | None = None) -> int:
@@ -67,4 +73,5 @@ def HPDSolveMixed(A, B, opts: IROptions | None = None, workspace=None, stream=No
     if rc not in (0, -3):
         raise RuntimeError(f"elemamd_hpd_solve_mixed_f64 failed with status {rc}")
     return IRReport(rc, rep.lowprec_info, rep.iterations, rep.gmres_iterations, bool(rep.used_gmres),
-                    bool(rep.shifted), bool(rep.converged), rep.backward_error, rep.anorm_inf)
+                    bool(rep.shifted), bool(rep.converged), rep.backward_error, rep.anorm_inf,
+                    bool(rep.fallback), rep.info)

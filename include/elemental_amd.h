@@ -147,6 +147,56 @@ int elemamd_solve_after_f64(char uplo, char orientation, int n, int k, const dou
 int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double *A, int64_t lda, double *B,
                           int64_t ldb, int nb, int *dinfo, void *stream);
 
+/* C := alpha*op(A)*op(B) + beta*C, op = N or T ('C' == 'T'), op(A) m x k,
+ * op(B) k x n.  Replaces elem::Gemm(Orientation, Orientation, T, const
+ * Matrix<T>&, const Matrix<T>&, T, Matrix<T>&)
+ * (include/elemental/blas-like/level3/Gemm.hpp) -> blas::Gemm -> dgemm_.
+ * beta == 0: C is not read.  Asynchronous. */
+int elemamd_gemm_f64(char transa, char transb, int m, int n, int k, double alpha, const double *A, int64_t lda,
+                     const double *B, int64_t ldb, double beta, double *C, int64_t ldc, void *stream);
+
+/* A := A^{-1} for symmetric positive definite A, on the uplo triangle (the
+ * other triangle is untouched).  Replaces elem::HPDInverse(uplo, Matrix<F>& A)
+ * (include/elemental/lapack-like/Inverse.hpp:29-40) ->
+ * hpd_inverse::CholeskyLVar2 (Inverse/CholeskyLVar2.hpp:29-75); UPPER is the
+ * same algorithm on A^T.  Block size nb (1..128; the reference's default
+ * 128).  A non-positive pivot sets *dinfo to its 1-based column and stops
+ * (A is left as the reference leaves it when Cholesky(A11) throws).
+ * Asynchronous: read *dinfo after the stream. */
+int elemamd_hpd_inverse_f64(char uplo, int n, double *A, int64_t lda, int nb, int *dinfo, void *stream);
+
+/* A := A^{-1} for triangular A ('L' or 'U'), unit ('U') or non-unit ('N')
+ * diagonal; the other triangle (and a unit diagonal) is untouched.
+ * Replaces elem::TriangularInverse(uplo, diag, Matrix<F>& A)
+ * (include/elemental/lapack-like/TriangularInverse.hpp) ->
+ * triangular_inverse::LVar3 (TriangularInverse/LVar3.hpp:19-95).
+ * Asynchronous. */
+int elemamd_triangular_inverse_f64(char uplo, char diag, int n, double *A, int64_t lda, int nb, void *stream);
+
+/* Generalized-definite symmetric eigenproblem.  Replaces
+ * elem::HermitianGenDefiniteEig(type, uplo, Matrix<F>& A, Matrix<F>& B,
+ * Matrix<R>& w[, Matrix<F>& X][, a, b]) (include/elemental/lapack-like/
+ * HermitianGenDefiniteEig.hpp:22-126, 135-250): Cholesky(uplo, B), then
+ * A := inv(L) A inv(L)^T (type AXBX) or L^T A L (ABX, BAX), the standard
+ * eigenproblem of A, and, when X is not NULL, X := inv(L)^T Z (AXBX, ABX) or
+ * L Z (BAX).  Only the uplo triangles of A and B are read; on return B's
+ * holds the factor and A's the reduced matrix.  range: ELEMAMD_EIG_ALL,
+ * ELEMAMD_EIG_INDEX (eigenvalues il..iu, 0-based inclusive, ascending) or
+ * ELEMAMD_EIG_VALUE (eigenvalues in (vl, vu]).  w: device array of n doubles
+ * (the first *nev hold the eigenvalues, ascending); X: device n x *nev
+ * (ldx >= n).  *info (host) = B's failing column (1-based) when B is not
+ * numerically HPD (nothing else is done), else 0.  Synchronous; returns
+ * ELEMAMD_ERR_NOT_CONVERGED if the eigensolver does not converge. */
+#define ELEMAMD_AXBX 1
+#define ELEMAMD_ABX 2
+#define ELEMAMD_BAX 3
+#define ELEMAMD_EIG_ALL 0
+#define ELEMAMD_EIG_INDEX 1
+#define ELEMAMD_EIG_VALUE 2
+int elemamd_hermitian_gen_definite_eig_f64(int type, char uplo, int n, double *A, int64_t lda, double *B,
+                                           int64_t ldb, double *w, double *X, int64_t ldx, int range, int il, int iu,
+                                           double vl, double vu, int *nev, int *info, void *stream);
+
 /* The same three entry points for Matrix<float> (the reference templates
  * Cholesky<F>, cholesky::SolveAfter<F> and HPDSolve<F> on the field,
  * Cholesky.hpp:47-62, SolveAfter.hpp:20-75, HPDSolve.hpp:18-28): the float
@@ -168,11 +218,14 @@ typedef struct {
     int nb;               /* factorization block size, 1..128 (0 -> 128) */
     int trailing;         /* ELEMAMD_TRAILING_F32, _BF16X3 or _BF16X6 (_BF16X6: nb a multiple of 32) */
     int max_iter;         /* refinement steps before giving up (0 -> 30) */
-    int gmres_restart;    /* Krylov dimension of a GMRES-IR cycle: 0 -> 20, negative disables the fallback */
+    int gmres_restart;    /* Krylov dimension of a GMRES-IR cycle: 0 -> 20, negative disables GMRES-IR */
     int force_gmres;      /* 1: use GMRES-IR from the first correction */
     int allow_shift;      /* 1: on low-precision breakdown, retry the factor of A + sigma*diag(A) */
     double tol;           /* target normwise backward error (inf-norms); 0 -> 1e-14 */
     int refine;           /* ELEMAMD_REFINE_IR (classic) or ELEMAMD_REFINE_CG (preconditioned CG) */
+    int fallback;         /* 1: when the low-precision route fails (breakdown without a shift, a GMRES-IR
+                           * cycle that does not halve the backward error, or max_iter reached), solve with
+                           * the plain FP64 Cholesky + SolveAfter on a copy of A in the workspace */
 } elemamd_ir_opts;
 
 /* Refinement of the mixed solve.  IR: x += M^{-1} (b - A x) with the
@@ -185,7 +238,7 @@ typedef struct {
 #define ELEMAMD_REFINE_CG 1
 
 typedef struct {
-    int info;             /* reserved, 0 */
+    int info;             /* failing column of the FP64 fallback factor (0 = none) */
     int lowprec_info;     /* failing column of the last low-precision factor attempt (0 = none) */
     int iterations;       /* refinement steps (FP64 residuals computed) */
     int gmres_iterations; /* total inner GMRES iterations */
@@ -194,6 +247,7 @@ typedef struct {
     int converged;        /* 1 if backward_error <= tol */
     double backward_error;/* max over columns of ||b-Ax||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
     double anorm_inf;     /* ||A||_inf */
+    int fallback;         /* 1 if X came from the plain FP64 fallback solve */
 } elemamd_ir_report;
 
 /* Workspace bytes needed by elemamd_hpd_solve_mixed_f64. */
@@ -206,8 +260,12 @@ size_t elemamd_hpd_solve_mixed_workspace(int n, int k, const elemamd_ir_opts *op
  * the low-precision factor when classic refinement stagnates.  B is
  * overwritten with X.  Synchronous (it reads convergence norms each step).
  * `work` must hold elemamd_hpd_solve_mixed_workspace() bytes of device memory.
- * Returns ELEMAMD_OK, ELEMAMD_ERR_NOT_CONVERGED (X holds the best iterate),
- * or another error; *report (may be NULL) receives the details. */
+ * With opts->fallback the solve is finished in plain FP64 when the
+ * low-precision route fails (the workspace then includes an n x n FP64 copy
+ * of A).  Returns ELEMAMD_OK, ELEMAMD_ERR_NOT_CONVERGED (X holds the best
+ * iterate; with the fallback: A is not numerically HPD, report->info is the
+ * failing column, or the input holds NaN/Inf), or another error; *report
+ * (may be NULL) receives the details. */
 int elemamd_hpd_solve_mixed_f64(int n, int k, const double *A, int64_t lda, double *B, int64_t ldb,
                                 const elemamd_ir_opts *opts, void *work, size_t work_bytes,
                                 elemamd_ir_report *report, void *stream);

@@ -13,6 +13,10 @@
 //   elem::Trrk(...)                  src/blas-like/Trrk.cpp:19-38
 //   elem::Trsv(...)                  include/elemental/blas-like/level2/Trsv.hpp:20-44
 //   elem::Print / elem::Write        include/elemental/io/Print.hpp:15-35, io/Write.hpp:17-28
+//   elem::Gemm(...)                  include/elemental/blas-like/level3/Gemm.hpp
+//   elem::HPDInverse(uplo, A)        include/elemental/lapack-like/Inverse.hpp:29-40 (CholeskyLVar2)
+//   elem::TriangularInverse(...)     include/elemental/lapack-like/TriangularInverse.hpp (LVar3)
+//   elem::HermitianGenDefiniteEig    include/elemental/lapack-like/HermitianGenDefiniteEig.hpp:22-250
 //
 // A host Matrix is staged to device memory for the call and copied back, so
 // existing callers keep their buffers.  Callers that keep data on the GPU use
@@ -316,6 +320,101 @@ inline void Trsv(UpperOrLower uplo, Orientation orientation, UnitOrNonUnit diag,
                                    diag == NON_UNIT ? 'N' : 'U', A.Height(), dA.p, dA.ld, dx.p, 1, nullptr),
                   "elemamd_trsv_f64");
     dx.back(x);
+}
+
+// C := alpha op(A) op(B) + beta C.
+inline void Gemm(Orientation orientationOfA, Orientation orientationOfB, double alpha, const Matrix<double>& A,
+                 const Matrix<double>& B, double beta, Matrix<double>& C) {
+    const int k = orientationOfA == NORMAL ? A.Width() : A.Height();
+    const int kb = orientationOfB == NORMAL ? B.Height() : B.Width();
+    const int m = orientationOfA == NORMAL ? A.Height() : A.Width();
+    const int n = orientationOfB == NORMAL ? B.Width() : B.Height();
+    if (k != kb || m != C.Height() || n != C.Width()) throw std::logic_error("Nonconformal Gemm");
+    detail::Dev<double> dA(A), dB(B), dC(C);
+    detail::check(elemamd_gemm_f64(detail::orient_char(orientationOfA), detail::orient_char(orientationOfB), m, n, k,
+                                   alpha, dA.p, dA.ld, dB.p, dB.ld, beta, dC.p, dC.ld, nullptr),
+                  "elemamd_gemm_f64");
+    dC.back(C);
+}
+
+// The uplo triangle of A := inv(A) for HPD A; throws like Cholesky.
+inline void HPDInverse(UpperOrLower uplo, Matrix<double>& A, int nb = Blocksize()) {
+    if (A.Height() != A.Width()) throw std::logic_error("Nonsquare matrices cannot be triangular");
+    detail::Dev<double> d(A);
+    detail::DevInfo info;
+    detail::check(elemamd_hpd_inverse_f64(detail::uplo_char(uplo), A.Height(), d.p, d.ld, nb, info.p, nullptr),
+                  "elemamd_hpd_inverse_f64");
+    d.back(A);
+    info.raise_if_failed();
+}
+
+inline void TriangularInverse(UpperOrLower uplo, UnitOrNonUnit diag, Matrix<double>& A, int nb = Blocksize()) {
+    if (A.Height() != A.Width()) throw std::logic_error("Nonsquare matrices cannot be triangular");
+    detail::Dev<double> d(A);
+    detail::check(elemamd_triangular_inverse_f64(detail::uplo_char(uplo), diag == NON_UNIT ? 'N' : 'U', A.Height(),
+                                                 d.p, d.ld, nb, nullptr),
+                  "elemamd_triangular_inverse_f64");
+    d.back(A);
+}
+
+// lapack-like_decl.hpp: hermitian_gen_definite_eig_type_wrapper
+enum HermitianGenDefiniteEigType { AXBX = 1, ABX = 2, BAX = 3 };
+
+namespace detail {
+// w (nev x 1) and, when X is not null, X (n x nev); range as in the C ABI.
+inline void gen_eig(HermitianGenDefiniteEigType type, UpperOrLower uplo, Matrix<double>& A, Matrix<double>& B,
+                    Matrix<double>& w, Matrix<double>* X, int range, int il, int iu, double vl, double vu) {
+    const int n = A.Height();
+    if (A.Width() != n || B.Height() != B.Width() || B.Height() != n)
+        throw std::logic_error("Hermitian matrices must be square.");
+    Dev<double> dA(A), dB(B);
+    Matrix<double> wfull(n > 0 ? n : 1, 1), Xfull(n, n > 0 ? n : 1);
+    Dev<double> dw(wfull), dX(Xfull);
+    int nev = 0, info = 0;
+    check(elemamd_hermitian_gen_definite_eig_f64((int)type, uplo_char(uplo), n, dA.p, dA.ld, dB.p, dB.ld, dw.p,
+                                                 X ? dX.p : nullptr, dX.ld, range, il, iu, vl, vu, &nev, &info,
+                                                 nullptr),
+          "elemamd_hermitian_gen_definite_eig_f64");
+    if (info > 0) throw std::logic_error("A was not numerically HPD");
+    dA.back(A);
+    dB.back(B);
+    dw.back(wfull);
+    w = Matrix<double>(nev, 1);
+    for (int i = 0; i < nev; ++i) w.Set(i, 0, wfull.Get(i, 0));
+    if (X) {
+        dX.back(Xfull);
+        *X = Matrix<double>(n, nev);
+        for (int j = 0; j < nev; ++j)
+            for (int i = 0; i < n; ++i) X->Set(i, j, Xfull.Get(i, j));
+    }
+}
+}  // namespace detail
+
+// All eigenvalues, then all eigenpairs (HermitianGenDefiniteEig.hpp:22-126).
+inline void HermitianGenDefiniteEig(HermitianGenDefiniteEigType type, UpperOrLower uplo, Matrix<double>& A,
+                                    Matrix<double>& B, Matrix<double>& w) {
+    detail::gen_eig(type, uplo, A, B, w, nullptr, ELEMAMD_EIG_ALL, 0, 0, 0.0, 0.0);
+}
+inline void HermitianGenDefiniteEig(HermitianGenDefiniteEigType type, UpperOrLower uplo, Matrix<double>& A,
+                                    Matrix<double>& B, Matrix<double>& w, Matrix<double>& X) {
+    detail::gen_eig(type, uplo, A, B, w, &X, ELEMAMD_EIG_ALL, 0, 0, 0.0, 0.0);
+}
+// Eigenvalues a..b (0-based, inclusive) and the eigenvalues in (a, b].
+inline void HermitianGenDefiniteEig(HermitianGenDefiniteEigType type, UpperOrLower uplo, Matrix<double>& A,
+                                    Matrix<double>& B, Matrix<double>& w, int a, int b) {
+    detail::gen_eig(type, uplo, A, B, w, nullptr, ELEMAMD_EIG_INDEX, a, b, 0.0, 0.0);
+}
+inline void HermitianGenDefiniteEig(HermitianGenDefiniteEigType type, UpperOrLower uplo, Matrix<double>& A,
+                                    Matrix<double>& B, Matrix<double>& w, Matrix<double>& X, int a, int b) {
+    detail::gen_eig(type, uplo, A, B, w, &X, ELEMAMD_EIG_INDEX, a, b, 0.0, 0.0);
+}
+inline void HermitianGenDefiniteEig(HermitianGenDefiniteEigType type, UpperOrLower uplo, Matrix<double>& A,
+                                    Matrix<double>& B, Matrix<double>& w, double a, double b) {
+    detail::gen_eig(type, uplo, A, B, w, nullptr, ELEMAMD_EIG_VALUE, 0, 0, a, b);
+}
+inline void HermitianGenDefiniteEig(HermitianGenDefiniteEigType type, UpperOrLower uplo, Matrix<double>& A,
+                                    Matrix<double>& B, Matrix<double>& w, Matrix<double>& X, double a, double b) {
+    detail::gen_eig(type, uplo, A, B, w, &X, ELEMAMD_EIG_VALUE, 0, 0, a, b);
 }
 
 }  // namespace elem

@@ -636,3 +636,150 @@ void eo_sym_lower_residual(int n, int k, const double *A, int lda, const double 
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* Triangular inverse and HPD inverse: the callers of Cholesky that invert. */
+
+/* triangular_inverse::LVar3Unb (include/elemental/lapack-like/
+ * TriangularInverse/LVar3.hpp:19-49) on the lower triangle: per column j,
+ * row j left of the diagonal /= -lambda, the rank-1 Geru update of the block
+ * below-left of it (Netlib dger order: column by column, temp = 1*y[k],
+ * columns with y[k] == 0 skipped), column j below the diagonal /= lambda and
+ * the pivot inverted.  unit: lambda = 1 and the diagonal is left alone. */
+void eo_trtri_lvar3unb(int unit, int n, double *L, int ldl) {
+    for (int j = 0; j < n; ++j) {
+        const double lambda = unit ? 1.0 : AT(L, j, j, ldl);
+        for (int k = 0; k < j; ++k) AT(L, j, k, ldl) /= -lambda;
+        for (int k = 0; k < j; ++k) {
+            const double temp = 1.0 * AT(L, j, k, ldl);
+            if (temp != 0.0)
+                for (int i = j + 1; i < n; ++i) AT(L, i, k, ldl) += AT(L, i, j, ldl) * temp;
+        }
+        if (!unit) {
+            for (int k = j + 1; k < n; ++k) AT(L, k, j, ldl) /= lambda;
+            AT(L, j, j, ldl) = 1.0 / AT(L, j, j, ldl);
+        }
+    }
+}
+
+/* triangular_inverse::LVar3 (TriangularInverse/LVar3.hpp:52-95), lower: per
+ * diagonal block, Trsm(LEFT, LOWER, NORMAL, diag, -1, L11, L10),
+ * Gemm(NORMAL, NORMAL, 1, L21, L10, 1, L20), Trsm(RIGHT, LOWER, NORMAL,
+ * diag, 1, L11, L21), LVar3Unb(diag, L11). */
+void eo_trtri_lvar3(int unit, int n, double *L, int ldl, int nb) {
+    const char diag = unit ? 'U' : 'N';
+    if (nb < 1) nb = 1;
+    for (int k = 0; k < n; k += nb) {
+        const int b = (n - k < nb) ? n - k : nb, m = n - k - b;
+        double *L10 = &AT(L, k, 0, ldl), *L11 = &AT(L, k, k, ldl);
+        double *L20 = &AT(L, k + b, 0, ldl), *L21 = &AT(L, k + b, k, ldl);
+        if (k > 0) eo_dtrsm('L', 'L', 'N', diag, b, k, -1.0, L11, ldl, L10, ldl);
+        if (m > 0 && k > 0) eo_dgemm('N', 'N', m, k, b, 1.0, L21, ldl, L10, ldl, 1.0, L20, ldl);
+        if (m > 0) eo_dtrsm('R', 'L', 'N', diag, m, b, 1.0, L11, ldl, L21, ldl);
+        eo_trtri_lvar3unb(unit, b, L11, ldl);
+    }
+}
+
+/* internal::TrtrmmLUnblocked(TRANSPOSE) (include/elemental/blas-like/level3/
+ * Trtrmm/Unblocked.hpp:15-57): the lower triangle of L := L^T L. */
+void eo_trtrmm_lunb(int n, double *L, int ldl) {
+    for (int j = 0; j < n; ++j) {
+        for (int k = 0; k < j; ++k) {
+            const double gamma = AT(L, j, k, ldl);
+            for (int i = k; i < j; ++i) AT(L, i, k, ldl) += AT(L, j, i, ldl) * gamma;
+        }
+        const double lambda = AT(L, j, j, ldl);
+        for (int k = 0; k < j; ++k) AT(L, j, k, ldl) *= lambda;
+        AT(L, j, j, ldl) = lambda * lambda;
+    }
+}
+
+/* hpd_inverse::CholeskyLVar2 (include/elemental/lapack-like/Inverse/
+ * CholeskyLVar2.hpp:29-75) on the lower triangle with block size nb.  The
+ * calls on the diagonal block use the reference's global Blocksize() (128),
+ * so for nb <= 128 they are single-block: Cholesky -> LVar3Unb,
+ * TriangularInverse -> LVar3Unb, Trtrmm -> TrtrmmLUnblocked.  Returns 0, or
+ * the global failing column + 1 (the reference throws there). */
+int eo_hpd_inverse_lvar2(int n, double *A, int lda, int nb) {
+    if (nb < 1) nb = 1;
+    for (int k = 0; k < n; k += nb) {
+        const int b = (n - k < nb) ? n - k : nb, m = n - k - b;
+        double *A00 = A, *A10 = &AT(A, k, 0, lda), *A11 = &AT(A, k, k, lda);
+        double *A20 = &AT(A, k + b, 0, lda), *A21 = &AT(A, k + b, k, lda), *A22 = &AT(A, k + b, k + b, lda);
+        const int info = eo_lvar3(b, A11, lda, 128);
+        if (info) return k + info;
+        if (k > 0) eo_dtrsm('L', 'L', 'N', 'N', b, k, 1.0, A11, lda, A10, lda);
+        if (m > 0) eo_dtrsm('R', 'L', 'T', 'N', m, b, 1.0, A11, lda, A21, lda);
+        if (k > 0) eo_dsyrk('L', 'T', k, b, 1.0, A10, lda, 1.0, A00, lda);
+        if (k > 0 && m > 0) eo_dgemm('N', 'N', m, k, b, -1.0, A21, lda, A10, lda, 1.0, A20, lda);
+        if (m > 0) eo_dsyrk('L', 'N', m, b, -1.0, A21, lda, 1.0, A22, lda);
+        if (k > 0) eo_dtrsm('L', 'L', 'T', 'N', b, k, 1.0, A11, lda, A10, lda);
+        if (m > 0) eo_dtrsm('R', 'L', 'N', 'N', m, b, -1.0, A11, lda, A21, lda);
+        eo_trtri_lvar3(0, b, A11, lda, 128);
+        eo_trtrmm_lunb(b, A11, lda);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Two-sided reductions of the generalized-definite eigensolver.        */
+
+/* internal::TwoSidedTrsmLUnb (include/elemental/blas-like/level3/
+ * TwoSidedTrsm.hpp:14-60), variant 4, non-unit: the lower triangle of
+ * A := inv(L) A inv(L)^T, with the reference BLAS orders of Geru (column by
+ * column, temp = alpha*y[k], zero y skipped) and Her2 (dsyr2 lower: column
+ * by column, temp1 = alpha*y[c], temp2 = alpha*x[c]). */
+void eo_two_sided_trsm_lunb(int n, double *A, int lda, const double *L, int ldl) {
+    for (int j = 0; j < n; ++j) {
+        const int h = n - (j + 1);
+        const double lambda11 = AT(L, j, j, ldl);
+        for (int k = 0; k < j; ++k) AT(A, j, k, lda) /= lambda11;
+        for (int k = 0; k < j; ++k) { /* A20 -= l21 a10 */
+            const double temp = -1.0 * AT(A, j, k, lda);
+            if (temp != 0.0)
+                for (int i = 0; i < h; ++i) AT(A, j + 1 + i, k, lda) += AT(L, j + 1 + i, j, ldl) * temp;
+        }
+        AT(A, j, j, lda) /= lambda11 * lambda11;
+        const double alpha11 = AT(A, j, j, lda);
+        for (int k = 0; k < h; ++k) AT(A, j + 1 + k, j, lda) /= lambda11;
+        for (int k = 0; k < h; ++k) AT(A, j + 1 + k, j, lda) -= (alpha11 / 2) * AT(L, j + 1 + k, j, ldl);
+        for (int c = 0; c < h; ++c) { /* A22 -= l21 a21^T + a21 l21^T (lower) */
+            const double xc = AT(L, j + 1 + c, j, ldl), yc = AT(A, j + 1 + c, j, lda);
+            if (xc != 0.0 || yc != 0.0) {
+                const double temp1 = -1.0 * yc, temp2 = -1.0 * xc;
+                for (int i = c; i < h; ++i)
+                    AT(A, j + 1 + i, j + 1 + c, lda) +=
+                        AT(L, j + 1 + i, j, ldl) * temp1 + AT(A, j + 1 + i, j, lda) * temp2;
+            }
+        }
+        for (int k = 0; k < h; ++k) AT(A, j + 1 + k, j, lda) -= (alpha11 / 2) * AT(L, j + 1 + k, j, ldl);
+    }
+}
+
+/* internal::TwoSidedTrmmLUnb (include/elemental/blas-like/level3/
+ * TwoSidedTrmm.hpp:14-64), variant 4, non-unit: the lower triangle of
+ * A := L^T A L. */
+void eo_two_sided_trmm_lunb(int n, double *A, int lda, const double *L, int ldl) {
+    for (int j = 0; j < n; ++j) {
+        const int h = n - (j + 1);
+        const double alpha11 = AT(A, j, j, lda);
+        const double lambda11 = AT(L, j, j, ldl);
+        for (int k = 0; k < j; ++k) AT(A, j, k, lda) += (alpha11 / 2) * AT(L, j, k, ldl);
+        for (int c = 0; c < j; ++c) { /* A00 += a10^T l10 + l10^T a10 (lower), x = a10, y = l10 */
+            const double xc = AT(A, j, c, lda), yc = AT(L, j, c, ldl);
+            if (xc != 0.0 || yc != 0.0) {
+                const double temp1 = 1.0 * yc, temp2 = 1.0 * xc;
+                for (int i = c; i < j; ++i) AT(A, i, c, lda) += AT(A, j, i, lda) * temp1 + AT(L, j, i, ldl) * temp2;
+            }
+        }
+        for (int k = 0; k < j; ++k) AT(A, j, k, lda) += (alpha11 / 2) * AT(L, j, k, ldl);
+        for (int k = 0; k < j; ++k) AT(A, j, k, lda) *= lambda11;
+        AT(A, j, j, lda) *= lambda11 * lambda11;
+        for (int k = 0; k < j; ++k) { /* A20 += a21 l10 */
+            const double temp = 1.0 * AT(L, j, k, ldl);
+            if (temp != 0.0)
+                for (int i = 0; i < h; ++i) AT(A, j + 1 + i, k, lda) += AT(A, j + 1 + i, j, lda) * temp;
+        }
+        for (int k = 0; k < h; ++k) AT(A, j + 1 + k, j, lda) *= lambda11;
+    }
+}

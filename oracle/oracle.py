@@ -72,6 +72,13 @@ def lib():
         L.eo_solve_after_upper.argtypes = [i, i, vp, i, vp, i]
         L.eo_hpd_solve_upper.argtypes = [i, i, vp, i, vp, i, i]
         L.eo_hpd_solve_upper.restype = i
+        L.eo_trtri_lvar3unb.argtypes = [i, i, vp, i]
+        L.eo_trtri_lvar3.argtypes = [i, i, vp, i, i]
+        L.eo_trtrmm_lunb.argtypes = [i, vp, i]
+        L.eo_hpd_inverse_lvar2.argtypes = [i, vp, i, i]
+        L.eo_hpd_inverse_lvar2.restype = i
+        L.eo_two_sided_trsm_lunb.argtypes = [i, vp, i, vp, i]
+        L.eo_two_sided_trmm_lunb.argtypes = [i, vp, i, vp, i]
         L.eo_bind_blas.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i]
         L.eo_bind_blas.restype = i
         L.eo_blas_bound.restype = i
@@ -190,6 +197,77 @@ def hpd_solve_upper(A, B, nb: int = 128) -> int:
     _f(A), _f(B)
     k = B.shape[1] if B.ndim == 2 else 1
     return lib().eo_hpd_solve_upper(A.shape[0], k, _p(A), _ld(A), _p(B), _ld(B), nb)
+
+
+def trtri_lvar3unb(L, unit: bool = False):
+    """triangular_inverse::LVar3Unb on the lower triangle, in place."""
+    _f(L)
+    lib().eo_trtri_lvar3unb(int(unit), L.shape[0], _p(L), _ld(L))
+    return L
+
+
+def trtri(L, unit: bool = False, nb: int = 128):
+    """triangular_inverse::LVar3 on the lower triangle, in place."""
+    _f(L)
+    lib().eo_trtri_lvar3(int(unit), L.shape[0], _p(L), _ld(L), nb)
+    return L
+
+
+def trtrmm_lunb(L):
+    """TrtrmmLUnblocked(TRANSPOSE): lower triangle of L := L^T L, in place."""
+    _f(L)
+    lib().eo_trtrmm_lunb(L.shape[0], _p(L), _ld(L))
+    return L
+
+
+def hpd_inverse(A, nb: int = 128) -> int:
+    """hpd_inverse::CholeskyLVar2: lower triangle of A := inv(A), in place;
+    returns 0 or the 1-based failing column."""
+    _f(A)
+    return lib().eo_hpd_inverse_lvar2(A.shape[0], _p(A), _ld(A), nb)
+
+
+def two_sided_trsm(A, L):
+    """TwoSidedTrsmLUnb: lower triangle of A := inv(L) A inv(L)^T, in place."""
+    _f(A), _f(L)
+    lib().eo_two_sided_trsm_lunb(A.shape[0], _p(A), _ld(A), _p(L), _ld(L))
+    return A
+
+
+def two_sided_trmm(A, L):
+    """TwoSidedTrmmLUnb: lower triangle of A := L^T A L, in place."""
+    _f(A), _f(L)
+    lib().eo_two_sided_trmm_lunb(A.shape[0], _p(A), _ld(A), _p(L), _ld(L))
+    return A
+
+
+def hermitian_gen_definite_eig(gtype: int, A, B, nb: int = 128):
+    """elem::HermitianGenDefiniteEig(type, LOWER, A, B, w, X)
+    (HermitianGenDefiniteEig.hpp:60-86) restated: Cholesky(LOWER, B) (LVar3),
+    TwoSidedTrsm (type 1) or TwoSidedTrmm (types 2, 3) in their unblocked
+    variant-4 loops, the standard eigenproblem of the reduced lower triangle,
+    and the back-transformation (Trsm with L^T, or L Z for type 3).  The
+    standard eigenproblem is LAPACK's (numpy.linalg.eigh, dsyevd); the
+    reference calls LAPACK's dsyevr there (src/lapack-like/HermitianEig.cpp:
+    224-240), which agrees to rounding.  Returns (info, w, X, reduced A)."""
+    A = np.array(A, dtype=np.float64, order="F")
+    L = np.array(B, dtype=np.float64, order="F")
+    info = lvar3(L, nb)
+    if info:
+        return info, None, None, A
+    if gtype == 1:
+        two_sided_trsm(A, L)
+    else:
+        two_sided_trmm(A, L)
+    S = np.tril(A) + np.tril(A, -1).T
+    w, Z = np.linalg.eigh(S)
+    Z = np.asfortranarray(Z)
+    Lt = np.tril(L)
+    if gtype == 3:
+        X = Lt @ Z
+    else:
+        X = dtrsm("L", "T", 1.0, np.asfortranarray(Lt), Z)
+    return 0, w, X, A
 
 
 def scale_trapezoid_lower(alpha, A):

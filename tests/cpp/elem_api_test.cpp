@@ -152,6 +152,50 @@ int main() {
         fails += !(relf < 1e-5);
     }
 
+    // HPDInverse: inv(A) on the lower triangle, the upper one untouched
+    {
+        const int m = 300;
+        elem::Matrix<double> H(m, m), H0;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) H.Set(i, j, A0.Get(i, j));
+        H0 = H;
+        elem::HPDInverse(elem::LOWER, H);
+        double worst = 0;
+        bool upper_ok = true;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) {
+                double s = 0;  // (inv(A) A)(i, j) from the lower triangle of inv(A)
+                for (int l = 0; l < m; ++l) s += (i >= l ? H.Get(i, l) : H.Get(l, i)) * H0.Get(l, j);
+                worst = std::fmax(worst, std::fabs(s - (i == j ? 1.0 : 0.0)));
+                if (i < j && H.Get(i, j) != H0.Get(i, j)) upper_ok = false;
+            }
+        std::printf("HPDInverse n=%d: max |inv(A) A - I| = %.3e, upper untouched %d\n", m, worst, (int)upper_ok);
+        if (!(worst < 1e-10) || !upper_ok) ++fails;
+    }
+    // HermitianGenDefiniteEig(AXBX): A x = lambda B x for every pair
+    {
+        const int m = 120;
+        elem::Matrix<double> S(m, m), P(m, m), w, X;
+        for (int j = 0; j < m; ++j)
+            for (int i = 0; i < m; ++i) {
+                S.Set(i, j, G.Get(i, j) + G.Get(j, i));
+                P.Set(i, j, A0.Get(i, j));
+            }
+        elem::Matrix<double> S0(S), P0(P);
+        elem::HermitianGenDefiniteEig(elem::AXBX, elem::LOWER, S, P, w, X);
+        double worst = 0;
+        for (int c = 0; c < w.Height(); ++c)
+            for (int i = 0; i < m; ++i) {
+                double ax = 0, bx = 0;
+                for (int l = 0; l < m; ++l) {
+                    ax += S0.Get(i, l) * X.Get(l, c);
+                    bx += P0.Get(i, l) * X.Get(l, c);
+                }
+                worst = std::fmax(worst, std::fabs(ax - w.Get(c, 0) * bx));
+            }
+        std::printf("HermitianGenDefiniteEig n=%d: %d eigenpairs, max |Ax - wBx| = %.3e\n", m, w.Height(), worst);
+        if (w.Height() != m || !(worst < 1e-9)) ++fails;
+    }
     std::printf("%s\n", fails ? "FAILED" : "ALL OK");
     return fails ? 1 : 0;
 }

@@ -216,20 +216,47 @@ def test_precision_sweep_n32768():
 
 
 # ------------------------------------------------- low-precision breakdown
-def test_mixed_shifted_refactor_with_gmres(oracle):
-    """kappa = 1e9 breaks the float factor of A (a non-positive float
-    pivot); the solver refactors A + sigma diag(A) and GMRES-IR still reaches
-    FP64 backward error <= 1e-14.  n = 2048 so the sweep preparation runs
-    beside the factorization on the third stream."""
+def test_mixed_breakdown_falls_back_to_fp64(oracle):
+    """kappa = 1e9 breaks the split-BF16x3 factor of A (a non-positive float
+    pivot).  The dispatch rule finishes in plain FP64 (Cholesky + SolveAfter
+    on a copy of A): FP64 backward error, X as the oracle's FP64 HPDSolve, A
+    untouched.  n = 2048 so the sweep preparation runs beside the
+    factorization on the third stream."""
     n, k = 2048, 4
     A = inputs.spectrum_host(n, 1e9, seed=71)
     B = inputs.normal_host(n, k, seed=72)
     Ad, Bd = dev(A), dev(B)
-    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing="bf16x3", refine="ir", max_iter=30))
-    assert rep.shifted and rep.lowprec_info == 0, rep
-    assert rep.converged and rep.used_gmres, rep
-    assert eta_inf_host(A, host(Bd), B) <= 1e-14
+    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing="bf16x3", refine="ir"))
+    assert rep.lowprec_info > 0 and not rep.shifted, rep
+    assert rep.fallback and rep.converged and rep.status == 0 and rep.info == 0, rep
+    X = host(Bd)
+    eta = eta_inf_host(A, X, B)
+    assert eta <= 1e-14 and rep.backward_error <= 1e-14, (eta, rep)  # both at the rounding level
+    Lr, Xr = A.copy(order="F"), B.copy(order="F")
+    assert oracle.hpd_solve(Lr, Xr, 128) == 0
+    # both are FP64 backward-stable solves of a kappa = 1e9 system
+    assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= 1e9 * 100 * EPS
     assert np.array_equal(host(Ad), A)
+
+
+def test_mixed_shifted_refactor_runs_then_falls_back():
+    """allow_shift: after the breakdown the factor of A + sigma diag(A)
+    succeeds and GMRES-IR preconditioned by it makes progress; with the
+    fallback switched off the call stops short of the target (the float
+    preconditioner's rounding keeps GMRES-IR near 1e-8 at this kappa), with
+    it on the FP64 route reaches it."""
+    n, k = 2048, 4
+    A = inputs.spectrum_host(n, 1e9, seed=71)
+    B = inputs.normal_host(n, k, seed=72)
+    Ad, Bd = dev(A), dev(B)
+    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing="bf16x3", refine="ir", max_iter=8, allow_shift=True,
+                                              fallback=False))
+    assert rep.shifted and rep.lowprec_info == 0 and rep.used_gmres and not rep.fallback, rep
+    assert rep.status in (0, -3) and rep.backward_error <= 1e-6, rep
+    Bd = dev(B)
+    rep = E.HPDSolveMixed(Ad, Bd, E.IROptions(trailing="bf16x3", refine="ir", allow_shift=True))
+    assert rep.shifted and rep.converged, rep
+    assert eta_inf_host(A, host(Bd), B) <= 1e-14
 
 
 def test_mixed_non_hpd_without_shift_reports_column():
@@ -242,6 +269,10 @@ def test_mixed_non_hpd_without_shift_reports_column():
     assert rep.status == -3 and not rep.converged, rep
     assert rep.lowprec_info == col + 1, rep
     assert not rep.shifted
+    # the FP64 fallback meets the same column: A is not numerically HPD
+    assert rep.fallback and rep.info == col + 1, rep
+    rep = E.HPDSolveMixed(dev(A), dev(B), E.IROptions(trailing="bf16x3", fallback=False))
+    assert rep.status == -3 and rep.lowprec_info == col + 1 and not rep.fallback and rep.info == 0, rep
 
 
 @pytest.mark.parametrize("where", ["b_nan", "a_inf"])

@@ -351,3 +351,172 @@ def test_upper_blas_restatements_vs_host_blas(oracle):
         assert np.array_equal(np.tril(C1, -1), np.tril(C, -1))
         C2 = sb.dsyrk(-0.75, A, beta=2.0, c=C.copy(order="F"), trans=0 if trans == "N" else 1, lower=0)
         assert np.allclose(np.triu(C1), np.triu(C2), atol=1e-12)
+
+
+# ------------------------------------------------ inverses (CholeskyLVar2)
+def _bidiag_unit(n, seed):
+    """Unit lower bidiagonal L0 with subdiagonal in {-1, 0, 1}: inv(L0) has
+    entries in {-1, 0, 1} and A = L0 L0^T is an integer matrix whose inverse
+    inv(L0)^T inv(L0) is an integer matrix too (entries <= n), so every
+    intermediate of the inverse algorithms is an exact integer."""
+    rng = np.random.default_rng(seed)
+    L0 = np.eye(n)
+    L0[np.arange(1, n), np.arange(n - 1)] = rng.integers(-1, 2, n - 1)
+    return L0
+
+
+def _exact_inverse_of_bidiag(L0):
+    n = L0.shape[0]
+    X = np.zeros((n, n), dtype=np.int64)  # inv(L0) by forward substitution in integers
+    s = np.rint(np.diag(L0, -1)).astype(np.int64)
+    for k in range(n):
+        X[k, k] = 1
+        for i in range(k + 1, n):
+            X[i, k] = -s[i - 1] * X[i - 1, k]
+    return X
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 64, 128])
+def test_trtri_unblocked_vs_substitution_and_unit(oracle, n):
+    rng = np.random.default_rng(n)
+    L = np.tril(rng.standard_normal((n, n))) + n * np.eye(n)
+    X = oracle.trtri_lvar3unb(L.copy(order="F"))
+    ref = sl.solve_triangular(L, np.eye(n), lower=True)
+    assert np.array_equal(np.triu(X, 1), np.triu(L, 1))  # strictly upper untouched
+    assert np.max(np.abs(np.tril(X) - ref)) <= 1e-14 * max(1.0, np.max(np.abs(ref)))
+    # unit diagonal: the stored diagonal is neither read nor written
+    Lu = L.copy(order="F")
+    Lu[np.diag_indices(n)] = 7.5
+    Xu = oracle.trtri_lvar3unb(Lu, unit=True)
+    Lunit = np.tril(L, -1) + np.eye(n)
+    refu = sl.solve_triangular(Lunit, np.eye(n), lower=True)
+    assert np.all(np.diag(Xu) == 7.5)
+    assert np.max(np.abs(np.tril(Xu, -1) - np.tril(refu, -1))) <= 1e-13 * max(1.0, np.max(np.abs(refu)))
+
+
+@pytest.mark.parametrize("n,nb", [(1, 128), (5, 2), (200, 37), (300, 64), (256, 128)])
+def test_trtri_blocked_integer_exact_and_vs_unblocked(oracle, n, nb):
+    L0 = _bidiag_unit(n, seed=n + nb)
+    X = oracle.trtri(L0.copy(order="F"), nb=nb)
+    assert np.array_equal(np.tril(X), _exact_inverse_of_bidiag(L0).astype(float))
+    rng = np.random.default_rng(nb)
+    L = np.tril(rng.standard_normal((n, n))) + n * np.eye(n)
+    Xb = oracle.trtri(L.copy(order="F"), nb=nb)
+    ref = sl.solve_triangular(L, np.eye(n), lower=True)
+    assert np.max(np.abs(np.tril(Xb) - ref)) <= 1e-13 * max(1.0, np.max(np.abs(ref)))
+    if nb >= n:  # one block: exactly the unblocked loop
+        assert np.array_equal(Xb, oracle.trtri_lvar3unb(L.copy(order="F")))
+
+
+@pytest.mark.parametrize("n", [1, 3, 50, 128])
+def test_trtrmm_integer_exact(oracle, n):
+    rng = np.random.default_rng(3 * n)
+    X = np.tril(rng.integers(-4, 5, (n, n))).astype(float)
+    C = oracle.trtrmm_lunb(X.copy(order="F"))
+    assert np.array_equal(np.tril(C), np.tril(X.T @ X))  # exact: small integers
+    assert np.array_equal(np.triu(C, 1), np.triu(X, 1))
+
+
+@pytest.mark.parametrize("n,nb", [(1, 128), (2, 1), (100, 128), (300, 64), (500, 37), (700, 128)])
+def test_hpd_inverse_vs_numpy_inverse(oracle, n, nb):
+    A = inputs.gram_host(n, seed=5 * n + nb)
+    Ai = A.copy(order="F")
+    assert oracle.hpd_inverse(Ai, nb) == 0
+    ref = np.linalg.inv(A)
+    kappa = np.linalg.cond(A)
+    err = np.linalg.norm(np.tril(Ai - ref)) / np.linalg.norm(np.tril(ref))
+    assert err <= 10 * kappa * n * np.finfo(float).eps, (err, kappa)
+    assert np.array_equal(np.triu(Ai, 1), np.triu(A, 1))  # strictly upper untouched
+
+
+@pytest.mark.parametrize("n,nb", [(60, 128), (300, 64), (513, 128)])
+def test_hpd_inverse_integer_exact(oracle, n, nb):
+    L0 = _bidiag_unit(n, seed=n)
+    A = L0 @ L0.T
+    Ai = A.copy(order="F")
+    assert oracle.hpd_inverse(Ai, nb) == 0
+    X = _exact_inverse_of_bidiag(L0)
+    assert np.array_equal(np.tril(Ai), np.tril(X.T @ X).astype(float))
+
+
+@pytest.mark.parametrize("col", [0, 63, 64, 200])
+def test_hpd_inverse_non_hpd_column_and_state(oracle, col):
+    n, nb = 300, 64
+    A = inputs.gram_host(n, seed=17)
+    A[col, col] = -50.0
+    Ai = A.copy(order="F")
+    assert oracle.hpd_inverse(Ai, nb) == col + 1
+    # the steps before the failing block are those of the matrix's leading part
+    k0 = (col // nb) * nb
+    if k0 > 0:
+        lead = A[:k0, :k0].copy(order="F")
+        assert oracle.hpd_inverse(lead, nb) == 0
+        assert np.array_equal(np.tril(Ai[:k0, :k0]), np.tril(lead))
+
+
+# -------------------------------------- generalized-definite eigenproblem
+@pytest.mark.parametrize("n", [1, 2, 9, 130])
+def test_two_sided_reductions_exact_on_dyadic_problems(oracle, n):
+    """L unit bidiagonal (entries in {-1, 0, 1}) and A a small-integer
+    symmetric matrix: every intermediate of the variant-4 loops is an exact
+    dyadic rational (the only divisions are by 1 and 2), so the reductions
+    must equal inv(L) A inv(L)^T and L^T A L computed in exact integers."""
+    rng = np.random.default_rng(n)
+    L0 = _bidiag_unit(n, seed=n + 1)
+    M = rng.integers(-3, 4, (n, n))
+    A = (M + M.T).astype(float)
+    Li = _exact_inverse_of_bidiag(L0)
+    A1 = oracle.two_sided_trsm(A.copy(order="F"), L0.copy(order="F"))
+    assert np.array_equal(np.tril(A1), np.tril(Li @ A.astype(np.int64) @ Li.T).astype(float))
+    assert np.array_equal(np.triu(A1, 1), np.triu(A, 1))
+    A2 = oracle.two_sided_trmm(A.copy(order="F"), L0.copy(order="F"))
+    Lz = np.rint(L0).astype(np.int64)
+    assert np.array_equal(np.tril(A2), np.tril(Lz.T @ A.astype(np.int64) @ Lz).astype(float))
+
+
+@pytest.mark.parametrize("n", [5, 64, 200])
+def test_two_sided_reductions_vs_products(oracle, n):
+    rng = np.random.default_rng(2 * n)
+    L = np.tril(rng.standard_normal((n, n))) / np.sqrt(n) + 2 * np.eye(n)
+    M = rng.standard_normal((n, n))
+    A = M + M.T
+    A1 = oracle.two_sided_trsm(A.copy(order="F"), L.copy(order="F"))
+    Li = np.linalg.inv(L)
+    ref1 = Li @ A @ Li.T
+    assert np.max(np.abs(np.tril(A1) - np.tril(ref1))) <= 1e-12 * np.max(np.abs(ref1))
+    A2 = oracle.two_sided_trmm(A.copy(order="F"), L.copy(order="F"))
+    ref2 = L.T @ A @ L
+    assert np.max(np.abs(np.tril(A2) - np.tril(ref2))) <= 1e-12 * np.max(np.abs(ref2))
+
+
+@pytest.mark.parametrize("gtype", [1, 2, 3])
+def test_gen_definite_eig_vs_lapack_sygvd(oracle, gtype):
+    n = 150
+    rng = np.random.default_rng(gtype)
+    M = rng.standard_normal((n, n))
+    A = M + M.T
+    B = inputs.gram_host(n, seed=gtype + 10)
+    info, w, X, _ = oracle.hermitian_gen_definite_eig(gtype, A, B)
+    assert info == 0
+    ref = sl.eigh(A, B, type=gtype, eigvals_only=True)
+    assert np.max(np.abs(w - ref)) <= 1e-11 * np.max(np.abs(ref))
+    # eigenvector relations of each type, and B-orthonormality (X^T B X = I
+    # for AXBX / ABX, X^T inv(B) X = I for BAX)
+    if gtype == 1:
+        R = A @ X - B @ X * w
+    elif gtype == 2:
+        R = A @ (B @ X) - X * w
+    else:
+        R = B @ (A @ X) - X * w
+    assert np.max(np.abs(R)) <= 1e-10 * np.max(np.abs(w)) * np.max(np.abs(X))
+    G = X.T @ B @ X if gtype != 3 else X.T @ np.linalg.solve(B, X)
+    assert np.max(np.abs(G - np.eye(n))) <= 1e-10
+
+
+def test_gen_definite_eig_b_not_hpd(oracle):
+    n = 50
+    A = np.eye(n)
+    B = inputs.gram_host(n, seed=3)
+    B[20, 20] = -1.0
+    info, *_ = oracle.hermitian_gen_definite_eig(1, A, B)
+    assert info == 21
