@@ -90,6 +90,8 @@ SIGNATURES = {
     "elemamd_cholesky_f64": (_i, [_c, _i, _vp, _i64, _i, _vp, _vp]),
     "elemamd_solve_after_f64": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _vp]),
     "elemamd_hpd_solve_f64": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _i, _vp, _vp]),
+    "elemamd_solve_after_f64_noalloc_limit": (_sz, []),
+    "elemamd_solve_after_f64_noalloc": (_i, [_c, _c, _i, _i, _vp, _i64, _vp, _i64, _vp]),
     "elemamd_gemm_f64": (_i, [_c, _c, _i, _i, _i, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
     "elemamd_hpd_inverse_f64": (_i, [_c, _i, _vp, _i64, _i, _vp, _vp]),
     "elemamd_triangular_inverse_f64": (_i, [_c, _c, _i, _vp, _i64, _i, _vp]),

@@ -34,6 +34,7 @@ SOURCES = [
     "runtime.cpp",
     "server.cpp",
     "capi.cpp",
+    "noalloc.cpp",
 ]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

@@ -384,6 +384,24 @@ int elemamd_fill_normal_f64(double *out, int64_t rows, int64_t cols, int64_t ld,
 int elemamd_sym_residual_f64(int n, int k, const double *A, int64_t lda, const double *X, int64_t ldx,
                              const double *B, int64_t ldb, double *R, int64_t ldr, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Allocation-free SolveAfter.                                         */
+
+/* elemamd_solve_after_f64 without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (tight serving loops
+ * and harnesses that forbid hipMalloc; like elemamd_solve_after_f64 it ends
+ * by synchronizing the stream to read the solve's error word).  The chain
+ * solves' flags and partial sums live in a fixed per-GPU device array of
+ * elemamd_solve_after_f64_noalloc_limit() bytes (9 MiB: 8*n*k bytes of partial
+ * sums plus the flags, e.g. n = 32768 with k = 32), held until the process
+ * ends; larger n*k returns ELEMAMD_ERR_WORKSPACE before anything is enqueued.
+ * Arguments, results and errors are otherwise those of
+ * elemamd_solve_after_f64.  Calls that overlap on one GPU share its array and
+ * race: serialise them. */
+size_t elemamd_solve_after_f64_noalloc_limit(void);
+int elemamd_solve_after_f64_noalloc(char uplo, char orientation, int n, int k, const double *A, int64_t lda,
+                                    double *B, int64_t ldb, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
