@@ -18,12 +18,14 @@ Other workloads (each prints its own single JSON line):
                  FP32 vs split-BF16x3 vs split-BF16x6 trailing update with FP64 refinement
   --mode serve : Poisson arrivals of factor+solve requests, n log-uniform in [64, 8192],
                  kappa = 1e4, 30% reuse of a resident factor, at 3 arrival rates
+  --mode inverse : HPDInverse and TriangularInverse at n (kappa as above) and the
+                 generalized-definite eigensolver (AXBX, eigenvectors) at min(n, 4096)
   --mode roofline : only the trailing-update replays of the single workload (the
                  dominant kernel, both precisions) and a 1 GiB device copy; the
                  command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
                  for the HBM traffic figure (the copy calibrates the counters)
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--mode single|batch|sweep|serve|roofline]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--mode single|batch|sweep|serve|roofline|inverse]
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -61,7 +63,7 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline")
     p.add_argument("--cpu-runs", type=int, default=5)
-    p.add_argument("--mode", default="single", choices=["single", "batch", "sweep", "serve", "roofline"])
+    p.add_argument("--mode", default="single", choices=["single", "batch", "sweep", "serve", "roofline", "inverse"])
     p.add_argument("--count", type=int, default=8192, help="batch: number of matrices")
     p.add_argument("--requests", type=int, default=160, help="serve: requests per arrival rate")
     return p.parse_args()
@@ -425,6 +427,102 @@ def bench_sweep(args):
     return out
 
 
+def bench_inverse(args):
+    """The direct callers of the Cholesky factor: HPDInverse (CholeskyLVar2,
+    n^3 flops), TriangularInverse (LVar3, n^3/3 flops) and the
+    generalized-definite eigensolver (AXBX with eigenvectors), each timed on
+    a resident matrix with its accuracy beside it."""
+    import torch
+
+    import elemental_amd as E
+    from elemental_amd import inputs
+
+    torch.cuda.init()
+    n = args.n
+    out = {}
+    # HPDInverse at kappa = 1e6 (the single workload's matrix construction)
+    A = inputs.spectrum_device(n, args.kappa, seed=args.seed)
+    W = E.Matrix(n, n)
+    times = []
+    for it in range(args.warmup + args.steps):
+        W.tensor().copy_(A.tensor())
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        E.HPDInverse(E.LOWER, W)
+        e1.record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(e0.elapsed_time(e1))
+    Wi = torch.tril(W.tensor()) + torch.tril(W.tensor(), -1).t()
+    # ||inv(A) A - I||_inf / (||A||_inf ||inv(A)||_inf) on a 64-column sample
+    cols = torch.arange(0, n, max(1, n // 64), device="cuda")
+    At = A.tensor()
+    S = torch.tril(At) + torch.tril(At, -1).t()
+    R = Wi @ S[:, cols]
+    R[cols, torch.arange(len(cols), device="cuda")] -= 1.0
+    rel = float(R.abs().max() / (S.abs().sum(1).max() * Wi.abs().sum(1).max()))
+    ms = float(np.median(times))
+    out["hpd_inverse"] = {"n": n, "ms": round(ms, 3), "tflops": round(n ** 3 / (ms * 1e-3) / 1e12, 2),
+                          "frac_fp64_peak": round(n ** 3 / (ms * 1e-3) / PEAK["f64"], 4),
+                          "relative_residual_sampled": rel}
+    del Wi, R, S
+    # TriangularInverse of the HPD factor
+    L = E.Matrix(n, n)
+    L.tensor().copy_(A.tensor())
+    E.Cholesky(E.LOWER, L)
+    Lt = torch.tril(L.tensor()).clone()
+    times = []
+    for it in range(args.warmup + args.steps):
+        W.tensor().copy_(Lt)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        E.TriangularInverse(E.LOWER, E.NON_UNIT, W)
+        e1.record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(e0.elapsed_time(e1))
+    X = torch.tril(W.tensor())
+    R = X @ Lt[:, cols]
+    R[cols, torch.arange(len(cols), device="cuda")] -= 1.0
+    ms = float(np.median(times))
+    out["triangular_inverse"] = {"n": n, "ms": round(ms, 3), "tflops": round(n ** 3 / 3 / (ms * 1e-3) / 1e12, 2),
+                                 "frac_fp64_peak": round(n ** 3 / 3 / (ms * 1e-3) / PEAK["f64"], 4),
+                                 "max_abs_inv_L_L_minus_I_sampled": float(R.abs().max())}
+    del L, Lt, X, R, W, A
+    torch.cuda.empty_cache()
+    # generalized-definite eigenproblem A x = lambda B x, n = 4096, with vectors
+    ng = min(n, 4096)
+    G = inputs.normal_device(ng, ng, args.seed + 5).tensor()
+    Ag = E.Matrix(ng, ng)
+    Ag.tensor().copy_(G + G.t())
+    Bg = inputs.spectrum_device(ng, 1e3, seed=args.seed + 6)
+    A0, B0 = Ag.tensor().clone(), Bg.tensor().clone()
+    times = []
+    for it in range(1 + min(args.steps, 3)):
+        Ag.tensor().copy_(A0)
+        Bg.tensor().copy_(B0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        w, Xv = E.HermitianGenDefiniteEig(E.AXBX, E.LOWER, Ag, Bg, vectors=True)
+        torch.cuda.synchronize()
+        if it >= 1:
+            times.append((time.perf_counter() - t0) * 1e3)
+    wv, Xt = w.tensor()[:, 0], Xv.tensor()
+    Rg = A0 @ Xt - (B0 @ Xt) * wv.unsqueeze(0)
+    out["hermitian_gen_definite_eig"] = {
+        "n": ng, "type": "AXBX", "vectors": True, "ms": round(float(np.median(times)), 2),
+        "max_abs_AX_minus_BXL": float(Rg.abs().max()),
+        "max_abs_XtBX_minus_I": float((Xt.t() @ B0 @ Xt - torch.eye(ng, dtype=torch.float64, device="cuda")).abs().max())}
+    res = _common("HPD inverse / triangular inverse / generalized eigensolver (ms, one GPU)", "ms",
+                  out["hpd_inverse"]["ms"], args, out["hpd_inverse"]["ms"], "f64",
+                  {"workload": f"HPDInverse and TriangularInverse n={n} kappa={args.kappa:g}, "
+                               f"HermitianGenDefiniteEig(AXBX) n={ng}", "parallelism": "single GPU"}, higher=False)
+    res.update(out)
+    return res
+
+
 def bench_serve(args):
     """Serving loop under Poisson arrivals at three rates, plain FP64 and
     FP32 + FP64 refinement."""
@@ -524,7 +622,7 @@ def main():
         sys.exit(2)
     if args.mode != "single":
         out = {"batch": bench_batch, "sweep": bench_sweep, "serve": bench_serve,
-               "roofline": bench_roofline}[args.mode](args)
+               "roofline": bench_roofline, "inverse": bench_inverse}[args.mode](args)
         print(json.dumps(out))
         return
     import torch

@@ -375,10 +375,7 @@ __global__ void batch_eta_kernel(BatchArgs a, const double* red, double* eta, do
     double e = 0.0;
     for (int c = 0; c < a.k[p]; ++c) {
         const double rm = rd[1 + c], xm = rd[1 + kmax + c], bm = rd[1 + 2 * kmax + c];
-        const double den = anorm * xm + bm;
-        double ec = den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
-        if (ec != ec) ec = INFINITY;
-        e = fmax(e, ec);
+        e = fmax(e, column_backward_error(rm, anorm, xm, bm));
         if (rscale != nullptr) rscale[(i64)p * kmax + c] = rm;
     }
     eta[p] = e;

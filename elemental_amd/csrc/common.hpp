@@ -29,6 +29,17 @@ enum : int {
 
 __host__ __device__ inline i64 ceil_div(i64 a, i64 b) { return (a + b - 1) / b; }
 
+// Normwise backward error of one column, ||r||_inf / (||A||_inf ||x||_inf +
+// ||b||_inf), from the norms: a NaN anywhere (a NaN in A, x or b reaches the
+// residual or one of the norms) or a zero denominator with a nonzero
+// residual gives +inf, so such a column never counts as converged.
+__host__ __device__ inline double column_backward_error(double rnorm, double anorm, double xnorm, double bnorm) {
+    const double den = anorm * xnorm + bnorm;
+    if (rnorm != rnorm || den != den) return __builtin_inf();
+    const double e = den > 0.0 ? rnorm / den : (rnorm > 0.0 ? __builtin_inf() : 0.0);
+    return e != e ? __builtin_inf() : e;
+}
+
 // Column-major element (i, j) of a matrix with leading dimension ld.
 template <typename T>
 __host__ __device__ inline T& at(T* A, i64 i, i64 j, i64 ld) { return A[i + j * ld]; }

@@ -99,9 +99,6 @@ struct GemmLoader {
 };
 
 template <bool TA, bool TB>
-// C may be the same memory as A when op(B) has at most 128 columns (one
-// tile column): each workgroup reads all of its rows of A before it writes
-// them, and rows are disjoint between workgroups (hence no __restrict__).
 __global__ __launch_bounds__(256, 2) void gemm_f64_kernel(int m, int n, int k, double alpha, const double* A,
                                                           i64 lda, int ashape, const double* B, i64 ldb, int bshape,
                                                           double beta, double* C, i64 ldc, int tri,

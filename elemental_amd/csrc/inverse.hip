@@ -114,3 +114,112 @@ hipError_t launch_trtrmm_unb_f64(int n, const double* X, i64 ldx, double* C, i64
 }
 
 }  // namespace elemamd
+
+namespace elemamd {
+
+// inv(L) of an n x n (n <= 128) lower-triangular block by recursive doubling
+// (not the reference's operation order; for the blocked algorithms, whose
+// diagonal blocks only need an accurate inverse).  For a block split as
+// L = [L11 0; L21 L22] with X11 = inv(L11), X22 = inv(L22) known,
+//     X21 = -X22 (L21 X11),
+// applied for block sizes 1, 2, 4, ..., 64 over the whole 128 x 128 block
+// (padded with the identity past n), so the 128-step substitution chain
+// becomes 7 levels of short dot products.  X lives in LDS (128 x 129
+// doubles).  At each level the b x b blocks L21 are first staged into the
+// upper-right b x b block of their 2b x 2b diagonal block (zero in the
+// result), T = L21 X11 is formed in registers, written over them, and
+// X21 = -X22 T read from there.  Output: X (ldx >= 128), the full 128 x 128
+// inverse with zeros above the diagonal (the identity past n); when Lout is
+// not null its lower triangle (strictly lower for a unit diagonal) receives
+// inv(L) too.  unit: the diagonal of L is taken as one and never read.
+constexpr int kTrtriFastPerThread = 16;  // T elements per thread at the 64-level
+__global__ __launch_bounds__(256) void trtri_fast_kernel(int n, const double* L, i64 ldl, int unit,
+                                                         double* __restrict__ X, i64 ldx, double* Lout, i64 ldo,
+                                                         const int* __restrict__ info) {
+    __shared__ double sX[kInvN][kInvN + 1];
+    if (info != nullptr && *info != 0) return;
+    const int t = threadIdx.x;
+    // padded L element (identity past n and on a unit diagonal)
+    auto Lat = [&](int i, int j) -> double {
+        const bool in = i < n && j < n && !(unit && i == j);
+        const double v = L[in ? i + (i64)j * ldl : 0];
+        return in ? v : (i == j ? 1.0 : 0.0);
+    };
+    for (int e = t; e < kInvN; e += 256) sX[e][e] = 1.0 / Lat(e, e);
+    for (int e = t; e < kInvN * kInvN; e += 256) {
+        const int i = e & 127, j = e >> 7;
+        if (i != j) sX[i][j] = 0.0;
+    }
+    int lb = 0;
+    for (int b = 1; b < kInvN; b *= 2, ++lb) {
+        const int ne = 64 * b;  // elements of all the level's b x b blocks (128 / 2b pairs)
+        const int bb = 2 * lb;  // log2(b * b)
+        __syncthreads();
+        // stage L21 of every pair (lanes along rows: coalesced column reads)
+        for (int e = t; e < ne; e += 256) {
+            const int q = e >> bb, r = e & ((1 << bb) - 1), i = r & (b - 1), j = r >> lb;
+            const int r0 = 2 * b * q, rb = r0 + b;
+            sX[r0 + i][rb + j] = Lat(rb + i, r0 + j);
+        }
+        __syncthreads();
+        // T = L21 X11 (X11 lower: m >= j), kept in registers until every
+        // thread is done reading the staged L21
+        double tv[kTrtriFastPerThread];
+#pragma unroll
+        for (int u = 0; u < kTrtriFastPerThread; ++u) {
+            const int e = t + 256 * u;
+            tv[u] = 0.0;
+            if (e < ne) {
+                const int q = e >> bb, r = e & ((1 << bb) - 1), i = r & (b - 1), j = r >> lb;
+                const int r0 = 2 * b * q, rb = r0 + b;
+                double acc = 0.0;
+                for (int m = j; m < b; ++m) acc = fma(sX[r0 + i][rb + m], sX[r0 + m][r0 + j], acc);
+                tv[u] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kTrtriFastPerThread; ++u) {
+            const int e = t + 256 * u;
+            if (e < ne) {
+                const int q = e >> bb, r = e & ((1 << bb) - 1), i = r & (b - 1), j = r >> lb;
+                const int r0 = 2 * b * q, rb = r0 + b;
+                sX[r0 + i][rb + j] = tv[u];
+            }
+        }
+        __syncthreads();
+        // X21 = -X22 T (X22 lower: m <= i)
+        for (int e = t; e < ne; e += 256) {
+            const int q = e >> bb, r = e & ((1 << bb) - 1), i = r & (b - 1), j = r >> lb;
+            const int r0 = 2 * b * q, rb = r0 + b;
+            double acc = 0.0;
+            for (int m = 0; m <= i; ++m) acc = fma(sX[rb + i][rb + m], sX[r0 + m][rb + j], acc);
+            sX[rb + i][r0 + j] = -acc;
+        }
+        __syncthreads();
+        // the T blocks back to zero (the upper triangle of the result)
+        for (int e = t; e < ne; e += 256) {
+            const int q = e >> bb, r = e & ((1 << bb) - 1), i = r & (b - 1), j = r >> lb;
+            const int r0 = 2 * b * q, rb = r0 + b;
+            sX[r0 + i][rb + j] = 0.0;
+        }
+    }
+    __syncthreads();
+    for (int e = t; e < kInvN * kInvN; e += 256) {
+        const int i = e & 127, j = e >> 7;
+        const double v = sX[i][j];
+        X[i + (i64)j * ldx] = v;
+        if (Lout != nullptr && i < n && (i > j || (i == j && !unit))) Lout[i + (i64)j * ldo] = v;
+    }
+}
+
+hipError_t launch_trtri_fast_f64(int n, const double* L, i64 ldl, bool unit, double* X, i64 ldx, double* Lout,
+                                 i64 ldo, const int* info, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (n > kInvN || ldx < kInvN) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(trtri_fast_kernel, dim3(1), dim3(256), 0, s, n, L, ldl, unit ? 1 : 0, X, ldx, Lout, ldo,
+                       info);
+    return hipGetLastError();
+}
+
+}  // namespace elemamd

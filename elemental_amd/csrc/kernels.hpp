@@ -101,6 +101,12 @@ hipError_t launch_gemm_tn_f64(int m, int n, int k, double alpha, const double* A
 // inv(L).  L and Lout may be the same block.
 hipError_t launch_trtri_unb_f64(int n, const double* L, i64 ldl, bool unit, double* X, i64 ldx, double* Lout,
                                 i64 ldo, const int* info, hipStream_t s);
+// The same inverse by recursive doubling (7 levels of short products, not
+// the reference's operation order): for the diagonal blocks of the blocked
+// algorithms.  X (ldx >= 128) receives the full 128 x 128 inverse (the
+// identity past n), Lout (may be null) the lower triangle.
+hipError_t launch_trtri_fast_f64(int n, const double* L, i64 ldl, bool unit, double* X, i64 ldx, double* Lout,
+                                 i64 ldo, const int* info, hipStream_t s);
 // internal::TrtrmmLUnblocked(TRANSPOSE) on the lower triangle of X: the lower
 // triangle of C := X^T X.
 hipError_t launch_trtrmm_unb_f64(int n, const double* X, i64 ldx, double* C, i64 ldc, const int* info, hipStream_t s);

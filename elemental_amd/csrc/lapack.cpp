@@ -74,6 +74,7 @@ static size_t plane_region_bytes(int n, int nb, int trailing) {
 }
 size_t lowprec_scratch_bytes(int n, int nb, int trailing) {
     if (nb < 1 || nb > 128) return 0;
+    // the planes, then the diagonal pieces' inverses (4 x 32 x 32)
     return plane_region_bytes(n, nb, trailing) + 4 * 1024 * sizeof(float);
 }
 
@@ -105,11 +106,6 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
     const size_t plane_bytes = planes ? plane_region_bytes(n, nb, trailing) : 0;
     unsigned short* pbuf = nullptr;
     float* Xd = nullptr;
-    double* Xinv = nullptr;  // FP64: the diagonal block's inverse (128 x 128)
-    if (sizeof(T) == 8 && n > nb) {
-        Xinv = (double*)stream_scratch(128 * 128 * sizeof(double), s, kScratchFactor);
-        if (Xinv == nullptr) return kErrHip;
-    }
     if (sizeof(T) == 4) {
         char* scratch = scratch_in != nullptr ? (char*)scratch_in
                                               : (char*)stream_scratch(lowprec_scratch_bytes(n, nb, trailing), s);
@@ -151,16 +147,8 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
             }
             if constexpr (sizeof(T) == 8) {
                 ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
-                // the panel A21 := A21 inv(L11)^T as a product with the
-                // block's inverse (in place: one tile column, every
-                // workgroup reads its rows before it writes them)
-                if (n - k - b > 0) {
-                    ELEMAMD_HIP_TRY(launch_trtri_unb_f64(b, (const double*)Akk, lda, false, Xinv, 128, nullptr, 0,
-                                                         dinfo, s));
-                    ELEMAMD_HIP_TRY(launch_gemm_f64(false, true, n - k - b, b, b, 1.0, (const double*)Akk + b, lda,
-                                                    kGemmShapeFull, Xinv, 128, kGemmShapeLower, 0.0,
-                                                    (double*)Akk + b, lda, 0, dinfo, s));
-                }
+                if (n - k - b > 0)
+                    ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
             } else {  // the low-precision factor: diagonal inverse, panel as a product with it
                 ELEMAMD_HIP_TRY(launch_potrf_f32(b, (float*)Akk, lda, Xd, dinfo, k, s));
                 if (below_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_d, 0));
@@ -506,9 +494,7 @@ int host_eta(Ctx& c, const std::vector<double>& bn, std::vector<double>& eta_c, 
     eta_c.resize(k);
     eta = 0.0;
     for (int col = 0; col < k; ++col) {
-        const double den = c.anorm * nx[k + col] + bn[col];
-        const double e = (den > 0.0) ? nx[col] / den : (nx[col] > 0.0 ? INFINITY : 0.0);
-        eta_c[col] = std::isnan(e) ? INFINITY : e;
+        eta_c[col] = column_backward_error(nx[col], c.anorm, nx[k + col], bn[col]);
         eta = std::max(eta, eta_c[col]);
     }
     return kOk;
@@ -574,9 +560,7 @@ int cg_refine(Ctx& c, const MixedOpts& o, const std::vector<double>& bn, MixedRe
         eta_c.resize(k);
         eta = 0.0;
         for (int col = 0; col < k; ++col) {
-            const double den = c.anorm * nx.p[k + col] + bn[col];
-            const double v = (den > 0.0) ? nx.p[col] / den : (nx.p[col] > 0.0 ? INFINITY : 0.0);
-            eta_c[col] = std::isnan(v) ? INFINITY : v;
+            eta_c[col] = column_backward_error(nx.p[col], c.anorm, nx.p[k + col], bn[col]);
             eta = std::max(eta, eta_c[col]);
         }
         return hipSuccess;
@@ -878,9 +862,7 @@ static int fallback_fp64(int n, int k, const double* A, i64 lda, double* B, i64 
     ELEMAMD_HIP_TRY(hipStreamSynchronize(s));
     double eta = 0.0;
     for (int c = 0; c < k; ++c) {
-        const double den = f.anorm * h[k + c] + h[2 * k + c];
-        const double e = den > 0.0 ? h[c] / den : (h[c] > 0.0 ? INFINITY : 0.0);
-        eta = std::max(eta, std::isnan(e) ? INFINITY : e);
+        eta = std::max(eta, column_backward_error(h[c], f.anorm, h[k + c], h[2 * k + c]));
     }
     r.backward_error = eta;
     r.converged = eta <= o.tol ? 1 : 0;
@@ -983,9 +965,7 @@ int mixed_refine(int n, int k, const double* A, i64 lda, double* B, i64 ldb, con
         if (rc) return rc;
         eta = 0.0;
         for (int col = 0; col < k; ++col) {
-            const double den = c.anorm * xn[col] + bn[col];
-            const double e = (den > 0.0) ? rn[col] / den : (rn[col] > 0.0 ? INFINITY : 0.0);
-            eta = std::max(eta, std::isnan(e) ? INFINITY : e);
+            eta = std::max(eta, column_backward_error(rn[col], c.anorm, xn[col], bn[col]));
         }
         if (it > 0 && eta <= o.tol) break;
         if (it == o.max_iter) break;

@@ -27,6 +27,7 @@ namespace elemamd {
 template <typename T>
 __global__ __launch_bounds__(kPotrfThreads) void potrf_unb_kernel(int n, T* __restrict__ A, i64 lda,
                                                                   int* __restrict__ info, int col_offset) {
+    __builtin_amdgcn_s_setprio(3);  // chain kernel: issues ahead of the trailing update's waves
     potrf_unb_block<T>(n, A, lda, info, col_offset);
 }
 

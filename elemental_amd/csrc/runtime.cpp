@@ -3,7 +3,6 @@
 #include <cstdlib>
 #include <initializer_list>
 #include <map>
-#include <tuple>
 #include <utility>
 
 namespace elemamd {
@@ -14,7 +13,7 @@ struct Buf {
     size_t cap = 0;
 };
 struct Cache {
-    std::map<std::tuple<int, hipStream_t, int>, Buf> bufs;
+    std::map<std::pair<int, hipStream_t>, Buf> bufs;
     ~Cache() {
         for (auto& kv : bufs)
             if (kv.second.p) (void)hipFree(kv.second.p);
@@ -62,11 +61,11 @@ long solve_spin_limit() {
     return 1L << 26;
 }
 
-void* stream_scratch(size_t bytes, hipStream_t stream, int slot) {
+void* stream_scratch(size_t bytes, hipStream_t stream) {
     static thread_local Cache cache;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-    Buf& b = cache.bufs[{dev, stream, slot}];
+    Buf& b = cache.bufs[{dev, stream}];
     if (b.cap >= bytes && b.p) return b.p;
     if (b.p) {
         if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;

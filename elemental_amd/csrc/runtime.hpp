@@ -9,12 +9,7 @@ namespace elemamd {
 // Device memory of at least `bytes`, owned by the calling thread and reserved
 // for work enqueued on `stream`; grows on demand (the old buffer is released
 // only after `stream` has drained).  Returns nullptr on allocation failure.
-// Separate slots are separate buffers: a caller that holds a slot-0 buffer
-// across a call that needs scratch of its own gives that call another slot
-// (slot 1: the FP64 factorization's diagonal-block inverse), so a growth
-// never frees memory still in use.
-enum : int { kScratchMain = 0, kScratchFactor = 1 };
-void* stream_scratch(size_t bytes, hipStream_t stream, int slot = kScratchMain);
+void* stream_scratch(size_t bytes, hipStream_t stream);
 
 // An auxiliary stream (default priority) and two timing-free events paired
 // with `stream` for the look-ahead factorization: work forked onto `aux`

@@ -29,6 +29,9 @@ template <typename T>
 __global__ __launch_bounds__(256) void trsm_rlt_blocked_kernel(int m, int nb, const T* __restrict__ L, i64 ldl,
                                                                T* __restrict__ B, i64 ldb,
                                                                const int* __restrict__ info, int limit) {
+    // the factorization's chain: its waves issue ahead of the trailing
+    // update's waves sharing the SIMD (wave priority, not queue priority)
+    __builtin_amdgcn_s_setprio(3);
     trsm_rlt_blocked_rows<T>(m, nb, L, ldl, B, ldb, info, limit, blockIdx.x);
 }
 
