@@ -58,20 +58,30 @@ __device__ inline int factor32(float (*sA)[kPiLd], double (*sD)[34], double (*sC
         int jv = j;  // opaque per column: the lane masks are made here, not all up front
         asm volatile("" : "+s"(jv));
         const bool own = hh == (jv >> 4), diag = i == jv;
-        if (own) {
-            col[i] = (i > jv) ? d[jq] : 0.0;
-            if (diag) col[32] = d[jq];
-        }
-        const double piv = col[32];
+        if (own) col[i] = (i > jv) ? d[jq] : 0.0;
+        // the column's readers are other lanes of this wave: the wave
+        // barrier keeps every read below the store (without it the compiler
+        // may read in the non-owners' branch, ahead of the owners' store)
+        __builtin_amdgcn_wave_barrier();
+        // the pivot comes straight from its lane's register (no LDS round
+        // trip on the test); the 16 column entries are read in one batch
+        const int pl = 32 * jh + j;
+        const double piv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(d[jq]), pl),
+                                            __builtin_amdgcn_readlane(__double2loint(d[jq]), pl));
+        const double cij = col[i];
+        double ck[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) ck[q] = col[16 * hh + q];
         if (!(piv > 0.0)) return j + 1;
         double y = __builtin_amdgcn_rsq(piv);
         y = fma(y, fma(-0.5 * piv * y, y, 0.5), y);
-        const double lij = col[i] * y;
+        const double lij = cij * y;
         if (own) d[jq] = diag ? piv * y : lij;
+        const double sij = lij * y;  // a_ij / a_jj: one multiplier for the whole row
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const int q = (jq + 1 + u) & 15;  // the next column's entry first
-            d[q] = fma(-lij, col[16 * hh + q] * y, d[q]);
+            d[q] = fma(-sij, ck[q], d[q]);
         }
         if (lv == 0) sRd[j] = y;
     }
