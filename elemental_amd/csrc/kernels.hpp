@@ -32,11 +32,14 @@ hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64
 // inverses go to Xd, 4 x 32 x 32 floats), and B (m x b) := B L^{-T} with the
 // factored block L and those inverses; planes (may be null): the result's np
 // bf16 planes (launch_split_bf16's form) are also written, row r of B to
-// planes + p*ps + r*ldk (columns 0..b-1).
+// planes + p*ps + r*ldk (columns 0..b-1).  upd (may be null): B first takes
+// B -= P_B P_top^T over uk columns of the same np planes, P_top = upd rows
+// 0..b-1 and P_B = upd rows b..b+m-1 (the block column's pending in-group
+// update; the planes at upd must not overlap B's output planes).
 hipError_t launch_potrf_f32(int b, float* A, i64 lda, float* Xd, int* info, int col_offset, hipStream_t s);
 hipError_t launch_panel_f32(int m, int b, const float* L, i64 ldl, const float* Xd, float* B, i64 ldb,
                             const int* info, int limit, unsigned short* planes, int np, i64 ps, i64 ldk,
-                            hipStream_t s);
+                            hipStream_t s, const unsigned short* upd = nullptr, int uk = 0);
 
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
 // With info != null the k columns are factor columns [kcol0, kcol0 + k) and a

@@ -123,21 +123,14 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
         for (int k = c0; k < c1; k += nb) {
             const int b = std::min(nb, n - k);
             T* Akk = A + k + (i64)k * lda;
-            bool below_pending = false;  // rows below the diagonal block updated on aux2
+            // the block column's pending in-group update (panels c0 .. k-1):
+            // with planes, the diagonal block here and the rows below it
+            // inside the panel kernel (each slab before its solve)
+            const unsigned short* upd = nullptr;
             if (k > c0) {  // panels c0 .. k-1 of this group into block column k
                 if (planes) {
-                    // look-ahead: the diagonal block first (the potrf waits for
-                    // it alone), the rows below it on aux2 under the potrf
                     const unsigned short* Pk = Pl + (i64)(k - c0) * ldk;
-                    if (n - k > b) {
-                        ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_c, s));
-                        ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux2, sp->ev_c, 0));
-                        ELEMAMD_HIP_TRY(launch_syrk_col_bf16_planes(np, n - k, b, b, n - k, k - c0, Pk,
-                                                                    (i64)plane_elems, ldk, (float*)Akk, lda, dinfo,
-                                                                    c0, nb, sp->aux2));
-                        ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_d, sp->aux2));
-                        below_pending = true;
-                    }
+                    upd = Pk;
                     ELEMAMD_HIP_TRY(launch_syrk_diag_bf16_planes(np, b, k - c0, Pk, (i64)plane_elems, ldk,
                                                                  (float*)Akk, lda, dinfo, c0, nb, s));
                 } else {
@@ -151,11 +144,11 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
                     ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
             } else {  // the low-precision factor: diagonal inverse, panel as a product with it
                 ELEMAMD_HIP_TRY(launch_potrf_f32(b, (float*)Akk, lda, Xd, dinfo, k, s));
-                if (below_pending) ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_d, 0));
                 if (n - k - b > 0)
                     ELEMAMD_HIP_TRY(launch_panel_f32(
                         n - k - b, b, (const float*)Akk, lda, Xd, (float*)Akk + b, lda, dinfo, k + b,
-                        planes ? Pl + (i64)(k + b - c0) * ldk + (k - c0) : nullptr, np, (i64)plane_elems, ldk, s));
+                        planes ? Pl + (i64)(k + b - c0) * ldk + (k - c0) : nullptr, np, (i64)plane_elems, ldk, s,
+                        upd, k - c0));
             }
         }
         if (after_group != nullptr) {

@@ -253,48 +253,184 @@ __global__ __launch_bounds__(256) void potrf_f32_kernel(int b, float* __restrict
     if (fail != 0 && t == 0 && info != nullptr) *info = col_offset + fail;
 }
 
+// Rows rw .. rw+15 of a panel slab (staged in sB) -= P_slab P_top^T: the
+// update of the block column from the group's earlier panels, which these
+// rows have not received yet.  upd points at the planes' row of the block's
+// first (diagonal) row, so P_top = upd rows 0 .. b-1 and the slab's rows are
+// upd rows b + r0 + rw + i; K = kk columns (the planes are zero-padded to a
+// multiple of 32 columns).  One wave, split-bf16 products (all plane pairs
+// p + q < NP), fragments loaded straight from the planes into registers,
+// the next 32-wide chunk loading while the current one multiplies.
+template <int NP>
+__device__ __forceinline__ void panel_slab_update(float (*sB)[kPiLd], const unsigned short* __restrict__ upd, i64 ps,
+                                                  i64 ldk, int kk, int b, int m, i64 r0, int rw, int m16, int g) {
+    const i64 rows = (i64)b + m;  // rows of the planes block
+    const i64 ra = b + r0 + rw + m16 < rows ? b + r0 + rw + m16 : rows - 1;  // clamped; its outputs are dropped
+    const unsigned short* a_p = upd + ra * ldk + 8 * g;
+    const s8_t neg = {(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
+                      (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000};
+    const int nchunks = (kk + 31) / 32;
+    // two passes over 64 columns each keep the fragments in few registers
+    // (this kernel must stay small enough to be placed beside the trailing
+    // update's workgroups)
+    for (int h = 0; h < 2; ++h) {
+        if (64 * h >= b) break;
+        f4_t acc[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[nt][v] = sB[rw + 4 * g + v][64 * h + 16 * nt + m16];
+        constexpr int kR = 3;  // chunks in flight: the loads are L2-latency bound
+        s8_t fa[kR][NP], fb[kR][4][NP];  // [buffer][plane], [buffer][column tile][plane]
+        auto load = [&](int buf, int kc) __attribute__((always_inline)) {
+#pragma unroll
+            for (int p = 0; p < NP; ++p) fa[buf][p] = *reinterpret_cast<const s8_t*>(a_p + p * ps + kc) ^ neg;
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                const int c = 64 * h + 16 * nt + m16;
+                const i64 rb = (i64)(c < b ? c : b - 1) * ldk + 8 * g + kc;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) fb[buf][nt][p] = *reinterpret_cast<const s8_t*>(upd + p * ps + rb);
+            }
+        };
+        auto step = [&](int buf) __attribute__((always_inline)) {
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+                for (int o = NP - 1; o >= 0; --o)  // term order o = p + q, smallest first
+#pragma unroll
+                    for (int p = o; p >= 0; --p) mfma_acc_bf16(acc[nt], fa[buf][p], fb[buf][nt][o - p]);
+        };
+        // chunk c lives in buffer c % kR; chunk c + kR - 1 loads while chunk c multiplies
+#pragma unroll
+        for (int c = 0; c < kR - 1; ++c)
+            if (c < nchunks) load(c, 32 * c);
+        for (int c0 = 0; c0 < nchunks; c0 += kR) {
+#pragma unroll
+            for (int u = 0; u < kR; ++u) {
+                const int c = c0 + u;
+                if (c >= nchunks) break;
+                if (c + kR - 1 < nchunks) load((u + kR - 1) % kR, 32 * (c + kR - 1));
+                step(u);
+            }
+        }
+        mfma_retire();
+        // columns past b stay zero (the padding the solve relies on)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (64 * h + 16 * nt + m16 < b) sB[rw + 4 * g + v][64 * h + 16 * nt + m16] = acc[nt][v];
+    }
+}
+
+// The same update for two planes with P_top staged through LDS: the four
+// waves share each 32-column chunk of P_top's planes (16 KB) instead of
+// each loading all of it (the update is bound by that traffic), and chunk
+// c + 2's global loads are in flight while chunk c multiplies.  sP: two
+// chunk buffers, [buffer][plane][row 0..127][32 columns] bf16 (32 KB).
+// All threads of the workgroup call it (it synchronises the workgroup).
+__device__ __forceinline__ void panel_slab_update_lds(float (*sB)[kPiLd], unsigned short* sP,
+                                                      const unsigned short* __restrict__ upd, i64 ps, i64 ldk, int kk,
+                                                      int b, int m, i64 r0, int t, int w, int m16, int g) {
+    constexpr int NP = 2, kR = 3;
+    const int rw = 16 * w;
+    const i64 rows = (i64)b + m;
+    const i64 ra = b + r0 + rw + m16 < rows ? b + r0 + rw + m16 : rows - 1;  // clamped; its outputs are dropped
+    const unsigned short* a_p = upd + ra * ldk + 8 * g;
+    const s8_t neg = {(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
+                      (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000};
+    // staging: thread t carries 16-byte segments 2 (t / 128) and 2 (t / 128) + 1
+    // of P_top row t % 128 (rows past b repeat row b - 1; their outputs are dropped)
+    const int sr = t & 127, sseg = (t >> 7) * 2;
+    const unsigned short* s_src = upd + (i64)(sr < b ? sr : b - 1) * ldk + 8 * sseg;
+    s8_t st[kR][NP][2], fa[kR][NP];
+    auto gload = [&](int slot, int kc) __attribute__((always_inline)) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) st[slot][p][h] = *reinterpret_cast<const s8_t*>(s_src + p * ps + kc + 8 * h);
+            fa[slot][p] = *reinterpret_cast<const s8_t*>(a_p + p * ps + kc) ^ neg;
+        }
+    };
+    auto sstore = [&](int slot, int buf) __attribute__((always_inline)) {
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<s8_t*>(sP + ((buf * NP + p) * 128 + sr) * 32 + 8 * (sseg + h)) = st[slot][p][h];
+    };
+    f4_t acc[8];
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[nt][v] = sB[rw + 4 * g + v][16 * nt + m16];
+    const int nchunks = (kk + 31) / 32;
+#pragma unroll
+    for (int c = 0; c < kR - 1; ++c)
+        if (c < nchunks) gload(c, 32 * c);
+    for (int c0 = 0; c0 < nchunks; c0 += kR) {
+#pragma unroll
+        for (int u = 0; u < kR; ++u) {
+            const int c = c0 + u;  // chunk c: registers slot u, LDS buffer c % 2
+            if (c >= nchunks) break;
+            const int buf = c & 1;
+            sstore(u, buf);
+            // chunk c is staged, and every wave is done with chunk c - 2,
+            // the last user of this buffer
+            __syncthreads();
+            if (c + kR - 1 < nchunks) gload((u + kR - 1) % kR, 32 * (c + kR - 1));
+#pragma unroll
+            for (int nt = 0; nt < 8; ++nt) {
+                s8_t fb[NP];
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    fb[p] = *reinterpret_cast<const s8_t*>(sP + ((buf * NP + p) * 128 + 16 * nt + m16) * 32 + 8 * g);
+#pragma unroll
+                for (int o = NP - 1; o >= 0; --o)  // term order o = p + q, smallest first
+#pragma unroll
+                    for (int p = o; p >= 0; --p) mfma_acc_bf16(acc[nt], fa[u][p], fb[o - p]);
+            }
+        }
+    }
+    mfma_retire();
+    // columns past b stay zero (the padding the solve relies on)
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (16 * nt + m16 < b) sB[rw + 4 * g + v][16 * nt + m16] = acc[nt][v];
+}
+
 // B (m x b, b <= 128) := B L11^{-T}, L11 the b x b float factor at L; one
 // 64-row slab per workgroup, the result staged through LDS so the
-// column-major stores are contiguous.
+// column-major stores are contiguous.  With NPU > 0 the slab first takes the
+// block column's pending update from the group's earlier panels (NPU planes
+// at upd, uk columns; panel_slab_update), so no separate kernel has to
+// update the rows below the diagonal block before this one runs.
+template <int NPU>
 __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const float* __restrict__ L, i64 ldl,
                                                         const float* __restrict__ Xd, float* __restrict__ B, i64 ldb,
                                                         const int* __restrict__ info, int limit,
                                                         unsigned short* __restrict__ planes, int np, i64 ps,
-                                                        i64 ldk) {
+                                                        i64 ldk, const unsigned short* __restrict__ upd, int uk) {
     // LDS kept under half a CU (80 KB) so workgroups of this kernel fit beside
     // two of the trailing update's: L11 only as its six 32 x 32 blocks below
     // the diagonal pieces (block (p, q), q < p, at p (p - 1) / 2 + q; zero
-    // outside the b x b block, whose padding is the identity)
-    __shared__ float sLb[6][32][36];
-    __shared__ float sXd[4][32][36];  // inverses of the four 32 x 32 diagonal pieces
+    // outside the b x b block, whose padding is the identity) and the
+    // inverses of its four 32 x 32 diagonal pieces, in one area that first
+    // stages the slab's pending update
+    __shared__ __align__(16) float sU[10 * 32 * 36];
+    float(*sLb)[32][36] = reinterpret_cast<float(*)[32][36]>(sU);
+    float(*sXd)[32][36] = reinterpret_cast<float(*)[32][36]>(sU + 6 * 32 * 36);
     __shared__ float sB[64][kPiLd];
     __builtin_amdgcn_s_setprio(3);  // chain kernel (see potrf_f32_kernel)
     if (info != nullptr && *info != 0 && *info <= limit) return;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
     const i64 r0 = (i64)blockIdx.x * 64;
-    {
-        float v[32];
-        // block e / 1024: row e % 32 (lanes along the column), column (e / 32) % 32
-        // (clamped loads, then select)
-        for (int e0 = 0; e0 < 6 * 1024; e0 += 32 * 256) {
-#pragma unroll
-            for (int q = 0; q < 32; ++q) {
-                const int e = e0 + t + 256 * q;
-                const int blk = e >> 10, rr = e & 31, cc = (e >> 5) & 31;
-                const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
-                const int r = 32 * p + rr, c = 32 * qb + cc;
-                v[q] = (e < 6 * 1024) ? L[(r < b ? r : b - 1) + (i64)(c < b ? c : b - 1) * ldl] : 0.0f;
-            }
-#pragma unroll
-            for (int q = 0; q < 32; ++q) {
-                const int e = e0 + t + 256 * q;
-                const int blk = e >> 10, rr = e & 31, cc = (e >> 5) & 31;
-                const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
-                const int r = 32 * p + rr, c = 32 * qb + cc;
-                if (e < 6 * 1024) sLb[blk][rr][cc] = (r < b && c < b) ? v[q] : 0.0f;
-            }
-        }
-        // the slab: row t % 64, columns t / 64 + 4q
+    const int rw = 16 * w;
+    float v[32];
+    {  // the slab: row t % 64, columns t / 64 + 4q
         const int rs = t & 63, c_lo = t >> 6;
         const i64 rb = r0 + rs < m ? r0 + rs : m - 1;
 #pragma unroll
@@ -308,10 +444,39 @@ __global__ __launch_bounds__(256) void panel_f32_kernel(int m, int b, const floa
             sB[rs][c] = (r0 + rs < m && c < b) ? v[q] : 0.0f;
         }
     }
+    // the slab's pending update: wave w on rows 16w .. 16w+15
+    if constexpr (NPU > 0) {
+        __syncthreads();
+        if constexpr (NPU == 2)
+            panel_slab_update_lds(sB, reinterpret_cast<unsigned short*>(sU), upd, ps, ldk, uk, b, m, r0, t, w, m16, g);
+        else
+            panel_slab_update<NPU>(sB, upd, ps, ldk, uk, b, m, r0, rw, m16, g);
+        __syncthreads();  // the staging area becomes L11's blocks
+    }
+    // block e / 1024: row e % 32 (lanes along the column), column (e / 32) % 32
+    // (clamped loads, then select)
+    for (int e0 = 0; e0 < 6 * 1024; e0 += 32 * 256) {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int e = e0 + t + 256 * q;
+            const int blk = e >> 10, rr = e & 31, cc = (e >> 5) & 31;
+            const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
+            const int r = 32 * p + rr, c = 32 * qb + cc;
+            v[q] = (e < 6 * 1024) ? L[(r < b ? r : b - 1) + (i64)(c < b ? c : b - 1) * ldl] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int e = e0 + t + 256 * q;
+            const int blk = e >> 10, rr = e & 31, cc = (e >> 5) & 31;
+            const int p = blk < 1 ? 1 : blk < 3 ? 2 : 3, qb = blk - p * (p - 1) / 2;
+            const int r = 32 * p + rr, c = 32 * qb + cc;
+            if (e < 6 * 1024) sLb[blk][rr][cc] = (r < b && c < b) ? v[q] : 0.0f;
+        }
+    }
     for (int e = t; e < 4 * 1024; e += 256) sXd[e >> 10][(e >> 5) & 31][e & 31] = Xd[e];
     __syncthreads();
-    // wave w: rows 16w .. 16w+15 of the slab, pieces p = 0..3 in order
-    const int rw = 16 * w;
+    // wave w: rows 16w .. 16w+15 of the slab, pieces p = 0..3 in order (no
+    // other wave touches these rows)
     for (int p = 0; p < 4; ++p) {
         f4_t acc[2] = {};
         for (int q = 0; q < p; ++q) {  // S = sum_q X_q L_pq^T
@@ -368,10 +533,20 @@ hipError_t launch_potrf_f32(int b, float* A, i64 lda, float* Xd, int* info, int 
 
 hipError_t launch_panel_f32(int m, int b, const float* L, i64 ldl, const float* Xd, float* B, i64 ldb,
                             const int* info, int limit, unsigned short* planes, int np, i64 ps, i64 ldk,
-                            hipStream_t s) {
+                            hipStream_t s, const unsigned short* upd, int uk) {
     if (m <= 0 || b <= 0) return hipSuccess;
-    hipLaunchKernelGGL(panel_f32_kernel, dim3((unsigned)ceil_div(m, 64)), dim3(256), 0, s, m, b, L, ldl, Xd, B, ldb,
-                       info, limit, planes, np, ps, ldk);
+    const dim3 grid((unsigned)ceil_div(m, 64));
+    if (upd == nullptr || uk <= 0)
+        hipLaunchKernelGGL(panel_f32_kernel<0>, grid, dim3(256), 0, s, m, b, L, ldl, Xd, B, ldb, info, limit, planes,
+                           np, ps, ldk, upd, 0);
+    else if (np == 2)
+        hipLaunchKernelGGL(panel_f32_kernel<2>, grid, dim3(256), 0, s, m, b, L, ldl, Xd, B, ldb, info, limit, planes,
+                           np, ps, ldk, upd, uk);
+    else if (np == 3)
+        hipLaunchKernelGGL(panel_f32_kernel<3>, grid, dim3(256), 0, s, m, b, L, ldl, Xd, B, ldb, info, limit, planes,
+                           np, ps, ldk, upd, uk);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
