@@ -64,15 +64,10 @@ hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, 
                                       i64 ldk, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,
                                       hipStream_t s, int chain = 0);
 
-// One block column (ncol <= 128 columns) from the planes: C (m x ncol;
-// lower part of its top ncol x ncol block and all rows below) -= P P_top^T,
-// P = rows 0..m-1 of the planes, k columns (the group's earlier panels);
-// only rows [rbeg, rend) (rend >= min(m, ncol)).
-hipError_t launch_syrk_col_bf16_planes(int np, int m, int ncol, int rbeg, int rend, int k, const unsigned short* planes,
-                                       i64 ps, i64 ldk, float* C, i64 ldc, const int* info, int kcol0, int nbp,
-                                       hipStream_t s);
-// Its diagonal block alone (ncol <= 128), with many workgroups: C (ncol x
-// ncol, lower) -= P P^T, P = the first ncol rows of the planes.
+// The in-group update of one block column's diagonal block (ncol <= 128),
+// with many workgroups: C (ncol x ncol, lower) -= P P^T, P = the first ncol
+// rows of the planes, k columns (the group's earlier panels); the rows below
+// it take theirs inside launch_panel_f32.
 hipError_t launch_syrk_diag_bf16_planes(int np, int ncol, int k, const unsigned short* planes, i64 ps, i64 ldk,
                                         float* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
 

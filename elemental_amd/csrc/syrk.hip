@@ -162,98 +162,6 @@ hipError_t launch_syrk_ln_bf16_planes(int np, int m, int n, int k, float alpha, 
     return hipGetLastError();
 }
 
-// One block column of the trailing matrix from pre-split planes (the
-// in-group update of the float factorization): C (m x ncol, ncol <= 128;
-// rows r >= c of its top ncol x ncol block and every row below; rows
-// rbeg <= r < m of it) := C - P P_top^T, P = the m x k
-// panel rows of the NP planes (row-major, ld ldk, plane stride ps; P_top = its
-// first 128 rows).  One 64-row slab per workgroup, wave w owns 16 rows and
-// all 128 columns; the fragments are 16-byte loads straight from the planes
-// into registers (the planes' row-major order is the MFMA operand order),
-// the next 32-wide k chunk loading while the current one multiplies.  The
-// accumulators start from C and the slab's operand is negated (sign bits).
-template <int NP>
-__global__ __launch_bounds__(256) void syrk_col_bf16p_kernel(int m, int k, const unsigned short* __restrict__ planes,
-                                                             i64 ps, i64 ldk, int ncol, int rbeg, float* __restrict__ C,
-                                                             i64 ldc, const int* __restrict__ info, int kcol0,
-                                                             int nbp) {
-    __builtin_amdgcn_s_setprio(3);  // chain kernel: issues ahead of the trailing update's waves
-    const int ke = syrk_valid_k(info, k, kcol0, nbp);
-    if (info != nullptr && ke == 0) return;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, m16 = lane & 15, g = lane >> 4;
-    const int r0 = rbeg + blockIdx.x * 64 + 16 * w;            // this wave's rows
-    const int ra = r0 + m16 < m ? r0 + m16 : m - 1;            // A-operand row (clamped; its outputs are dropped)
-    const unsigned short* a_p = planes + (i64)ra * ldk + 8 * g;
-    const s8_t neg = {(short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000,
-                      (short)0x8000, (short)0x8000, (short)0x8000, (short)0x8000};
-    f4_t acc[8];
-#pragma unroll
-    for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = r0 + 4 * g + v, c = 16 * nt + m16;
-            acc[nt][v] = (r < m && c < ncol && (r >= ncol || c <= r)) ? C[r + (i64)c * ldc] : 0.0f;
-        }
-    s8_t fa[2][NP], fb[2][8][NP];  // [buffer][plane], [buffer][column tile][plane]
-    auto load = [&](int buf, int kc) {
-#pragma unroll
-        for (int p = 0; p < NP; ++p) fa[buf][p] = *reinterpret_cast<const s8_t*>(a_p + p * ps + kc) ^ neg;
-#pragma unroll
-        for (int nt = 0; nt < 8; ++nt) {
-            const i64 rb = (i64)(16 * nt + m16 < m ? 16 * nt + m16 : m - 1) * ldk + 8 * g + kc;
-#pragma unroll
-            for (int p = 0; p < NP; ++p) fb[buf][nt][p] = *reinterpret_cast<const s8_t*>(planes + p * ps + rb);
-        }
-    };
-    auto step = [&](int buf) {
-#pragma unroll
-        for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-            for (int o = NP - 1; o >= 0; --o)  // term order o = p + q, smallest first
-#pragma unroll
-                for (int p = o; p >= 0; --p) mfma_acc_bf16(acc[nt], fa[buf][p], fb[buf][nt][o - p]);
-    };
-    // ldk pads the planes with zeros up to a multiple of 32, so whole chunks are read
-    const int nchunks = (ke + 31) / 32;
-    load(0, 0);
-    int c = 0;
-    for (; c + 2 <= nchunks; c += 2) {
-        if (c + 1 < nchunks) load(1, 32 * (c + 1));
-        step(0);
-        if (c + 2 < nchunks) load(0, 32 * (c + 2));
-        step(1);
-    }
-    if (c < nchunks) step(0);
-    mfma_retire();
-#pragma unroll
-    for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = r0 + 4 * g + v, cc = 16 * nt + m16;
-            if (r < m && cc < ncol && (r >= ncol || cc <= r)) C[r + (i64)cc * ldc] = acc[nt][v];
-        }
-}
-
-hipError_t launch_syrk_col_bf16_planes(int np, int m, int ncol, int rbeg, int rend, int k, const unsigned short* planes,
-                                       i64 ps, i64 ldk, float* C, i64 ldc, const int* info, int kcol0, int nbp,
-                                       hipStream_t s) {
-    // rows [rbeg, rend) of the m-row block column; rend >= min(m, ncol) (the
-    // top block's rows are the B operand)
-    rend = std::min(rend, m);
-    if (m <= 0 || k <= 0 || ncol <= 0 || rbeg >= rend) return hipSuccess;
-    if (ncol > 128 || rbeg < 0 || rend < std::min(m, ncol)) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)ceil_div(rend - rbeg, 64));
-    if (np == 2)
-        hipLaunchKernelGGL(syrk_col_bf16p_kernel<2>, grid, dim3(256), 0, s, rend, k, planes, ps, ldk, ncol, rbeg, C,
-                           ldc, info, kcol0, nbp);
-    else if (np == 3)
-        hipLaunchKernelGGL(syrk_col_bf16p_kernel<3>, grid, dim3(256), 0, s, rend, k, planes, ps, ldk, ncol, rbeg, C,
-                           ldc, info, kcol0, nbp);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
-}
-
 // The diagonal block alone: C (ncol x ncol, lower) -= P P^T, P = the first
 // ncol rows of the planes, k columns -- the update the panel step waits for.
 // One 16 x 16 output tile per workgroup (the lower ones), its four waves
