@@ -21,7 +21,7 @@ ARCH = os.environ.get("ELEMAMD_ARCH", "gfx950")
 
 SOURCES = [
     "potrf.hip", "potrf_f32.hip",
-    "trsm_panel.hip",
+    "trsm_panel.hip", "panel_f64.hip",
     "syrk.hip",
     "gemm.hip",
     "inverse.hip", "hpd_inverse.cpp", "gen_eig.cpp",

@@ -240,7 +240,7 @@ int elemamd_hpd_solve_f64(char uplo, char orientation, int n, int k, double* A, 
     // transpose back
     if (upper && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
     int rc = cholesky_lower<double>(n, A, lda, nb, kTrailingF64, dinfo, s);
-    if (rc == kOk) rc = solve_after_lower<double>(n, k, A, lda, B, ldb, dinfo, work, s);
+    if (rc == kOk) rc = solve_after_lower<double>(n, k, A, lda, B, ldb, dinfo, work, s, false, true);
     if (upper && launch_transpose_inplace_f64(n, A, lda, s) != hipSuccess) return ELEMAMD_ERR_HIP;
     if (rc || n == 0 || k == 0) return rc;
     return solve_read_error(solve_ctrl<double>(n, work), s);

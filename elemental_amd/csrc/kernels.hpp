@@ -21,11 +21,12 @@ template <typename T>
 hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
                            hipStream_t s);
 
-// The same solve in 32-column sub-blocks (MFMA products between them): the
-// panel solve of the blocked factorization.
-template <typename T>
-hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
-                                   hipStream_t s);
+
+// panel_f64.hip -- the FP64 factorization's panel solve B := B L^{-T}
+// (b <= 128): substitution inside 32-column pieces, FP64 MFMA products
+// between them, rows in registers (not bit-identical to launch_trsm_rlt).
+hipError_t launch_panel_f64(int m, int b, const double* L, i64 ldl, double* B, i64 ldb, const int* info, int limit,
+                            hipStream_t s);
 
 // potrf_f32.hip -- panel step of the float-storage factor: the b x b
 // (b <= 128) diagonal block factored in place (FP64 on 32 x 32 pieces, whose
@@ -44,8 +45,11 @@ hipError_t launch_panel_f32(int m, int b, const float* L, i64 ldl, const float* 
 // syrk.hip -- C := beta*C + alpha*A*B^T on the entries j <= i of C (m x n, n <= m).
 // With info != null the k columns are factor columns [kcol0, kcol0 + k) and a
 // recorded failure drops the panels of its nbp-block and later ones.
+// beside_chain: the launch runs concurrently with the factorization's chain
+// kernels; its workgroups reserve extra LDS so that room for them stays.
 hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
-                              double beta, double* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s);
+                              double beta, double* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s,
+                              bool beside_chain = false);
 // float storage; bf16x3 = false: FP32 MFMA, true: split-BF16x3 MFMA with FP32 accumulation.
 hipError_t launch_syrk_ln_f32(bool bf16x3, int m, int n, int k, float alpha, const float* A, i64 lda, const float* B,
                               i64 ldb, float beta, float* C, i64 ldc, const int* info, int kcol0, int nbp,

@@ -44,10 +44,10 @@ namespace elemamd {
 //    state when it throws, up to the rounding order of the updates.
 template <typename T>
 static hipError_t syrk_launch(int trailing, int m, int n, int k, const T* A, i64 lda, T* C, i64 ldc, const int* info,
-                              int kcol0, int nbp, hipStream_t s) {
+                              int kcol0, int nbp, hipStream_t s, bool beside_chain = false) {
     if constexpr (sizeof(T) == 8)
         return launch_syrk_ln_f64(m, n, k, -1.0, (const double*)A, lda, (const double*)A, lda, 1.0, (double*)C, ldc,
-                                  info, kcol0, nbp, s);
+                                  info, kcol0, nbp, s, beside_chain);
     else
         return launch_syrk_ln_f32(trailing == kTrailingBf16x3, m, n, k, -1.0f, (const float*)A, lda, (const float*)A,
                                   lda, 1.0f, (float*)C, ldc, info, kcol0, nbp, s);
@@ -141,7 +141,8 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
             if constexpr (sizeof(T) == 8) {
                 ELEMAMD_HIP_TRY(launch_potrf_unb<T>(b, Akk, lda, dinfo, k, s));
                 if (n - k - b > 0)
-                    ELEMAMD_HIP_TRY(launch_trsm_rlt_blocked<T>(n - k - b, b, Akk, lda, Akk + b, lda, dinfo, k + b, s));
+                    ELEMAMD_HIP_TRY(launch_panel_f64(n - k - b, b, (const double*)Akk, lda, (double*)Akk + b, lda,
+                                                     dinfo, k + b, s));
             } else {  // the low-precision factor: diagonal inverse, panel as a product with it
                 ELEMAMD_HIP_TRY(launch_potrf_f32(b, (float*)Akk, lda, Xd, dinfo, k, s));
                 if (n - k - b > 0)
@@ -175,7 +176,7 @@ int cholesky_lower(int n, T* A, i64 lda, int nb, int trailing, int* dinfo, hipSt
                                                            sp->aux));
             else
                 ELEMAMD_HIP_TRY(syrk_launch<T>(trailing, m2 - bn, m2 - bn, kg, P + bn, lda, C + bn + (i64)bn * lda,
-                                               lda, dinfo, c0, nb, sp->aux));
+                                               lda, dinfo, c0, nb, sp->aux, true));
             ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_b, sp->aux));
         }
         rest_pending = rest;
@@ -205,11 +206,6 @@ template <typename T>
 size_t solve_work_bytes(int n, int k) {
     return align256(solve_dinv_count(n) * sizeof(T)) + align256(solve_ctrl_count(n, k) * sizeof(int));
 }
-
-// The FP64 solves substitute inside the diagonal blocks (no inverses); the
-// float solves of the refinement use the precomputed inverses.
-template <typename T>
-static constexpr bool kSolveUsesInverse = (sizeof(T) == 4);
 
 template <typename T>
 int solve_prepare(int n, const T* L, i64 ldl, T* Dinv, hipStream_t s) {
@@ -245,13 +241,14 @@ int* solve_ctrl(int n, void* work) {
 
 template <typename T>
 int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s,
-                      bool upper) {
+                      bool upper, bool inverse_blocks) {
     if (n < 0 || k < 0 || ldl < std::max(1, n) || ldb < std::max(1, n)) return kErrArg;
     if (n == 0 || k == 0) return kOk;
     if (work == nullptr) return kErrWorkspace;
-    T* Dinv = kSolveUsesInverse<T> ? (T*)work : nullptr;
+    const bool inv = inverse_blocks && !upper;
+    T* Dinv = inv ? (T*)work : nullptr;
     int* ctrl = solve_ctrl<T>(n, work);
-    if (kSolveUsesInverse<T>) {
+    if (inv) {
         int rc = solve_prepare<T>(n, L, ldl, Dinv, s);
         if (rc) return rc;
     }
@@ -269,8 +266,9 @@ template int solve_apply<float>(bool, bool, int, int, const float*, i64, float*,
 template int* solve_ctrl<double>(int, void*);
 template int* solve_ctrl<float>(int, void*);
 template int solve_after_lower<double>(int, int, const double*, i64, double*, i64, const int*, void*, hipStream_t,
-                                       bool);
-template int solve_after_lower<float>(int, int, const float*, i64, float*, i64, const int*, void*, hipStream_t, bool);
+                                       bool, bool);
+template int solve_after_lower<float>(int, int, const float*, i64, float*, i64, const int*, void*, hipStream_t, bool,
+                                      bool);
 
 // ------------------------------------------------------ mixed precision
 namespace {
@@ -841,7 +839,7 @@ static int fallback_fp64(int n, int k, const double* A, i64 lda, double* B, i64 
     r.info = info;
     if (info != 0) return kErrNotConverged;
     ELEMAMD_HIP_TRY(launch_copy_f64(n, k, fw.B0, n, B, ldb, s));
-    rc = solve_after_lower<double>(n, k, fw.Af, n, B, ldb, nullptr, fw.solve, s);
+    rc = solve_after_lower<double>(n, k, fw.Af, n, B, ldb, nullptr, fw.solve, s, false, true);
     if (rc) return rc;
     rc = solve_read_error(solve_ctrl<double>(n, fw.solve), s);
     if (rc) return rc;

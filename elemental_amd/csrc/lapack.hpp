@@ -61,10 +61,14 @@ template <typename T>
 size_t solve_work_bytes(int n, int k);
 // upper: the factor is stored in the upper triangle as U = L^T
 // (cholesky::SolveAfter(UPPER, ...): the forward solve with U^T, the backward
-// one with U).
+// one with U).  inverse_blocks: the 64 x 64 diagonal blocks are applied as
+// products with their inverses (computed first) instead of by substitution
+// (always for float; for double the faster form that HPDSolve uses, while
+// the SolveAfter entry point keeps the substitution, exact on integer
+// problems); lower-stored factors only.
 template <typename T>
 int solve_after_lower(int n, int k, const T* L, i64 ldl, T* B, i64 ldb, const int* dinfo, void* work, hipStream_t s,
-                      bool upper = false);
+                      bool upper = false, bool inverse_blocks = sizeof(T) == 4);
 // The control words of solve_after_lower inside its `work` buffer.
 template <typename T>
 int* solve_ctrl(int n, void* work);

@@ -299,7 +299,7 @@ void Server::worker(int slot) {
                 res.info = info;
                 if (rc == kOk && info != 0) rc = kServeNotHpd;
             }
-            if (rc == kOk) rc = solve_after_lower<double>(q.n, q.k, L, q.n, q.B, q.ldb, nullptr, work, s);
+            if (rc == kOk) rc = solve_after_lower<double>(q.n, q.k, L, q.n, q.B, q.ldb, nullptr, work, s, false, true);
             // synchronizes; a solve whose bounded wait gave up is an error
             if (rc == kOk && q.n > 0 && q.k > 0) rc = solve_read_error(solve_ctrl<double>(q.n, work), s);
             if (rc == kOk && hipStreamSynchronize(s) != hipSuccess) rc = kErrHip;

@@ -82,9 +82,13 @@ static dim3 syrk_grid(int m, int n) {
 }
 
 hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A, i64 lda, const double* B, i64 ldb,
-                              double beta, double* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s) {
+                              double beta, double* C, i64 ldc, const int* info, int kcol0, int nbp, hipStream_t s,
+                              bool beside_chain) {
     if (m <= 0 || n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(syrk_ln_f64_kernel, syrk_grid(m, n), dim3(256), 0, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
+    // beside the chain: 56 KB of LDS per workgroup (36 KB used), so at most
+    // two share a CU and 48 KB stay free for the chain's kernels
+    const unsigned pad = beside_chain ? 20u * 1024u : 0u;
+    hipLaunchKernelGGL(syrk_ln_f64_kernel, syrk_grid(m, n), dim3(256), pad, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
                        ldc, info, kcol0, nbp);
     return hipGetLastError();
 }

@@ -33,11 +33,14 @@ __device__ inline bool syrk_tile(int m, int n, int& ti, int& tj) {
 }
 
 // ---------------------------------------------------------------- FP64
-// K chunks of 16 (two workgroups per CU: 2 x 68 KB of LDS; chunks of 32 with
-// one workgroup per CU measured 10-15% slower).  With beta == 1 (the
-// factorization) the accumulators start from C, read at the start under the
-// first chunk's loads, and the tile is only stored at the end.
-constexpr int kSyrkKC64 = 16;
+// K chunks of 8, double-buffered: 36 KB of LDS per workgroup, so the
+// factorization's chain kernels fit on a CU beside two of them (the launches
+// beside the chain reserve extra LDS so that no third one is placed: see
+// launch_syrk_ln_f64; chunks of 32 with one workgroup per CU measured 10-15%
+// slower).  With beta == 1 (the factorization) the accumulators start from
+// C, read at the start under the first chunk's loads, and the tile is only
+// stored at the end.
+constexpr int kSyrkKC64 = 8;
 
 __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,

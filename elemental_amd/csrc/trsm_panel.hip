@@ -26,30 +26,6 @@ __global__ __launch_bounds__(256) void trsm_rlt_kernel(int m, int nb, const T* _
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_rlt_blocked_kernel(int m, int nb, const T* __restrict__ L, i64 ldl,
-                                                               T* __restrict__ B, i64 ldb,
-                                                               const int* __restrict__ info, int limit) {
-    // the factorization's chain: its waves issue ahead of the trailing
-    // update's waves sharing the SIMD (wave priority, not queue priority)
-    __builtin_amdgcn_s_setprio(3);
-    trsm_rlt_blocked_rows<T>(m, nb, L, ldl, B, ldb, info, limit, blockIdx.x);
-}
-
-template <typename T>
-hipError_t launch_trsm_rlt_blocked(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
-                                   hipStream_t s) {
-    if (m <= 0 || nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(trsm_rlt_blocked_kernel<T>, dim3((unsigned)ceil_div(m, kTrsmRowsPerBlock)), dim3(256), 0, s, m,
-                       nb, L, ldl, B, ldb, info, limit);
-    return hipGetLastError();
-}
-
-template hipError_t launch_trsm_rlt_blocked<double>(int, int, const double*, i64, double*, i64, const int*, int,
-                                                    hipStream_t);
-template hipError_t launch_trsm_rlt_blocked<float>(int, int, const float*, i64, float*, i64, const int*, int,
-                                                   hipStream_t);
-
-template <typename T>
 hipError_t launch_trsm_rlt(int m, int nb, const T* L, i64 ldl, T* B, i64 ldb, const int* info, int limit,
                            hipStream_t s) {
     if (m <= 0 || nb <= 0) return hipSuccess;

@@ -368,7 +368,9 @@ def test_mixed_ir_correction_sweeps_shapes(oracle, n, k):
 def test_full_size_integer_factor_and_solve_exact():
     """n = 16384: the factor of A = L0 L0^T must be L0 exactly and the
     solution of A X = A X0 must be X0 exactly (all intermediates are
-    integers below 2^53)."""
+    integers below 2^53).  Cholesky + SolveAfter (substitution inside the
+    diagonal blocks); HPDSolve's solves multiply by the blocks' inverses,
+    whose entries grow far past 2^53 for these integer factors."""
     n, k = 16384, 32
     A, L0 = inputs.integer_device(n, seed=5)
     g = torch.Generator(device="cuda")
@@ -377,7 +379,8 @@ def test_full_size_integer_factor_and_solve_exact():
     B = Matrix(n, k)
     B.tensor().copy_(A.tensor() @ X0)
     upper_before = torch.triu(A.tensor(), 1).clone()
-    E.HPDSolve(E.LOWER, E.NORMAL, A, B)
+    E.Cholesky(E.LOWER, A)
+    E.SolveAfter(E.LOWER, E.NORMAL, A, B)
     torch.cuda.synchronize()
     assert torch.equal(torch.tril(A.tensor()), L0.tensor())
     assert torch.equal(torch.triu(A.tensor(), 1), upper_before)
