@@ -106,6 +106,7 @@ SIGNATURES = {
     "elemamd_fill_normal_f64": (_i, [_vp, _i64, _i64, _i64, _u64, _i64, _vp]),
     "elemamd_sym_residual_f64": (_i, [_i, _i, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     "elemamd_batch_plan_create": (_i, [_i, _vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "elemamd_batch_set_launch": (_i, [_vp, _vp]),
     "elemamd_batch_plan_destroy": (_i, [_vp]),
     "elemamd_batch_plan_layout": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "elemamd_batch_set_refinement": (_i, [_vp, _i, _d]),

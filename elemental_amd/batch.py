@@ -24,9 +24,11 @@ from ._lib import lib
 F64, F32_IR, BF16X3_IR = 0, 1, 2
 PRECISIONS = {"f64": F64, "f32": F32_IR, "bf16x3": BF16X3_IR}
 
-# size buckets (upper bounds of n) and the block size used when no tuned
-# value is cached
-DEFAULT_BUCKETS = [(64, 64), (128, 128), (256, 64), (512, 128), (1 << 30, 128)]
+# size buckets: (upper bound of n, block size, launch form) used when no tuned
+# value is cached; launch form 0 = one launch per block step and kernel,
+# g > 0 = one persistent launch of g workgroups (block size <= 64)
+DEFAULT_BUCKETS = [(64, 64, 0), (128, 128, 0), (256, 64, 0), (512, 128, 0), (1 << 30, 128, 0)]
+PERSIST_MAX_NB = 64
 TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "batch_gfx950.json")
 
 
@@ -41,14 +43,18 @@ def _ptr(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _bucket3(b):
+    m, nb = int(b[0]), int(b[1])
+    return (m, nb, int(b[2]) if len(b) > 2 else 0)
+
+
 def tuned_buckets(precision: str = "f64", path: str = TUNING_FILE):
-    """(max_n, nb) pairs from the autotuner's cache, else the defaults."""
+    """(max_n, nb, grid) triples from the autotuner's cache, else the defaults."""
     try:
         with open(path) as f:
             data = json.load(f)
-        b = data[precision]["buckets"]
-        return [(int(m), int(nb)) for m, nb in b]
-    except (OSError, KeyError, ValueError):
+        return [_bucket3(b) for b in data[precision]["buckets"]]
+    except (OSError, KeyError, ValueError, IndexError, TypeError):
         return list(DEFAULT_BUCKETS)
 
 
@@ -63,9 +69,10 @@ class BatchPlan:
         if self.n.shape != self.k.shape:
             raise ValueError("n and k must have the same length")
         self.precision = precision
-        self.buckets = list(buckets) if buckets is not None else tuned_buckets(precision)
-        bmax = np.array([m for m, _ in self.buckets], dtype=np.int32)
-        bnb = np.array([nb for _, nb in self.buckets], dtype=np.int32)
+        self.buckets = [_bucket3(b) for b in buckets] if buckets is not None else tuned_buckets(precision)
+        bmax = np.array([m for m, _, _ in self.buckets], dtype=np.int32)
+        bnb = np.array([nb for _, nb, _ in self.buckets], dtype=np.int32)
+        bgrid = np.array([g for _, _, g in self.buckets], dtype=np.int32)
         h = ctypes.c_void_p()
         rc = lib().elemamd_batch_plan_create(len(self.n), self.n.ctypes.data_as(ctypes.c_void_p),
                                              self.k.ctypes.data_as(ctypes.c_void_p), PRECISIONS[precision],
@@ -74,6 +81,11 @@ class BatchPlan:
         if rc != 0:
             raise ValueError(f"elemamd_batch_plan_create failed ({rc})")
         self._h = h
+        rc = lib().elemamd_batch_set_launch(h, bgrid.ctypes.data_as(ctypes.c_void_p))
+        if rc != 0:
+            self.close()
+            raise ValueError(f"elemamd_batch_set_launch failed ({rc}): a persistent bucket needs nb <= "
+                             f"{PERSIST_MAX_NB}")
         cnt = len(self.n)
         self.a_off = np.zeros(cnt, dtype=np.int64)
         self.b_off = np.zeros(cnt, dtype=np.int64)

@@ -108,12 +108,11 @@ __device__ inline float bcast_lane<float>(float v, int lane) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void batch_solve_kernel(BatchArgs a, const T* L_all, T* B_all) {
+__device__ __forceinline__ void batch_solve_one(const BatchArgs& a, const T* L_all, T* B_all, int p) {
     __shared__ T sL[kBS][kBS + 1];   // contraction tile / diagonal block
     __shared__ T sX[kBS][kBC + 1];   // solved block / right-hand sides
     __shared__ T sR[kBS][kBC + 1];   // block being solved
     __shared__ T sRinv[kBS];
-    const int p = blockIdx.x;
     if (a.info[p] != 0) return;
     if (a.skip != nullptr && a.skip[p]) return;
     const int n = a.n[p], k = a.k[p];
@@ -221,6 +220,87 @@ __global__ __launch_bounds__(256) void batch_solve_kernel(BatchArgs a, const T* 
                 __threadfence_block();
                 __syncthreads();
             }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void batch_solve_kernel(BatchArgs a, const T* L_all, T* B_all) {
+    batch_solve_one<T>(a, L_all, B_all, blockIdx.x);
+}
+
+// ------------------------------------------------------ persistent launch
+// One launch per size bucket: every workgroup takes the bucket's matrices in
+// processing order (largest first) from a counter and factors each one whole
+// with the device code of the stepwise launches above -- LVar3Unb on the
+// diagonal block (256 threads, the same operation order), the bit-exact
+// panel sweep, the trailing tiles in the same K order -- so a matrix's factor
+// is the same bit for bit in either form; for FP64 it then solves the
+// matrix's right-hand sides.  Workgroups never wait on each other.  The
+// block size is at most kPersistMaxNb = 64 here (the panel's L11 then takes
+// 32 KB of LDS).
+
+// The phases are separate (not inlined) functions: each gets the registers
+// it needs on its own (inlined together they spilled).
+template <typename T>
+__device__ __noinline__ void persist_factor_block(int b, T* L, int n, int* info, int c0) {
+    potrf_unb_block<T, 256>(b, L, n, info, c0);
+}
+
+template <typename T>
+__device__ __noinline__ void persist_panel(int m, int b, T* L, int n) {
+    for (int rb = 0; rb < (m + 63) / 64; ++rb) {
+        trsm_rlt_rows<T, false, kPersistMaxNb>(m, b, L, n, L + b, n, nullptr, 0, rb);
+        __syncthreads();
+    }
+}
+
+template <typename T, int MODE>
+__device__ __noinline__ void persist_trailing(int m, int b, const T* P, T* C, int n) {
+    const int nt = (m + 127) / 128;
+    for (int tt = 0; tt < nt * (nt + 1) / 2; ++tt) {
+        int ti, tj;
+        tri_tile(tt, ti, tj);
+        if constexpr (MODE == 0)
+            syrk_tile_f64(m, m, b, -1.0, (const double*)P, n, (const double*)P, n, 1.0, (double*)C, n, ti, tj);
+        else if constexpr (MODE == 1)
+            syrk_tile_f32(m, m, b, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj);
+        else
+            syrk_tile_bf16x3(m, m, b, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj);
+        __syncthreads();
+    }
+}
+
+__device__ __noinline__ void persist_solve(const BatchArgs& a, const double* A, double* B, int p) {
+    batch_solve_one<double>(a, A, B, p);
+}
+
+template <typename T, int MODE, bool SOLVE>
+__global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a, T* A, double* B, int nb, int* counter) {
+    __shared__ int s_p;
+    for (;;) {
+        __syncthreads();  // every thread is done with the previous matrix
+        if (threadIdx.x == 0) s_p = atomicAdd(counter, 1);
+        __syncthreads();
+        const int p = s_p;
+        if (p >= a.count) return;
+        const int n = a.n[p];
+        T* M = A + a.a_off[p];
+        int* info = a.info + p;
+        for (int c0 = 0; c0 < n; c0 += nb) {
+            const int b = min(nb, n - c0);
+            T* L = M + c0 + (i64)c0 * n;
+            persist_factor_block<T>(b, L, n, info, c0);
+            __syncthreads();  // the block and info are written (same CU: visible)
+            if (*info != 0) break;
+            const int m = n - c0 - b;
+            if (m <= 0) break;
+            persist_panel<T>(m, b, L, n);
+            persist_trailing<T, MODE>(m, b, L + b, M + (c0 + b) + (i64)(c0 + b) * n, n);
+        }
+        if constexpr (SOLVE) {
+            __syncthreads();
+            persist_solve(a, (const double*)A, B, p);
         }
     }
 }
@@ -472,6 +552,23 @@ hipError_t batch_syrk(const BatchArgs& a, void* A, int mode, int step, int nb, c
     else
         hipLaunchKernelGGL((batch_syrk_kernel<float, 2>), dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb, prefix,
                            cnt);
+    return hipGetLastError();
+}
+
+hipError_t batch_persistent(const BatchArgs& a, void* F, double* B, int mode, int nb, int* counter, int grid,
+                            hipStream_t s) {
+    if (a.count <= 0) return hipSuccess;
+    if (nb < 1 || nb > kPersistMaxNb || grid < 1) return hipErrorInvalidValue;
+    const dim3 g((unsigned)std::min(grid, a.count));
+    if (mode == 0)
+        hipLaunchKernelGGL((batch_persistent_kernel<double, 0, true>), g, dim3(256), 0, s, a, (double*)F, B, nb,
+                           counter);
+    else if (mode == 1)
+        hipLaunchKernelGGL((batch_persistent_kernel<float, 1, false>), g, dim3(256), 0, s, a, (float*)F, B, nb,
+                           counter);
+    else
+        hipLaunchKernelGGL((batch_persistent_kernel<float, 2, false>), g, dim3(256), 0, s, a, (float*)F, B, nb,
+                           counter);
     return hipGetLastError();
 }
 

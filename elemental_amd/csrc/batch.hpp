@@ -11,6 +11,8 @@ namespace elemamd {
 // Per-launch view of a group of matrices (processing order): sizes,
 // right-hand-side counts, element offsets into the packed buffers, factor
 // status and the refinement's "converged" flags (may be null).
+constexpr int kPersistMaxNb = 64;  // largest block size of the persistent launch
+
 struct BatchArgs {
     int count;
     const int* n;
@@ -27,6 +29,12 @@ hipError_t batch_trsm(const BatchArgs& a, void* A, bool f32, int step, int nb, c
 hipError_t batch_syrk(const BatchArgs& a, void* A, int mode, int step, int nb, const int* prefix, int cnt, int wgs,
                       hipStream_t s);
 hipError_t batch_solve(const BatchArgs& a, const void* L, void* B, bool f32, hipStream_t s);
+// The bucket's whole factorization (mode 0 FP64, 1 FP32, 2 BF16x3 trailing;
+// F = A, or the float copy) and, for FP64, its solves into B, as one launch
+// of `grid` workgroups that take the matrices from *counter (zeroed before).
+// nb <= kPersistMaxNb.
+hipError_t batch_persistent(const BatchArgs& a, void* F, double* B, int mode, int nb, int* counter, int grid,
+                            hipStream_t s);
 hipError_t batch_to_float(const BatchArgs& a, const double* A, float* S, hipStream_t s);
 // R := B - A X per matrix; eta (and rscale = ||r_c||_inf when non-null) per
 // matrix; prefix/wgs: 64-row blocks of the matrices; red: count*(1+3*kmax)
@@ -42,7 +50,9 @@ hipError_t batch_copy_rhs(const BatchArgs& a, const double* src, double* dst, fl
 
 
 struct BatchBucket {
+    int index = 0;  // position in the caller's bucket arrays
     int nb = 128;
+    int grid = 0;   // 0: one launch per step and kernel; > 0: one persistent launch of grid workgroups
     int first = 0, count = 0;  // processing positions [first, first + count)
     int steps = 0;
     int res_prefix_off = 0, res_wgs = 0;  // 64-row blocks of the residual
@@ -67,6 +77,7 @@ struct BatchPlan {
     int *d_n = nullptr, *d_k = nullptr, *d_prefix = nullptr;
     i64 *d_aoff = nullptr, *d_boff = nullptr;
     int *d_info = nullptr, *d_skip = nullptr, *d_iters = nullptr;
+    int* d_counters = nullptr;  // one per bucket: the persistent launches' matrix counters
     double *d_eta = nullptr, *d_rscale = nullptr, *d_red = nullptr;
     std::vector<void*> dev_allocs;
     // captured graph and the arguments it was captured with
@@ -81,6 +92,10 @@ struct BatchPlan {
 // size bucket_nb[b] (1..128).
 int batch_plan_create(int count, const int* n, const int* k, int precision, int nbuckets, const int* bucket_max_n,
                       const int* bucket_nb, BatchPlan** out);
+// Launch form per bucket (caller's bucket order): 0 = one launch per step
+// and kernel, g > 0 = one persistent launch of g workgroups (block size at
+// most kPersistMaxNb).
+int batch_set_launch(BatchPlan* P, const int* bucket_grid);
 size_t batch_workspace_bytes(const BatchPlan* P);
 int batch_hpd_solve(BatchPlan* P, double* A, double* B, void* work, size_t work_bytes, bool use_graph,
                     hipStream_t s);

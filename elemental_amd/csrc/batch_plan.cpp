@@ -99,6 +99,7 @@ int batch_plan_create(int count, const int* n, const int* k, int precision, int 
     int pos = 0;
     for (int b = 0; b < nbuckets; ++b) {
         BatchBucket B;
+        B.index = b;
         B.nb = bucket_nb[b];
         B.first = pos;
         while (pos < count && bk[P->order[pos]] == b) ++pos;
@@ -145,7 +146,7 @@ int batch_plan_create(int count, const int* n, const int* k, int precision, int 
     if (d_bo) P->dev_allocs.push_back(d_bo);
     ok = ok && upload(&d_pref, prefix_all) == hipSuccess;
     if (d_pref) P->dev_allocs.push_back(d_pref);
-    ok = ok && hipMalloc(&d_ints, sizeof(int) * (size_t)std::max(3 * count, 1)) == hipSuccess;
+    ok = ok && hipMalloc(&d_ints, sizeof(int) * (size_t)(3 * count + nbuckets)) == hipSuccess;
     if (d_ints) P->dev_allocs.push_back(d_ints);
     // eta, rscale (kmax per matrix), reduction scratch (1 + 3 kmax per matrix)
     ok = ok && hipMalloc(&d_dbl, sizeof(double) * (size_t)std::max(count * (2 + 4 * P->kmax), 1)) == hipSuccess;
@@ -162,10 +163,23 @@ int batch_plan_create(int count, const int* n, const int* k, int precision, int 
     P->d_info = d_ints;
     P->d_skip = d_ints + count;
     P->d_iters = d_ints + 2 * count;
+    P->d_counters = d_ints + 3 * count;
     P->d_eta = d_dbl;
     P->d_rscale = d_dbl + count;
     P->d_red = d_dbl + (size_t)count * (1 + P->kmax);
     *out = P;
+    return kOk;
+}
+
+int batch_set_launch(BatchPlan* P, const int* bucket_grid) {
+    if (P == nullptr || bucket_grid == nullptr) return kErrArg;
+    for (const BatchBucket& bk : P->buckets)
+        if (bucket_grid[bk.index] < 0 || (bucket_grid[bk.index] > 0 && bk.nb > kPersistMaxNb)) return kErrArg;
+    for (BatchBucket& bk : P->buckets) bk.grid = bucket_grid[bk.index];
+    if (P->graph_exec) {  // captured with the previous launch forms
+        (void)hipGraphExecDestroy(P->graph_exec);
+        P->graph_exec = nullptr;
+    }
     return kOk;
 }
 
@@ -179,7 +193,8 @@ size_t batch_workspace_bytes(const BatchPlan* P) {
 // The launch sequence: kernels only, fixed shapes, so it can be captured.
 static int enqueue(BatchPlan* P, double* A, double* B, void* work, hipStream_t s) {
     if (P->count == 0) return kOk;
-    ELEMAMD_HIP_TRY(launch_fill_int(P->d_info, 0, 3 * P->count, s));  // info, skip, iters
+    // info, skip, iters, the persistent launches' counters
+    ELEMAMD_HIP_TRY(launch_fill_int(P->d_info, 0, 3 * P->count + (int)P->buckets.size(), s));
     const bool f32 = P->precision != 0;
     const int mode = P->precision;  // trailing update: 0 FP64, 1 FP32, 2 BF16x3
     float* S = nullptr;
@@ -199,7 +214,12 @@ static int enqueue(BatchPlan* P, double* A, double* B, void* work, hipStream_t s
         const BatchArgs a = bucket_args(P, bk, false);
         if (f32) ELEMAMD_HIP_TRY(batch_to_float(a, A, S, s));
         void* F = f32 ? (void*)S : (void*)A;
-        for (int j = 0; j < bk.steps; ++j) {
+        if (bk.grid > 0) {  // the whole factorization (and FP64 solves) in one launch
+            ELEMAMD_HIP_TRY(batch_persistent(a, F, B, mode, bk.nb, P->d_counters + (&bk - P->buckets.data()),
+                                             bk.grid, s));
+            if (!f32) continue;
+        }
+        for (int j = 0; bk.grid == 0 && j < bk.steps; ++j) {
             const BatchBucket::Step& st = bk.step[j];
             ELEMAMD_HIP_TRY(batch_potrf(a, F, f32, j, bk.nb, st.active, s));
             ELEMAMD_HIP_TRY(
@@ -211,6 +231,7 @@ static int enqueue(BatchPlan* P, double* A, double* B, void* work, hipStream_t s
             ELEMAMD_HIP_TRY(batch_solve(a, A, B, false, s));
             continue;
         }
+        // (the refinement below follows either launch form)
         // low-precision factor + FP64-residual refinement from X = 0: a fixed
         // number of sweeps (the sequence is replayable); a converged matrix
         // is skipped by every kernel of the remaining sweeps

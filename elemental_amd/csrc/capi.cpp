@@ -472,6 +472,11 @@ int elemamd_batch_plan_create(int count, const int* n, const int* k, int precisi
     return ELEMAMD_OK;
 }
 
+int elemamd_batch_set_launch(elemamd_batch_plan* plan, const int* bucket_grid) {
+    if (plan == nullptr) return ELEMAMD_ERR_ARG;
+    return batch_set_launch(plan->p, bucket_grid);
+}
+
 int elemamd_batch_plan_destroy(elemamd_batch_plan* plan) {
     if (plan == nullptr) return ELEMAMD_ERR_ARG;
     delete plan->p;

@@ -290,6 +290,14 @@ typedef struct elemamd_batch_plan elemamd_batch_plan;
 int elemamd_batch_plan_create(int count, const int *n, const int *k, int precision, int nbuckets,
                               const int *bucket_max_n, const int *bucket_nb, elemamd_batch_plan **plan);
 int elemamd_batch_plan_destroy(elemamd_batch_plan *plan);
+/* Launch form of each size bucket (the create call's bucket order):
+ * bucket_grid[b] = 0 runs bucket b as one potrf, one panel and one trailing
+ * launch per block step (the default); g > 0 runs its whole factorization,
+ * and for ELEMAMD_BATCH_F64 its solves, as ONE persistent launch of g
+ * workgroups that take the bucket's matrices largest first (block size at
+ * most 64).  Results are the same bit for bit.  No reference counterpart
+ * (for callers factoring many small matrices). */
+int elemamd_batch_set_launch(elemamd_batch_plan *plan, const int *bucket_grid);
 /* Element offsets of every A_i / B_i and the total element counts. */
 int elemamd_batch_plan_layout(const elemamd_batch_plan *plan, int64_t *a_off, int64_t *b_off, int64_t *a_total,
                               int64_t *b_total);

@@ -8,6 +8,8 @@ blocks its panel solves differently) and with the oracle's HPDSolve
 (1e-12 relative); every matrix of the refinement modes reaches FP64 normwise
 backward error <= 1e-14; hipGraph replays equal the eager launch sequence;
 a non-HPD matrix reports its failing column and leaves the others intact.
+The persistent launch form (one launch per bucket, workgroups taking whole
+matrices) gives the stepwise form's results bit for bit.
 """
 import ctypes
 
@@ -174,3 +176,70 @@ def test_plan_rejects_bad_arguments():
         EB.BatchPlan([4], [1], buckets=[(64, 0)])
     with pytest.raises(ValueError):
         EB.BatchPlan([4], [1], buckets=[(64, 32), (32, 32)])
+
+
+# ------------------------------------------------------- persistent launch
+PERSIST_BUCKETS = [(96, 32), (400, 64), (1 << 30, 64)]
+
+
+def _grid(buckets, g):
+    return [(m, nb, g) for m, nb in buckets]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16x3"])
+@pytest.mark.parametrize("grid", [7, 256])
+def test_persistent_equals_stepwise(precision, grid):
+    """Same matrices, same block sizes: the persistent launch (fewer
+    workgroups than matrices, or more) against the stepwise launches."""
+    n, k = sizes(count=64, hi=700)
+    step, A1, B1 = make(n, k, precision, buckets=PERSIST_BUCKETS)
+    pers, A2, B2 = make(n, k, precision, buckets=_grid(PERSIST_BUCKETS, grid))
+    A0, B0 = A1.clone(), B1.clone()
+    for p in (step, pers):
+        p.set_refinement(6, 1e-14)
+    step.solve(A1, B1, use_graph=False)
+    pers.solve(A2, B2, use_graph=True)
+    torch.cuda.synchronize()
+    i1, e1, it1 = step.results()
+    i2, e2, it2 = pers.results()
+    assert (i1 == 0).all() and (i2 == 0).all()
+    assert torch.equal(A1, A2)
+    assert torch.equal(B1, B2)
+    assert np.array_equal(e1, e2, equal_nan=True) and np.array_equal(it1, it2)
+    # plain FP64 overwrote A with L: the backward error is taken against the input
+    assert pers.backward_error(A0, B2, B0).max() <= 1e-14
+
+
+def test_persistent_graph_replays_and_non_hpd_member():
+    n = np.array([40, 200, 130, 17, 600], dtype=np.int32)
+    k = np.array([2, 3, 1, 4, 8], dtype=np.int32)
+    b = [(1 << 30, 32, 3)]
+    plan, A, B = make(n, k, "f64", buckets=b)
+    bad, col = 1, 77
+    plan.a_view(A, bad)[col, col] = -5.0
+    A0, B0 = A.clone(), B.clone()
+    ok_plan, A2, B2 = make(n, k, "f64", buckets=b)
+    ok_plan.solve(A2, B2, use_graph=False)
+    plan.solve(A, B, use_graph=False)
+    torch.cuda.synchronize()
+    A_e, X_e = A.clone(), B.clone()
+    info, _, _ = plan.results()
+    assert info[bad] == col + 1
+    for i in range(len(n)):
+        if i != bad:
+            assert info[i] == 0
+            assert torch.equal(plan.b_view(B, i), ok_plan.b_view(B2, i))
+    for rep in range(3):
+        A.copy_(A0)
+        B.copy_(B0)
+        torch.cuda.synchronize()
+        plan.solve(A, B, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(B, X_e), f"replay {rep}"
+        assert torch.equal(A, A_e), f"replay {rep}"
+        assert plan.results()[0][bad] == col + 1
+
+
+def test_persistent_rejects_wide_blocks():
+    with pytest.raises(ValueError):
+        EB.BatchPlan([100, 300], [1, 2], buckets=[(1 << 30, 128, 64)])
