@@ -5,7 +5,7 @@ drawn log-uniformly inside the bucket, once per candidate shape -- the block
 size nb (the diagonal-block / panel width and the depth of the trailing
 update's 128 x 128 tiles) and the launch form: one potrf, panel and trailing
 launch per block step, or one persistent launch of 256, 512 or 1024
-workgroups that each take whole matrices (nb <= 64) -- and keeps the
+workgroups that each take whole matrices -- and keeps the
 fastest.  The result is cached as JSON (``batch.TUNING_FILE``), keyed by
 precision, as (max_n, nb, grid) triples that ``batch.tuned_buckets`` reads.
 

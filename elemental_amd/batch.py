@@ -26,9 +26,9 @@ PRECISIONS = {"f64": F64, "f32": F32_IR, "bf16x3": BF16X3_IR}
 
 # size buckets: (upper bound of n, block size, launch form) used when no tuned
 # value is cached; launch form 0 = one launch per block step and kernel,
-# g > 0 = one persistent launch of g workgroups (block size <= 64)
+# g > 0 = one persistent launch of g workgroups
 DEFAULT_BUCKETS = [(64, 64, 0), (128, 128, 0), (256, 64, 0), (512, 128, 0), (1 << 30, 128, 0)]
-PERSIST_MAX_NB = 64
+PERSIST_MAX_NB = 128
 TUNING_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "batch_gfx950.json")
 
 
@@ -84,8 +84,7 @@ class BatchPlan:
         rc = lib().elemamd_batch_set_launch(h, bgrid.ctypes.data_as(ctypes.c_void_p))
         if rc != 0:
             self.close()
-            raise ValueError(f"elemamd_batch_set_launch failed ({rc}): a persistent bucket needs nb <= "
-                             f"{PERSIST_MAX_NB}")
+            raise ValueError(f"elemamd_batch_set_launch failed ({rc})")
         cnt = len(self.n)
         self.a_off = np.zeros(cnt, dtype=np.int64)
         self.b_off = np.zeros(cnt, dtype=np.int64)

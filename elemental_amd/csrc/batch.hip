@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "batch.hpp"
+#include "chain64_dev.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 #include "panel_dev.hpp"
@@ -47,9 +48,12 @@ __global__ __launch_bounds__(512) void batch_potrf_kernel(BatchArgs a, T* A, int
     potrf_unb_block<T>(b, A + a.a_off[p] + c0 + (i64)c0 * n, n, a.info + p, c0);
 }
 
-template <typename T, bool FULL>
+template <typename T>
 __global__ __launch_bounds__(256) void batch_trsm_kernel(BatchArgs a, T* A, int step, int nb, const int* prefix,
                                                          int cnt) {
+    __shared__ double sDg[2][32][32];
+    __shared__ double sRinv[128];
+    __shared__ double sXq[4][16][34];
     const int p = find_item(prefix, cnt, blockIdx.x);
     const int rb = blockIdx.x - prefix[p];
     const int n = a.n[p];
@@ -57,8 +61,10 @@ __global__ __launch_bounds__(256) void batch_trsm_kernel(BatchArgs a, T* A, int 
     const int m = n - c0 - nb;
     if (m <= 0) return;  // (prefix gives such matrices no workgroups)
     if (a.skip != nullptr && a.skip[p]) return;
+    const int* info = a.info + p;
+    if (*info != 0 && *info <= c0 + nb) return;
     T* L = A + a.a_off[p] + c0 + (i64)c0 * n;
-    trsm_rlt_rows<T, FULL>(m, nb, L, n, L + nb, n, a.info + p, c0 + nb, rb);
+    chain64::panel64_slab<T>(m, nb, L, n, L + nb, n, rb, sDg, sRinv, sXq);
 }
 
 template <typename T, int MODE>
@@ -107,12 +113,11 @@ __device__ inline float bcast_lane<float>(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// LDS from the caller: sL (contraction tile / diagonal block), sX (solved
+// block / right-hand sides), sR (block being solved), sRinv.
 template <typename T>
-__device__ __forceinline__ void batch_solve_one(const BatchArgs& a, const T* L_all, T* B_all, int p) {
-    __shared__ T sL[kBS][kBS + 1];   // contraction tile / diagonal block
-    __shared__ T sX[kBS][kBC + 1];   // solved block / right-hand sides
-    __shared__ T sR[kBS][kBC + 1];   // block being solved
-    __shared__ T sRinv[kBS];
+__device__ __forceinline__ void batch_solve_one(const BatchArgs& a, const T* L_all, T* B_all, int p,
+                                                T (*sL)[kBS + 1], T (*sX)[kBC + 1], T (*sR)[kBC + 1], T* sRinv) {
     if (a.info[p] != 0) return;
     if (a.skip != nullptr && a.skip[p]) return;
     const int n = a.n[p], k = a.k[p];
@@ -226,19 +231,30 @@ __device__ __forceinline__ void batch_solve_one(const BatchArgs& a, const T* L_a
 
 template <typename T>
 __global__ __launch_bounds__(256) void batch_solve_kernel(BatchArgs a, const T* L_all, T* B_all) {
-    batch_solve_one<T>(a, L_all, B_all, blockIdx.x);
+    __shared__ T sL[kBS][kBS + 1];
+    __shared__ T sX[kBS][kBC + 1];
+    __shared__ T sR[kBS][kBC + 1];
+    __shared__ T sRinv[kBS];
+    batch_solve_one<T>(a, L_all, B_all, blockIdx.x, sL, sX, sR, sRinv);
 }
 
 // ------------------------------------------------------ persistent launch
 // One launch per size bucket: every workgroup takes the bucket's matrices in
 // processing order (largest first) from a counter and factors each one whole
 // with the device code of the stepwise launches above -- LVar3Unb on the
-// diagonal block (256 threads, the same operation order), the bit-exact
-// panel sweep, the trailing tiles in the same K order -- so a matrix's factor
-// is the same bit for bit in either form; for FP64 it then solves the
+// diagonal block (256 threads, the same operation order), the register
+// panel solve, the trailing tiles in the same K order -- so a matrix's
+// factor is the same bit for bit in either form; for FP64 it then solves the
 // matrix's right-hand sides.  Workgroups never wait on each other.  The
-// block size is at most kPersistMaxNb = 64 here (the panel's L11 then takes
-// 32 KB of LDS).
+// phases share one LDS arena (the panel, trailing and solve phases never
+// overlap), so two workgroups fit on a CU.
+constexpr int kArenaPanel = (2 * 32 * 32 + 128 + 4 * 16 * 34) * 8;
+constexpr int kArenaSyrk64 = 2 * 2 * kSyrkKC64 * kSyrkLdp * 8;
+constexpr int kArenaSyrk32 = 2 * 2 * kSyrkKC * kSyrkLdpF * 4;
+constexpr int kArenaBf16 = 4 * kSyrkTile * kBfLd * 2;
+constexpr int kArenaSolve = (kBS * (kBS + 1) + 2 * kBS * (kBC + 1) + kBS) * 8;
+constexpr int cmax(int x, int y) { return x > y ? x : y; }
+constexpr int kArenaBytes = cmax(cmax(kArenaPanel, kArenaSyrk64), cmax(cmax(kArenaSyrk32, kArenaBf16), kArenaSolve));
 
 // The phases are separate (not inlined) functions: each gets the registers
 // it needs on its own (inlined together they spilled).
@@ -248,35 +264,56 @@ __device__ __noinline__ void persist_factor_block(int b, T* L, int n, int* info,
 }
 
 template <typename T>
-__device__ __noinline__ void persist_panel(int m, int b, T* L, int n) {
+__device__ __noinline__ void persist_panel(int m, int b, T* L, int n, char* arena) {
+    double(*sDg)[32][32] = reinterpret_cast<double(*)[32][32]>(arena);
+    double* sRinv = reinterpret_cast<double*>(arena + 2 * 32 * 32 * 8);
+    double(*sXq)[16][34] = reinterpret_cast<double(*)[16][34]>(arena + (2 * 32 * 32 + 128) * 8);
     for (int rb = 0; rb < (m + 63) / 64; ++rb) {
-        trsm_rlt_rows<T, false, kPersistMaxNb>(m, b, L, n, L + b, n, nullptr, 0, rb);
+        chain64::panel64_slab<T>(m, b, L, n, L + b, n, rb, sDg, sRinv, sXq);
         __syncthreads();
     }
 }
 
 template <typename T, int MODE>
-__device__ __noinline__ void persist_trailing(int m, int b, const T* P, T* C, int n) {
+__device__ __noinline__ void persist_trailing(int m, int b, const T* P, T* C, int n, char* arena) {
     const int nt = (m + 127) / 128;
     for (int tt = 0; tt < nt * (nt + 1) / 2; ++tt) {
         int ti, tj;
         tri_tile(tt, ti, tj);
-        if constexpr (MODE == 0)
-            syrk_tile_f64(m, m, b, -1.0, (const double*)P, n, (const double*)P, n, 1.0, (double*)C, n, ti, tj);
-        else if constexpr (MODE == 1)
-            syrk_tile_f32(m, m, b, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj);
-        else
-            syrk_tile_bf16x3(m, m, b, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj);
+        if constexpr (MODE == 0) {
+            auto sI = reinterpret_cast<double(*)[kSyrkKC64][kSyrkLdp]>(arena);
+            auto sJ = reinterpret_cast<double(*)[kSyrkKC64][kSyrkLdp]>(arena + kArenaSyrk64 / 2);
+            syrk_tile_f64_lds(m, m, b, -1.0, (const double*)P, n, (const double*)P, n, 1.0, (double*)C, n, ti, tj,
+                              sI, sJ);
+        } else if constexpr (MODE == 1) {
+            auto sI = reinterpret_cast<float(*)[kSyrkKC][kSyrkLdpF]>(arena);
+            auto sJ = reinterpret_cast<float(*)[kSyrkKC][kSyrkLdpF]>(arena + kArenaSyrk32 / 2);
+            syrk_tile_f32_lds(m, m, b, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj,
+                              sI, sJ);
+        } else {
+            constexpr int q = kArenaBf16 / 4;
+            auto s0 = reinterpret_cast<unsigned short(*)[kBfLd]>(arena);
+            auto s1 = reinterpret_cast<unsigned short(*)[kBfLd]>(arena + q);
+            auto s2 = reinterpret_cast<unsigned short(*)[kBfLd]>(arena + 2 * q);
+            auto s3 = reinterpret_cast<unsigned short(*)[kBfLd]>(arena + 3 * q);
+            syrk_tile_bf16x3_lds(m, m, b, -1.0f, (const float*)P, n, (const float*)P, n, 1.0f, (float*)C, n, ti, tj,
+                                 s0, s1, s2, s3);
+        }
         __syncthreads();
     }
 }
 
-__device__ __noinline__ void persist_solve(const BatchArgs& a, const double* A, double* B, int p) {
-    batch_solve_one<double>(a, A, B, p);
+__device__ __noinline__ void persist_solve(const BatchArgs& a, const double* A, double* B, int p, char* arena) {
+    auto sL = reinterpret_cast<double(*)[kBS + 1]>(arena);
+    auto sX = reinterpret_cast<double(*)[kBC + 1]>(arena + kBS * (kBS + 1) * 8);
+    auto sR = reinterpret_cast<double(*)[kBC + 1]>(arena + (kBS * (kBS + 1) + kBS * (kBC + 1)) * 8);
+    double* sRinv = reinterpret_cast<double*>(arena + (kBS * (kBS + 1) + 2 * kBS * (kBC + 1)) * 8);
+    batch_solve_one<double>(a, A, B, p, sL, sX, sR, sRinv);
 }
 
 template <typename T, int MODE, bool SOLVE>
-__global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a, T* A, double* B, int nb, int* counter) {
+__global__ __launch_bounds__(256, 2) void batch_persistent_kernel(BatchArgs a, T* A, double* B, int nb, int* counter) {
+    __shared__ __align__(16) char arena[kArenaBytes];
     __shared__ int s_p;
     for (;;) {
         __syncthreads();  // every thread is done with the previous matrix
@@ -295,12 +332,12 @@ __global__ __launch_bounds__(256) void batch_persistent_kernel(BatchArgs a, T* A
             if (*info != 0) break;
             const int m = n - c0 - b;
             if (m <= 0) break;
-            persist_panel<T>(m, b, L, n);
-            persist_trailing<T, MODE>(m, b, L + b, M + (c0 + b) + (i64)(c0 + b) * n, n);
+            persist_panel<T>(m, b, L, n, arena);
+            persist_trailing<T, MODE>(m, b, L + b, M + (c0 + b) + (i64)(c0 + b) * n, n, arena);
         }
         if constexpr (SOLVE) {
             __syncthreads();
-            persist_solve(a, (const double*)A, B, p);
+            persist_solve(a, (const double*)A, B, p, arena);
         }
     }
 }
@@ -521,22 +558,11 @@ hipError_t batch_potrf(const BatchArgs& a, void* A, bool f32, int step, int nb, 
 hipError_t batch_trsm(const BatchArgs& a, void* A, bool f32, int step, int nb, const int* prefix, int cnt, int wgs,
                       hipStream_t s) {
     if (wgs <= 0) return hipSuccess;
-    const bool full = nb == kTrsmMaxN;
-    if (f32) {
-        if (full)
-            hipLaunchKernelGGL((batch_trsm_kernel<float, true>), dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb,
-                               prefix, cnt);
-        else
-            hipLaunchKernelGGL((batch_trsm_kernel<float, false>), dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb,
-                               prefix, cnt);
-    } else {
-        if (full)
-            hipLaunchKernelGGL((batch_trsm_kernel<double, true>), dim3(wgs), dim3(256), 0, s, a, (double*)A, step,
-                               nb, prefix, cnt);
-        else
-            hipLaunchKernelGGL((batch_trsm_kernel<double, false>), dim3(wgs), dim3(256), 0, s, a, (double*)A, step,
-                               nb, prefix, cnt);
-    }
+    if (f32)
+        hipLaunchKernelGGL(batch_trsm_kernel<float>, dim3(wgs), dim3(256), 0, s, a, (float*)A, step, nb, prefix, cnt);
+    else
+        hipLaunchKernelGGL(batch_trsm_kernel<double>, dim3(wgs), dim3(256), 0, s, a, (double*)A, step, nb, prefix,
+                           cnt);
     return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@ namespace elemamd {
 // Per-launch view of a group of matrices (processing order): sizes,
 // right-hand-side counts, element offsets into the packed buffers, factor
 // status and the refinement's "converged" flags (may be null).
-constexpr int kPersistMaxNb = 64;  // largest block size of the persistent launch
+constexpr int kPersistMaxNb = 128;  // largest block size of the persistent launch
 
 struct BatchArgs {
     int count;
@@ -93,8 +93,7 @@ struct BatchPlan {
 int batch_plan_create(int count, const int* n, const int* k, int precision, int nbuckets, const int* bucket_max_n,
                       const int* bucket_nb, BatchPlan** out);
 // Launch form per bucket (caller's bucket order): 0 = one launch per step
-// and kernel, g > 0 = one persistent launch of g workgroups (block size at
-// most kPersistMaxNb).
+// and kernel, g > 0 = one persistent launch of g workgroups.
 int batch_set_launch(BatchPlan* P, const int* bucket_grid);
 size_t batch_workspace_bytes(const BatchPlan* P);
 int batch_hpd_solve(BatchPlan* P, double* A, double* B, void* work, size_t work_bytes, bool use_graph,

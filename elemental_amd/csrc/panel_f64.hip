@@ -43,78 +43,7 @@ __global__ __launch_bounds__(256) void panel_f64_kernel(int m, int b, const doub
     // update's waves sharing the SIMD (wave priority, not queue priority)
     __builtin_amdgcn_s_setprio(3);
     if (info != nullptr && *info != 0 && *info <= limit) return;
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6, c = lane & 15, g = lane >> 4;
-    const int np = (b + 31) / 32;
-    const i64 rw = (i64)blockIdx.x * 64 + 16 * w;  // this wave's first row
-    // the wave's rows (zero past m and past b), loads in flight while L11 is staged
-    d4_t a[4][2];
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const i64 r = rw + g + 4 * v;
-                const int col = 32 * p + 16 * h + c;
-                const bool ok = r < m && col < b;
-                const double x = B[(ok ? r : 0) + (i64)(ok ? col : 0) * ldb];
-                a[p][h][v] = ok ? x : 0.0;
-            }
-    // the diagonal pieces of L11: rows past b are zero, the padded diagonal
-    // is the identity
-    for (int e = t; e < 4 * 1024; e += 256) {
-        const int q = e >> 10, cc = e & 31, k = (e >> 5) & 31;
-        if (cc <= k) continue;
-        const int r = 32 * q + cc, col = 32 * q + k;
-        const bool ok = r < b && col < b;
-        const double x = L[(ok ? r : 0) + (i64)(ok ? col : 0) * ldl];
-        if (q & 1) sDg[q >> 1][31 - k][31 - cc] = ok ? x : 0.0;
-        else sDg[q >> 1][k][cc] = ok ? x : 0.0;
-    }
-    if (t < 128) sRinv[t] = t < b ? 1.0 / L[t + (i64)t * ldl] : 1.0;
-    __syncthreads();
-    double(*sx)[34] = sXq[w];  // X_q, row-major
-    const int i16 = lane & 15;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (q >= np) break;
-        // operands of the updates this piece applies, L_pq[16h + i16][4 ks + g]
-        // (rows past b are zero; the columns, below 32 p <= b, all exist)
-        double y[3][2][8];
-#pragma unroll
-        for (int p = q + 1; p < 4; ++p) {
-            if (p >= np) break;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int r = 32 * p + 16 * h + i16;
-#pragma unroll
-                for (int ks = 0; ks < 8; ++ks) {
-                    const double x = L[(r < b ? r : 0) + (i64)(32 * q + 4 * ks + g) * ldl];
-                    y[p - q - 1][h][ks] = r < b ? x : 0.0;
-                }
-            }
-        }
-        if (q & 1) Substitute<0, PackedDiag<true>>::run(a[q], c, PackedDiag<true>{sDg[q >> 1]}, sRinv + 32 * q);
-        else Substitute<0, PackedDiag<false>>::run(a[q], c, PackedDiag<false>{sDg[q >> 1]}, sRinv + 32 * q);
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                sx[g + 4 * v][16 * h + c] = a[q][h][v];
-                const i64 r = rw + g + 4 * v;
-                const int col = 32 * q + 16 * h + c;
-                if (r < m && col < b) B[r + (i64)col * ldb] = a[q][h][v];
-            }
-        if (q + 1 >= np) break;
-        // the buffer holds X_q for every lane of the wave (the wave's LDS
-        // accesses are performed in order; the barrier keeps the compiler's)
-        __builtin_amdgcn_wave_barrier();
-#pragma unroll
-        for (int p = q + 1; p < 4; ++p)
-            if (p < np) sub_product_regs(a[p], sx, 0, y[p - q - 1][0], y[p - q - 1][1], lane);
-        mfma_retire();
-        __builtin_amdgcn_wave_barrier();
-    }
+    panel64_slab<double>(m, b, L, ldl, B, ldb, blockIdx.x, sDg, sRinv, sXq);
 }
 
 hipError_t launch_panel_f64(int m, int b, const double* L, i64 ldl, double* B, i64 ldb, const int* info, int limit,

@@ -42,12 +42,12 @@ __device__ inline bool syrk_tile(int m, int n, int& ti, int& tj) {
 // stored at the end.
 constexpr int kSyrkKC64 = 8;
 
-__device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
+// LDS from the caller: sI and sJ, each 2 x kSyrkKC64 x kSyrkLdp doubles.
+__device__ __forceinline__ void syrk_tile_f64_lds(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
-        double* __restrict__ C, i64 ldc, int ti, int tj) {
+        double* __restrict__ C, i64 ldc, int ti, int tj, double (*sI)[kSyrkKC64][kSyrkLdp],
+        double (*sJ)[kSyrkKC64][kSyrkLdp]) {
     constexpr int KC = kSyrkKC64;
-    __shared__ double sI[2][KC][kSyrkLdp];
-    __shared__ double sJ[2][KC][kSyrkLdp];
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64;  // MFMA M offset = C column offset
@@ -155,14 +155,22 @@ __device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
 }
 
 
+__device__ __forceinline__ void syrk_tile_f64(int m, int n, int k, double alpha,
+        const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
+        double* __restrict__ C, i64 ldc, int ti, int tj) {
+    __shared__ double sI[2][kSyrkKC64][kSyrkLdp];
+    __shared__ double sJ[2][kSyrkKC64][kSyrkLdp];
+    syrk_tile_f64_lds(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj, sI, sJ);
+}
+
 // ---------------------------------------------------------------- FP32
 constexpr int kSyrkLdpF = kSyrkTile + 16;  // 144 floats: k-rows 16 banks apart
 
-__device__ __forceinline__ void syrk_tile_f32(int m, int n, int k, float alpha,
+// LDS from the caller: sI and sJ, each 2 x kSyrkKC x kSyrkLdpF floats.
+__device__ __forceinline__ void syrk_tile_f32_lds(int m, int n, int k, float alpha,
         const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
-        float* __restrict__ C, i64 ldc, int ti, int tj) {
-    __shared__ float sI[2][kSyrkKC][kSyrkLdpF];
-    __shared__ float sJ[2][kSyrkKC][kSyrkLdpF];
+        float* __restrict__ C, i64 ldc, int ti, int tj, float (*sI)[kSyrkKC][kSyrkLdpF],
+        float (*sJ)[kSyrkKC][kSyrkLdpF]) {
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
@@ -244,18 +252,24 @@ __device__ __forceinline__ void syrk_tile_f32(int m, int n, int k, float alpha,
 // dropped), each a v_mfma_f32_16x16x32_bf16 with fp32 accumulation.
 // LDS holds row-major [row][k] planes so one lane fetches its 8 consecutive
 // k values (16 bytes) per operand.
+__device__ __forceinline__ void syrk_tile_f32(int m, int n, int k, float alpha,
+        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+        float* __restrict__ C, i64 ldc, int ti, int tj) {
+    __shared__ float sI[2][kSyrkKC][kSyrkLdpF];
+    __shared__ float sJ[2][kSyrkKC][kSyrkLdpF];
+    syrk_tile_f32_lds(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj, sI, sJ);
+}
+
 constexpr int kBfKC = 32;
 constexpr int kBfLd = kBfKC + 8;  // 40 bf16 = 80 B per row: 8 lanes of 16 B hit disjoint banks
 
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ void syrk_tile_bf16x3(int m, int n, int k, float alpha,
+// LDS from the caller: four kSyrkTile x kBfLd bf16 arrays, 16-byte aligned.
+__device__ __forceinline__ void syrk_tile_bf16x3_lds(int m, int n, int k, float alpha,
         const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
-        float* __restrict__ C, i64 ldc, int ti, int tj) {
-    __shared__ __attribute__((aligned(16))) unsigned short sIh[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sIl[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sJh[kSyrkTile][kBfLd];
-    __shared__ __attribute__((aligned(16))) unsigned short sJl[kSyrkTile][kBfLd];
+        float* __restrict__ C, i64 ldc, int ti, int tj, unsigned short (*sIh)[kBfLd], unsigned short (*sIl)[kBfLd],
+        unsigned short (*sJh)[kBfLd], unsigned short (*sJl)[kBfLd]) {
     const int i0 = ti * kSyrkTile, j0 = tj * kSyrkTile;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wj = (w >> 1) * 64, wi = (w & 1) * 64;
@@ -345,6 +359,16 @@ __device__ __forceinline__ void syrk_tile_bf16x3(int m, int n, int k, float alph
         }
 }
 
+
+__device__ __forceinline__ void syrk_tile_bf16x3(int m, int n, int k, float alpha,
+        const float* __restrict__ A, i64 lda, const float* __restrict__ B, i64 ldb, float beta,
+        float* __restrict__ C, i64 ldc, int ti, int tj) {
+    __shared__ __attribute__((aligned(16))) unsigned short sIh[kSyrkTile][kBfLd];
+    __shared__ __attribute__((aligned(16))) unsigned short sIl[kSyrkTile][kBfLd];
+    __shared__ __attribute__((aligned(16))) unsigned short sJh[kSyrkTile][kBfLd];
+    __shared__ __attribute__((aligned(16))) unsigned short sJl[kSyrkTile][kBfLd];
+    syrk_tile_bf16x3_lds(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, ti, tj, sIh, sIl, sJh, sJl);
+}
 
 // Split-BF16 tile from pre-split panels.  The panel is stored as NP bf16
 // planes, plane p at planes + p*ps: plane 0 = bf16(x), plane p = bf16 of

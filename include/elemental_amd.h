@@ -294,8 +294,8 @@ int elemamd_batch_plan_destroy(elemamd_batch_plan *plan);
  * bucket_grid[b] = 0 runs bucket b as one potrf, one panel and one trailing
  * launch per block step (the default); g > 0 runs its whole factorization,
  * and for ELEMAMD_BATCH_F64 its solves, as ONE persistent launch of g
- * workgroups that take the bucket's matrices largest first (block size at
- * most 64).  Results are the same bit for bit.  No reference counterpart
+ * workgroups that take the bucket's matrices largest first.  Results are
+ * the same bit for bit.  No reference counterpart
  * (for callers factoring many small matrices). */
 int elemamd_batch_set_launch(elemamd_batch_plan *plan, const int *bucket_grid);
 /* Element offsets of every A_i / B_i and the total element counts. */

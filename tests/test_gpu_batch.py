@@ -179,7 +179,7 @@ def test_plan_rejects_bad_arguments():
 
 
 # ------------------------------------------------------- persistent launch
-PERSIST_BUCKETS = [(96, 32), (400, 64), (1 << 30, 64)]
+PERSIST_BUCKETS = [(96, 32), (400, 64), (1 << 30, 128)]
 
 
 def _grid(buckets, g):
@@ -240,6 +240,6 @@ def test_persistent_graph_replays_and_non_hpd_member():
         assert plan.results()[0][bad] == col + 1
 
 
-def test_persistent_rejects_wide_blocks():
+def test_persistent_rejects_bad_grid():
     with pytest.raises(ValueError):
-        EB.BatchPlan([100, 300], [1, 2], buckets=[(1 << 30, 128, 64)])
+        EB.BatchPlan([100, 300], [1, 2], buckets=[(1 << 30, 128, -1)])
