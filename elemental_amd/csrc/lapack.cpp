@@ -726,15 +726,22 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     f.shifted = 0;
     f.anorm = 0.0;
     if (n == 0) return kOk;
-    double* rowsum = f.scratch;
-    double* dtmp = f.scratch + n;
-    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, symm_part_count(n, 1) ? f.scratch + n + 8 : nullptr, s));
-    ELEMAMD_HIP_TRY(launch_vec_max(n, rowsum, dtmp, s));
-    ELEMAMD_HIP_TRY(hipMemcpyAsync(&f.anorm, dtmp, sizeof(double), hipMemcpyDeviceToHost, s));
-    double sigma = 0.0;
-    int info = 0;
     StreamPair* sp = stream_pair(s);
     if (sp == nullptr) return kErrHip;
+    // ||A||_inf (only the refinement's backward errors need it) on aux3
+    // beside the factorization's first steps, into pinned memory behind
+    // ev_c; aux3 starts after the caller's work before this call
+    double* rowsum = f.scratch;
+    double* dtmp = f.scratch + n;
+    ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_c, s));
+    ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux3, sp->ev_c, 0));
+    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, symm_part_count(n, 1) ? f.scratch + n + 8 : nullptr,
+                                                sp->aux3));
+    ELEMAMD_HIP_TRY(launch_vec_max(n, rowsum, dtmp, sp->aux3));
+    ELEMAMD_HIP_TRY(hipMemcpyAsync(sp->pinned, dtmp, sizeof(double), hipMemcpyDeviceToHost, sp->aux3));
+    ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_c, sp->aux3));
+    double sigma = 0.0;
+    int info = 0;
     // the float copy: the first group of block columns on the caller's
     // stream, the columns right of it on aux under that group's factor steps
     // (the factorization waits for them before its first trailing update)
@@ -817,6 +824,8 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     // the caller's stream waits for the mirror (aux) and the preparation (aux3)
     ELEMAMD_HIP_TRY(hipStreamWaitEvent(s, sp->ev_b, 0));
     if (int rc = join_prep()) return rc;
+    ELEMAMD_HIP_TRY(hipEventSynchronize(sp->ev_c));  // the norm, long done by now
+    f.anorm = sp->pinned[0];
     if (info != 0) return kErrNotConverged;  // broke down even after shifting
     return kOk;
 }

@@ -24,6 +24,7 @@ void release(StreamPair& p) {
         if (*e) (void)hipEventDestroy(*e), *e = nullptr;
     for (hipStream_t* t : {&p.aux, &p.aux2, &p.aux3})
         if (*t) (void)hipStreamDestroy(*t), *t = nullptr;
+    if (p.pinned) (void)hipHostFree(p.pinned), p.pinned = nullptr;
 }
 struct PairCache {
     std::map<std::pair<int, hipStream_t>, StreamPair> pairs;
@@ -44,6 +45,7 @@ StreamPair* stream_pair(hipStream_t stream) {
                   hipStreamCreateWithFlags(&p.aux3, hipStreamNonBlocking) == hipSuccess;
         for (hipEvent_t* e : {&p.ev_a, &p.ev_b, &p.ev_c, &p.ev_d, &p.ev_f, &p.ev_g, &p.ev_h})
             ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+        ok = ok && hipHostMalloc((void**)&p.pinned, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
         if (!ok) {
             release(p);
             return nullptr;

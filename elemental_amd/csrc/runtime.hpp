@@ -20,6 +20,7 @@ struct StreamPair {
     hipStream_t aux3 = nullptr;  // solve preparation beside the factorization
     hipEvent_t ev_a = nullptr, ev_b = nullptr, ev_c = nullptr, ev_d = nullptr, ev_f = nullptr, ev_g = nullptr,
                ev_h = nullptr;
+    double* pinned = nullptr;  // 8 doubles of pinned host memory (device-to-host scalars behind an event)
 };
 // Owned by the calling thread; nullptr members on failure.
 StreamPair* stream_pair(hipStream_t stream);
