@@ -383,46 +383,52 @@ def bench_sweep(args):
         At.copy_((Q * s_.unsqueeze(0)) @ Q.t())
         At.copy_(0.5 * (At + At.t()))
         anorm = sym_inf_norm(A)
-        # plain FP64
-        Aw.tensor().copy_(At)
-        X.tensor().copy_(B.tensor())
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        try:
-            E.HPDSolve(E.LOWER, E.NORMAL, Aw, X)
+        # plain FP64 (a warm-up solve first: the library's scratch buffers
+        # are allocated on first use)
+        t64, eta64 = float("nan"), float("nan")
+        for rep_ in range(2):
+            Aw.tensor().copy_(At)
+            X.tensor().copy_(B.tensor())
             torch.cuda.synchronize()
-            t64 = time.perf_counter() - t0
-            eta64 = eta_inf(A, X, B, anorm)
-        except E.LogicError:
-            t64, eta64 = float("nan"), float("nan")
+            t0 = time.perf_counter()
+            try:
+                E.HPDSolve(E.LOWER, E.NORMAL, Aw, X)
+                torch.cuda.synchronize()
+                t64 = time.perf_counter() - t0
+                eta64 = eta_inf(A, X, B, anorm)
+            except E.LogicError:
+                t64, eta64 = float("nan"), float("nan")
         Xref = X.tensor().clone()
         row = {"kappa": kappa, "f64": {"ms": round(t64 * 1e3, 2), "backward_error": eta64,
                                        "tflops": round(n ** 3 / 3 / t64 / 1e12, 2),
                                        "frac_fp64_peak": round(n ** 3 / 3 / t64 / PEAK["f64"], 3)}}
         for trailing in ("f32", "bf16x3", "bf16x6"):
-            opts = E.IROptions(trailing=trailing)
+            opts = E.IROptions(trailing=trailing, refine=args.refine)
             ws = torch.empty(workspace_bytes(n, k, opts), dtype=torch.uint8, device="cuda")
-            X.tensor().copy_(B.tensor())
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            rep = E.HPDSolveMixed(A, X, opts, workspace=ws)
-            torch.cuda.synchronize()
-            t = time.perf_counter() - t0
+            for rep_ in range(2):  # a warm-up solve, then the timed one
+                X.tensor().copy_(B.tensor())
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                rep = E.HPDSolveMixed(A, X, opts, workspace=ws)
+                torch.cuda.synchronize()
+                t = time.perf_counter() - t0
             eta = eta_inf(A, X, B, anorm)
             fwd = float(torch.linalg.norm(X.tensor() - Xref) / torch.linalg.norm(Xref))
             peak = PEAK["f32"] if trailing == "f32" else PEAK["bf16"]
             row[trailing] = {"ms": round(t * 1e3, 2), "backward_error": eta, "forward_error_vs_fp64": fwd,
                              "ir_iterations": rep.iterations, "gmres_iterations": rep.gmres_iterations,
                              "used_gmres": rep.used_gmres, "shifted": rep.shifted, "converged": rep.converged,
+                             "fp64_finish": bool(rep.fallback), "lowprec_info": rep.lowprec_info,
                              "speedup_vs_fp64": round(t64 / t, 3) if t64 == t64 else None,
                              "factor_frac_of_peak": round(n ** 3 / 3 / t / peak, 4)}
             del ws
         rows.append(row)
         print(json.dumps(row), file=sys.stderr, flush=True)
-    best = rows[2]["f32"]["ms"]
-    out = _common("precision sweep: mixed+IR vs plain FP64 at n=32768 (ms per solve, kappa=1e6 headline)", "ms",
-                  best, args, best, "f64", {"workload": f"HPD factor+solve n={n} k={k} kappa in 1e2..1e8",
-                                             "n": n, "k": k, "parallelism": "single GPU"}, higher=False)
+    best = min(rows[2][t]["ms"] for t in ("f32", "bf16x3", "bf16x6"))
+    out = _common("precision sweep: mixed+IR vs plain FP64 at n=32768 (ms per solve, kappa=1e6 headline: the "
+                  "fastest trailing mode)", "ms", best, args, best, "f64",
+                  {"workload": f"HPD factor+solve n={n} k={k} kappa in 1e2..1e8", "n": n, "k": k,
+                   "refine": args.refine, "parallelism": "single GPU"}, higher=False)
     out["sweep"] = rows
     return out
 
