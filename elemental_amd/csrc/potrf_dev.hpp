@@ -15,27 +15,61 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// Columns j = NTC*A + r, r < NTC, with A fixed at compile time so that the
-// owner's register indices (a = j/32, b = A) are static.  NTC = threads/32
-// column groups: thread (tr, tc) = (t % 32, t / 32) owns the elements
-// (i, k) = (tr + 32a, tc + NTC b), a < 4, b < 128/NTC.
+// Rank-1 update of the elements with j < k <= i < n by the finished column
+// j published in cj (all of it read first, one wait), skipping the column
+// block b == SKIPB of the threads with tc == SKIPTC (elements already
+// updated by the column's owners; -1: none).
+template <int A, int NTC>
+__device__ __forceinline__ void potrf_rank1(double (&e)[4][128 / NTC], int n, int tr, int tc, int j, const double* cj,
+                                            int skip_tc) {
+#pragma clang fp contract(off)
+    constexpr int NBC = 128 / NTC;
+    double ri[4], ck[NBC];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) ri[a] = cj[tr + 32 * a];
+#pragma unroll
+    for (int b = A; b < NBC; ++b) ck[b] = cj[tc + NTC * b];
+#pragma unroll
+    for (int b = A; b < NBC; ++b) {
+        const int k = tc + NTC * b;
+        if (NTC * b + NTC - 1 <= j) continue;  // whole column block finished (uniform)
+        if (b == A && tc == skip_tc) continue;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            if (32 * a + 31 <= j) continue;  // whole row block finished (uniform)
+            const int i = tr + 32 * a;
+            if (k > j && i >= k && i < n) e[a][b] = e[a][b] - ri[a] * ck[b];
+        }
+    }
+}
+
+// Columns j = NTC*A + r, r < NTC, two per barrier, with A fixed at compile
+// time so that the owners' register indices (a = j/32, b = A) are static.
+// NTC = threads/32 column groups: thread (tr, tc) = (t % 32, t / 32) owns
+// the elements (i, k) = (tr + 32a, tc + NTC b), a < 4, b < 128/NTC.
+// Columns j and j+1 (r even) belong to one wave (tc = r: lanes 0..31,
+// tc = r+1: lanes 32..63): it factors column j, applies it to column j+1,
+// factors column j+1 and publishes both; after the barrier every thread
+// applies column j, then column j+1.  Each element sees the reference's
+// operations in the reference's order, so the result is unchanged bit for
+// bit; half the barriers.
 template <int A, int NTC>
 __device__ __forceinline__ bool potrf_col_group(double (&e)[4][128 / NTC], int n, int tr, int tc, int wave,
                                                 double (*col)[kPotrfMaxN], int* sfail) {
 #pragma clang fp contract(off)
-    constexpr int NBC = 128 / NTC;
     constexpr int a_own = NTC * A / 32;  // row block of the pivots (j / 32)
-    constexpr int b_own = A;             // column block of the columns (j / NTC)
-    for (int r = 0; r < NTC; ++r) {
+    for (int r = 0; r < NTC; r += 2) {
         const int j = NTC * A + r;
         if (j >= n) return true;
-        double* cj = col[j & 1];
-        // owners of column j: tc == r, threads 32r .. 32r+31, wave r/2
+        double* cj = col[((j >> 1) & 1) * 2];   // two pairs of buffers by step parity: this
+        double* cj1 = cj + kPotrfMaxN;          // step's writers never meet the last step's readers
+        const bool two = j + 1 < n;
         if (wave == (r >> 1)) {
-            const int own_lane = ((r & 1) << 5) + (j & 31);  // thread tr = j%32, tc = r
-            const double ajj = readlane_d(e[a_own][b_own], own_lane);
+            // column j: owners tc == r (lanes 0..31)
+            const double ajj = readlane_d(e[a_own][A], j & 31);
+            const bool okj = ajj > 0.0;
             if (tc == r) {
-                if (ajj <= 0.0) {
+                if (!okj) {
                     if (tr == 0) *sfail = j + 1;
                 } else {
                     const double alpha = sqrt(ajj);
@@ -43,36 +77,55 @@ __device__ __forceinline__ bool potrf_col_group(double (&e)[4][128 / NTC], int n
                     for (int a = 0; a < 4; ++a) {
                         const int i = tr + 32 * a;
                         if (i > j && i < n) {
-                            e[a][b_own] = e[a][b_own] / alpha;
-                            cj[i] = e[a][b_own];
+                            e[a][A] = e[a][A] / alpha;
+                            cj[i] = e[a][A];
                         } else if (i == j) {
-                            e[a][b_own] = alpha;
+                            e[a][A] = alpha;
+                        }
+                    }
+                }
+            }
+            // the column's entries are read by the other half of this wave
+            __builtin_amdgcn_wave_barrier();
+            if (okj && two) {
+                // column j+1 (owners tc == r + 1, lanes 32..63): column j's
+                // update first, then its own pivot
+                const double lkj = cj[j + 1];
+                if (tc == r + 1) {
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        if (32 * a + 31 <= j) continue;  // uniform
+                        const int i = tr + 32 * a;
+                        if (i >= j + 1 && i < n) e[a][A] = e[a][A] - cj[i] * lkj;
+                    }
+                }
+                const double a11 = readlane_d(e[a_own][A], 32 + ((j + 1) & 31));
+                if (tc == r + 1) {
+                    if (!(a11 > 0.0)) {
+                        if (tr == 0) *sfail = j + 2;
+                    } else {
+                        const double alpha = sqrt(a11);
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) {
+                            const int i = tr + 32 * a;
+                            if (i > j + 1 && i < n) {
+                                e[a][A] = e[a][A] / alpha;
+                                cj1[i] = e[a][A];
+                            } else if (i == j + 1) {
+                                e[a][A] = alpha;
+                            }
                         }
                     }
                 }
             }
         }
         __syncthreads();
-        if (*sfail != 0) return false;
-        // rank-1 update of the elements with j < k <= i < n.  All column-j
-        // values this thread needs are read from LDS first (one wait), then
-        // the masked products are applied from registers.
-        double ri[4], ck[NBC];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) ri[a] = cj[tr + 32 * a];
-#pragma unroll
-        for (int b = A; b < NBC; ++b) ck[b] = cj[tc + NTC * b];
-#pragma unroll
-        for (int b = A; b < NBC; ++b) {
-            const int k = tc + NTC * b;
-            if (NTC * b + NTC - 1 <= j) continue;  // whole column block finished (uniform)
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                if (32 * a + 31 <= j) continue;  // whole row block finished (uniform)
-                const int i = tr + 32 * a;
-                if (k > j && i >= k && i < n) e[a][b] = e[a][b] - ri[a] * ck[b];
-            }
-        }
+        const int f = *sfail;
+        if (f != 0 && f <= j + 1) return false;  // column j failed: nothing of it is applied
+        // column j (the owners already applied it to column j+1)
+        potrf_rank1<A, NTC>(e, n, tr, tc, j, cj, two ? r + 1 : -1);
+        if (f != 0) return false;  // column j+1 failed after column j's update
+        if (two) potrf_rank1<A, NTC>(e, n, tr, tc, j + 1, cj1, -1);
     }
     return true;
 }
@@ -101,7 +154,7 @@ __device__ __forceinline__ void potrf_unb_block(int n, T* __restrict__ A, i64 ld
                                                 int col_offset) {
 #pragma clang fp contract(off)
     constexpr int NTC = THREADS / 32, NBC = 128 / NTC;
-    __shared__ double col[2][kPotrfMaxN];
+    __shared__ double col[4][kPotrfMaxN];
     __shared__ int sfail;
     if (info != nullptr && *info != 0 && *info <= col_offset) return;  // an earlier block failed
     const int t = threadIdx.x;

@@ -47,7 +47,10 @@ __device__ inline int syrk_valid_k(const int* info, int k, int kcol0, int nbp) {
 
 __global__ __launch_bounds__(256, 2) void syrk_ln_f64_kernel(int m, int n, int k, double alpha,
         const double* __restrict__ A, i64 lda, const double* __restrict__ B, i64 ldb, double beta,
-        double* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp) {
+        double* __restrict__ C, i64 ldc, const int* __restrict__ info, int kcol0, int nbp, int chain) {
+    // a launch the factorization's chain waits for issues ahead of the
+    // concurrent bulk update's waves (wave priority, not queue priority)
+    if (chain) __builtin_amdgcn_s_setprio(3);
     const int ke = syrk_valid_k(info, k, kcol0, nbp);
     if (info != nullptr && ke == 0) return;
     int ti, tj;
@@ -86,10 +89,11 @@ hipError_t launch_syrk_ln_f64(int m, int n, int k, double alpha, const double* A
                               bool beside_chain) {
     if (m <= 0 || n <= 0) return hipSuccess;
     // beside the chain: 56 KB of LDS per workgroup (36 KB used), so at most
-    // two share a CU and 48 KB stay free for the chain's kernels
+    // two share a CU and 48 KB stay free for the chain's kernels; every other
+    // launch of the factorization is on its chain
     const unsigned pad = beside_chain ? 20u * 1024u : 0u;
     hipLaunchKernelGGL(syrk_ln_f64_kernel, syrk_grid(m, n), dim3(256), pad, s, m, n, k, alpha, A, lda, B, ldb, beta, C,
-                       ldc, info, kcol0, nbp);
+                       ldc, info, kcol0, nbp, (info != nullptr && !beside_chain) ? 1 : 0);
     return hipGetLastError();
 }
 
