@@ -10,23 +10,29 @@
 // sweep is
 //     y_b = X_b (r_b - sum_{j < b-d} L_bj y_j) + sum_{i=1..d} P(i)_b y_{b-i}.
 // Only the last product waits for the block solved just before, so one sweep
-// step costs one 64 x NB x k product per workgroup plus a flag hand-over;
+// step costs one 32 x NB x k product per workgroup plus a flag hand-over;
 // the older blocks are folded in with d steps of slack.  The backward sweep
 // is the same with U = L^T (the strict upper triangle of the factor's
 // storage holds the mirror of L, written by the preparation), X_b^T and
 // Q(i)_b = -X_b^T U_{b,b+i}, over the blocks in reverse order.
 //
-// Workgroup (s, rt) owns rows [64 rt, 64 rt + 64) of the block of step s and
-// all (<= 32) right-hand sides:
+// Workgroup (s, rt) owns rows [32 rt, 32 rt + 32) of the block of step s
+// and all (<= 32) right-hand sides; its four waves take 16 rows x one half
+// of each product's K range, and their partial sums are added at the two
+// publication points:
 //   1. u = sum over source steps s' < s - d of M(rows, block(s')) y_{s'}
 //      (the A operand straight from the factor into registers, y_{s'}
-//      through LDS);
+//      straight into LDS);
 //   2. z = r - u, published; then the block's other row tiles are awaited;
 //   3. y = X z + sum_i P(i) y_{s-i}, published.
+// A product's A operand (32 rows x NB, 64 KB) is loaded whole into a second
+// set of registers while the previous product multiplies.  A step then
+// costs the hand-over latencies (flag poll, the block vector's round trip,
+// the publication) plus one product's MFMAs (1.7 us for k = 32).
 // Solved blocks, z and the flags move through device-coherent loads and
 // stores (solve_dev.hpp); workgroups take tickets in launch order, so every
-// wait is on a workgroup that is running or done (the R row tiles of one
-// step need to be resident together), and every wait is bounded.
+// wait is on a workgroup that is running or done (the row tiles of one step
+// need to be resident together), and every wait is bounded.
 #include <algorithm>
 #include <type_traits>
 
@@ -37,23 +43,22 @@
 namespace elemamd {
 
 constexpr int kSwNB = 512;          // sweep block rows
-constexpr int kSwR = kSwNB / 64;    // row tiles (workgroups) per block
-constexpr int kSwH = 256;           // block-vector rows staged in LDS at a time
+constexpr int kSwR = kSwNB / 64;    // 64-row diagonal pieces per block (preparation)
+constexpr int kSwTR = 32;           // rows per sweep workgroup
+constexpr int kSwRT = kSwNB / kSwTR;  // sweep workgroups (row tiles) per block
+constexpr int kSwKH = kSwNB / 2;    // K range of one wave in a product
 constexpr int kSwLook = 3;          // look-ahead products d
 constexpr int kSwKP = 32;           // right-hand sides per sweep (row stride of the block vectors)
-constexpr int kSwVP = 48;           // LDS row stride of a block vector: B-operand reads are conflict-free
-constexpr int kSwVecQ = kSwH * kSwKP / 2 / 256;   // 64-bit loads per thread for one staged half
 constexpr long kSwSpinLimit = 1L << 22;           // polls before a wait is declared lost (seconds)
 
-typedef unsigned long long u64;
-
 // Offset of element (a, c) of an NB x NB tile in the fragment order of the
-// sweep's A operand: row tile, wave, then per lane 4 consecutive K steps as
-// one float4 (MFMA 16x16x4 f32: lane = m + 16 g holds A[m][4 j + g]).
+// sweep's A operand: 32-row tile, wave (16-row half + 2 x K half), then per
+// lane 4 consecutive K steps as one float4 (MFMA 16x16x4 f32: lane m + 16 g
+// holds A[m][4 j + g]).
 __host__ __device__ inline i64 sweep_frag(int a, int c) {
-    const int rt = a >> 6, w = (a >> 4) & 3, m = a & 15;
-    const int j = c >> 2, g = c & 3;
-    return ((((i64)(rt * 4 + w) * (kSwNB / 16) + (j >> 2)) * 64 + (m + 16 * g)) * 4 + (j & 3));
+    const int rt = a >> 5, w = ((a >> 4) & 1) + 2 * (c / kSwKH), m = a & 15;
+    const int j = (c % kSwKH) >> 2, g = c & 3;
+    return ((((i64)(rt * 4 + w) * (kSwKH / 16) + (j >> 2)) * 64 + (m + 16 * g)) * 4 + (j & 3));
 }
 
 // acc[nt] += sum_k sA[16 w + m][k] sB[k][16 nt + m], k = 0 .. 63 (one wave's
@@ -264,30 +269,44 @@ __global__ __launch_bounds__(256) void sweep_tiles_kernel(int n, int b0, int nbk
 // ------------------------------------------------------------------- sweep
 // rhs: forward, column-major n x kc (ld ldr); backward, the forward sweep's
 // row-major output.  Y: row-major output (T*NB x KP), Z: row-major scratch.
-// out (backward only): column-major destination of the solution.
+// out (backward only): column-major destination of the solution.  NT
+// 16-column tiles of right-hand sides (kc <= 16 NT).
 template <int NT, bool BWD>
-__global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* __restrict__ S, i64 lds,
+__global__ __launch_bounds__(256, 2) void sweep_kernel(int n, int kc, const float* __restrict__ S, i64 lds,
                                                     const float* __restrict__ tiles, const float* __restrict__ rhs,
                                                     i64 ldr, float* __restrict__ Y, float* __restrict__ Z,
                                                     float* __restrict__ out, i64 ldo, int* __restrict__ ctrl,
                                                     int* __restrict__ err) {
-    __shared__ float sV[2][kSwH][kSwVP];
+    // A block vector (NB rows x 16 NT columns) is staged in LDS whole, one
+    // coherent round trip per product, straight from memory (global_load_lds,
+    // no registers).  Row r, column c sits at r VP + (c ^ 16 ((r >> 1) & 1))
+    // for NT = 2 (the 16-column halves swapped on every other row pair), so
+    // the B-operand reads of a wave (rows 4j + g, columns 16 nt + m) fall in
+    // 64 distinct banks.  35 or 70 KB of LDS and at most 256 registers per
+    // lane: two workgroups share a CU, so every workgroup of a sweep of up
+    // to 16384 rows is resident from the start.
+    constexpr int VP = 16 * NT;
+    constexpr int kVecC = kSwNB * VP / 4 / 256;  // 16-byte pieces per thread for one block vector
+    constexpr int kJ = kSwKH / 4;                // K steps of a wave per product
+    __shared__ float sV[1][kSwNB * VP];
+    __shared__ float sRed[2][16][VP + 1];        // the upper K half's partial sums
     __shared__ int s_ord;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6, m = lane & 15, g = lane >> 4;
+    const int wr = w & 1, kh = w >> 1;  // 16-row half, K half
     const int T = (int)ceil_div(n, kSwNB);
     if (t == 0) s_ord = atomicAdd(&ctrl[0], 1);
     __syncthreads();
     const int ord = s_ord;
-    const int s = ord / kSwR, rt = ord - s * kSwR;
-    int* yflag = ctrl + 1;              // [step][row tile]: solved rows published
-    int* zflag = ctrl + 1 + T * kSwR;   // [step][row tile]: z rows published
+    const int s = ord / kSwRT, rt = ord - s * kSwRT;
+    int* yflag = ctrl + 1;                // [step][row tile]: solved rows published
+    int* zflag = ctrl + 1 + T * kSwRT;    // [step][row tile]: z rows published
     auto blk = [&](int st) { return BWD ? T - 1 - st : st; };
     const int b = blk(s);
-    const i64 row0 = (i64)b * kSwNB + 64 * rt;
-    const i64 arow = row0 + 16 * w + m < n ? row0 + 16 * w + m : (i64)n - 1;  // A-operand row (clamped in bounds)
-    const i64 orow = row0 + 16 * w + 4 * g;               // output rows orow + v of this lane
+    const i64 row0 = (i64)b * kSwNB + kSwTR * rt;
+    const i64 arow = row0 + 16 * wr + m < n ? row0 + 16 * wr + m : (i64)n - 1;  // A-operand row (clamped in bounds)
+    const i64 orow = row0 + 16 * wr + 4 * g;  // output rows orow + v of this lane
 
-    // The R row tiles of one step, then the workgroup.  A wait that expires
+    // The row tiles of one step, then the workgroup.  A wait that expires
     // records an error in *err (kept until the caller clears it); every later
     // wait sees it and gives up at once, so a lost flag costs one expiry, not
     // one per wait.
@@ -295,9 +314,9 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
         if (w == 0) {
             long spins = 0;
             for (;;) {
-                const int v = (lane < kSwR) ? coherent_load(&f[lane]) : (lane == kSwR ? coherent_load(err) == 0 : 1);
+                const int v = (lane < kSwRT) ? coherent_load(&f[lane]) : (lane == kSwRT ? coherent_load(err) == 0 : 1);
                 if (__ballot(v == 0) == 0ull) break;                      // all flags up
-                if (__ballot(lane == kSwR && v == 0) != 0ull) break;      // an earlier wait expired
+                if (__ballot(lane == kSwRT && v == 0) != 0ull) break;     // an earlier wait expired
                 if (++spins > kSwSpinLimit) {
                     if (lane == 0) atomicOr(err, 1);
                     break;
@@ -307,40 +326,47 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
         }
         __syncthreads();
     };
-    u64 gv[kSwVecQ];
-    auto vec_load = [&](const float* src, int bb, int h) __attribute__((always_inline)) {  // half h of a published block vector: coherent loads
-        const u64* p = reinterpret_cast<const u64*>(src + ((i64)bb * kSwNB + h * kSwH) * kSwKP);
+    auto vpos = [](int r, int c) { return NT == 2 ? r * VP + (c ^ (((r >> 1) & 1) << 4)) : r * VP + c; };
+    // A published block vector (its first 16 NT columns) into sV: each
+    // wave instruction fills 1 KB of LDS, lane L the 16 bytes at 16 L, with
+    // the row and columns whose place that is; device-coherent (sc1) loads
+    // like coherent_load.  The caller's barrier must come first (no wave
+    // still reads the buffer), then vec_wait().
+    auto vec_fill = [&](const float* src, int bb) __attribute__((always_inline)) {
+        const float* p = src + (i64)bb * kSwNB * kSwKP;
 #pragma unroll
-        for (int q = 0; q < kSwVecQ; ++q) gv[q] = coherent_load(&p[t + 256 * q]);
-    };
-    auto vec_store = [&](int buf) __attribute__((always_inline)) {
-#pragma unroll
-        for (int q = 0; q < kSwVecQ; ++q) {
-            const int e = t + 256 * q, r = e >> 4, cp = e & 15;
-            float2 v;
-            v.x = __uint_as_float((unsigned)gv[q]);
-            v.y = __uint_as_float((unsigned)(gv[q] >> 32));
-            *reinterpret_cast<float2*>(&sV[buf][r][2 * cp]) = v;
+        for (int q = 0; q < kVecC; ++q) {
+            const int chunk = w * kVecC + q;  // wave-uniform
+            const int pos = (chunk * 64 + lane) * 4;
+            const int r = pos / VP, cl = pos % VP;
+            const int c = NT == 2 ? cl ^ (((r >> 1) & 1) << 4) : cl;
+            __builtin_amdgcn_global_load_lds(p + (i64)r * kSwKP + c,
+                                             (__attribute__((address_space(3))) void*)(sV[0] + chunk * 256), 16, 0,
+                                             16 /* sc1 */);
         }
     };
-    float a[kSwNB / 4];  // the A operand of one product: K half h in a[64 h .. 64 h + 63]
-    constexpr int kJH = kSwH / 4;
-    // M(arow, bb*NB + 4j + g): L below the diagonal, its mirror above.  Rows
-    // and columns past n read clamped (finite) entries: they meet zero rows
-    // of the block vectors or produce output rows that are never kept.
-    auto a_from_factor = [&](int bb, int h) __attribute__((always_inline)) {
-        const i64 c0 = (i64)bb * kSwNB + g;
+    auto vec_wait = [&]() __attribute__((always_inline)) {
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    };
+    // The A operand of one product, this wave's rows and K half:
+    // M(arow, bb*NB + KH kh + 4j + g), L below the diagonal, its mirror
+    // above.  Rows and columns past n read clamped (finite) entries: they
+    // meet zero rows of the block vectors or produce output rows that are
+    // never kept.
+    auto a_from_factor = [&](float (&a)[kJ], int bb) __attribute__((always_inline)) {
+        const i64 c0 = (i64)bb * kSwNB + kSwKH * kh + g;
 #pragma unroll
-        for (int j = h * kJH; j < (h + 1) * kJH; ++j) {
+        for (int j = 0; j < kJ; ++j) {
             const i64 col = c0 + 4 * j < n ? c0 + 4 * j : (i64)n - 1;
             a[j] = S[arow + col * lds];
         }
     };
-    auto a_from_tile = [&](int i, int h) __attribute__((always_inline)) {
+    auto a_from_tile = [&](float (&a)[kJ], int i) __attribute__((always_inline)) {
         const float4* p = reinterpret_cast<const float4*>(tiles + ((i64)i * T + s) * kSwNB * kSwNB) +
-                          (i64)(rt * 4 + w) * (kSwNB / 16) * 64 + lane;
+                          (i64)(rt * 4 + w) * (kSwKH / 16) * 64 + lane;
 #pragma unroll
-        for (int jq = h * (kJH / 4); jq < (h + 1) * (kJH / 4); ++jq) {
+        for (int jq = 0; jq < kJ / 4; ++jq) {
             const float4 v = p[(i64)jq * 64];
             a[4 * jq] = v.x;
             a[4 * jq + 1] = v.y;
@@ -348,58 +374,44 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
             a[4 * jq + 3] = v.w;
         }
     };
-    const int nU = s > kSwLook ? s - kSwLook : 0;  // older blocks, folded in before z
-    // Half h of the A operand of a product: from the factor (block column
-    // idx) or a sweep tile (term idx of this step), chosen at compile time so
-    // that every path issues the same loads and the waits stay exact.
-    auto a_half = [&](auto from_factor, int idx, int h) __attribute__((always_inline)) {
-        if constexpr (decltype(from_factor)::value) a_from_factor(idx, h);
-        else a_from_tile(idx, h);
-    };
-    // acc += A-operand K rows h*kSwH .. (h+1)*kSwH x the half staged in
-    // sV[buf].  The MFMA statements are volatile, so the compiler keeps the
-    // LDS reads where the source puts them: they run kD K-steps ahead of
-    // their MFMAs through a ring of registers instead of each MFMA waiting
-    // out the LDS latency of its own operand.
-    auto product = [&](int buf, f4_t (&acc)[NT], int h) __attribute__((always_inline)) {
-        constexpr int kD = 6 / NT, kJ = kSwH / 4;
+    // acc += a x (this wave's K half of the vector in sV[bf]).  The MFMA
+    // statements are volatile, so the compiler keeps the LDS reads where
+    // the source puts them: they run kD K-steps ahead of their MFMAs through
+    // a ring of registers instead of each MFMA waiting out the LDS latency
+    // of its own operand.
+    auto product = [&](const float (&a)[kJ], int bf, f4_t (&acc)[NT]) __attribute__((always_inline)) {
+        constexpr int kD = 6 / NT;
+        const float* v = sV[bf];
+        const int r0 = kSwKH * kh + g;
         float bq[kD + 1][NT];
 #pragma unroll
         for (int j = 0; j < kD; ++j)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) bq[j][nt] = sV[buf][4 * j + g][16 * nt + m];
+            for (int nt = 0; nt < NT; ++nt) bq[j][nt] = v[vpos(r0 + 4 * j, 16 * nt + m)];
 #pragma unroll
         for (int j = 0; j < kJ; ++j) {
             if (j + kD < kJ)
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) bq[(j + kD) % (kD + 1)][nt] = sV[buf][4 * (j + kD) + g][16 * nt + m];
+                for (int nt = 0; nt < NT; ++nt) bq[(j + kD) % (kD + 1)][nt] = v[vpos(r0 + 4 * (j + kD), 16 * nt + m)];
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) mfma_acc(acc[nt], a[h * kJ + j], bq[j % (kD + 1)][nt]);
+            for (int nt = 0; nt < NT; ++nt) mfma_acc(acc[nt], a[j], bq[j % (kD + 1)][nt]);
         }
         mfma_retire();
     };
-    // acc += (A operand cur) x block bb of src, the two halves of the block
-    // vector through the two LDS buffers.  Loads of the A operand run one
-    // half ahead: half 0 of this product was issued while the previous one
-    // multiplied its half 1; half 1 is issued here before the first half
-    // multiplies, and half 0 of the next product (A operand nxt) before the
-    // second.  Every product ends with its MFMAs retired (product()), so
-    // the registers a half frees can be reloaded at once.
-    auto block_product = [&](auto cur_factor, auto nxt_factor, const float* src, int bb, f4_t (&acc)[NT], int cur,
-                             int nxt) __attribute__((always_inline)) {
-        vec_load(src, bb, 0);
-        a_half(cur_factor, cur, 1);
-        vec_store(0);
+    // the upper K half's partial sums into the lower half's waves
+    auto reduce = [&](f4_t (&acc)[NT]) __attribute__((always_inline)) {
+        if (kh == 1)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sRed[wr][4 * g + v][16 * nt + m] = acc[nt][v];
         __syncthreads();
-        vec_load(src, bb, 1);
-        product(0, acc, 0);
-        a_half(nxt_factor, nxt, 0);
-        vec_store(1);
-        __syncthreads();
-        product(1, acc, 1);
+        if (kh == 0)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[nt][v] += sRed[wr][4 * g + v][16 * nt + m];
     };
-    const std::true_type kFactor{};
-    const std::false_type kTile{};
 
     // own right-hand sides, in the accumulator layout (row orow + v, column 16 nt + m)
     float r[NT][4];
@@ -413,51 +425,73 @@ __global__ __launch_bounds__(256) void sweep_kernel(int n, int kc, const float* 
             else r[nt][v] = (row < n && col < kc) ? rhs[row + col * ldr] : 0.0f;
         }
 
-    // 1. older blocks
+    // The products, in order: the older blocks p < nU (u), then X z (p ==
+    // nU) and the look-ahead terms P(i), i = i_top .. 1 (y).
+    const int nU = s > kSwLook ? s - kSwLook : 0;
+    const int i_top = s < kSwLook ? s : kSwLook;
+    const int np = nU + 1 + i_top;
+    auto term = [&](int p) { return p == nU ? 0 : i_top - (p - nU - 1); };
+    auto load_a = [&](float (&a)[kJ], int p) __attribute__((always_inline)) {
+        if (p < nU) a_from_factor(a, blk(p));
+        else a_from_tile(a, term(p));
+    };
     f4_t acc[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
-    if (nU > 0) a_from_factor(blk(0), 0);
-    else a_from_tile(0, 0);
-    for (int sp = 0; sp < nU; ++sp) {
-        wait_all(yflag + sp * kSwR);
-        if (sp + 1 < nU) block_product(kFactor, kFactor, Y, blk(sp), acc, blk(sp), blk(sp + 1));
-        else block_product(kFactor, kTile, Y, blk(sp), acc, blk(sp), 0);
+    // product p with its A operand in a; the next one's loads into nxt
+    auto step = [&](const float (&a)[kJ], float (&nxt)[kJ], int p) __attribute__((always_inline)) {
+        if (p == nU) {
+            // z = r - u (rows past n are zero), published; the step's other
+            // row tiles awaited
+            reduce(acc);
+            if (kh == 0)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const i64 row = orow + v;
+                        coherent_store(&Z[row * kSwKP + 16 * nt + m], row < n ? r[nt][v] - acc[nt][v] : 0.0f);
+                    }
+            publish_flag(&zflag[s * kSwRT + rt], t);
+            // from here on the step's own products: the critical path, ahead
+            // of a workgroup sharing the SIMDs with older blocks to fold in
+            __builtin_amdgcn_s_setprio(3);
+            wait_all(zflag + s * kSwRT);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
+            vec_fill(Z, b);  // wait_all's barrier: the buffer is free
+        } else {
+            const int src_step = p < nU ? p : s - term(p);
+            wait_all(yflag + src_step * kSwRT);
+            vec_fill(Y, blk(src_step));
+        }
+        vec_wait();
+        if (p + 1 < np) load_a(nxt, p + 1);
+        product(a, 0, acc);
+    };
+    float a0[kJ], a1[kJ];
+    load_a(a0, 0);
+    for (int p = 0;;) {
+        step(a0, a1, p);
+        if (++p == np) break;
+        step(a1, a0, p);
+        if (++p == np) break;
     }
 
-    // 2. z = r - u (rows past n are zero), published
+    // y, published
+    reduce(acc);
+    if (kh == 0)
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
+        for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const i64 row = orow + v;
-            coherent_store(&Z[row * kSwKP + 16 * nt + m], row < n ? r[nt][v] - acc[nt][v] : 0.0f);
-        }
-    publish_flag(&zflag[s * kSwR + rt], t);
-
-    // 3. y = X z + sum_i P(i) y_{s-i}: terms i = 0, then d .. 1 while s - i >=
-    // 0 (after the last one, half 0 of term 0 is loaded again: unused)
-    wait_all(zflag + s * kSwR);
-    f4_t y[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) y[nt] = (f4_t){0.0f, 0.0f, 0.0f, 0.0f};
-    const int i_top = s < kSwLook ? s : kSwLook;
-    block_product(kTile, kTile, Z, b, y, 0, i_top);
-    for (int i = i_top; i >= 1; --i) {
-        wait_all(yflag + (s - i) * kSwR);
-        block_product(kTile, kTile, Y, blk(s - i), y, i, i - 1);
-    }
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const i64 row = orow + v;
-            const int col = 16 * nt + m;
-            const float yv = row < n ? y[nt][v] : 0.0f;
-            coherent_store(&Y[row * kSwKP + col], yv);
-            if (BWD && row < n && col < kc) out[row + col * ldo] = yv;
-        }
-    publish_flag(&yflag[s * kSwR + rt], t);
+            for (int v = 0; v < 4; ++v) {
+                const i64 row = orow + v;
+                const int col = 16 * nt + m;
+                const float yv = row < n ? acc[nt][v] : 0.0f;
+                coherent_store(&Y[row * kSwKP + col], yv);
+                if (BWD && row < n && col < kc) out[row + col * ldo] = yv;
+            }
+    publish_flag(&yflag[s * kSwRT + rt], t);
 }
 
 // ------------------------------------------------------------------ host
@@ -469,7 +503,7 @@ size_t sweep_xinv_count(int n) { return (size_t)ceil_div(n, kSwNB) * kSwNB * kSw
 
 // Work layout: Yf, Yb, Z (rows x KP floats each), then the error word
 // (64 ints reserved), then the forward and backward tickets and flags.
-static size_t sweep_ctrl_ints(int n) { return 1 + 2 * (size_t)ceil_div(n, kSwNB) * kSwR; }
+static size_t sweep_ctrl_ints(int n) { return 1 + 2 * (size_t)ceil_div(n, kSwNB) * kSwRT; }
 static size_t sweep_vec_bytes(int n) {
     const size_t rows = (size_t)ceil_div(n, kSwNB) * kSwNB;
     return (3 * rows * kSwKP * sizeof(float) + 255) & ~(size_t)255;
@@ -518,9 +552,9 @@ static void launch_pair(int n, int kc, const float* S, i64 lds, const float* til
                         float* Yb, float* Z, int* ctrl, int* err, hipStream_t s) {
     const unsigned T = (unsigned)ceil_div(n, kSwNB);
     const size_t half = (size_t)(kSwLook + 1) * T * kSwNB * kSwNB;
-    hipLaunchKernelGGL((sweep_kernel<NT, false>), dim3(T * kSwR), dim3(256), 0, s, n, kc, S, lds, tiles, B, ldb, Yf,
+    hipLaunchKernelGGL((sweep_kernel<NT, false>), dim3(T * kSwRT), dim3(256), 0, s, n, kc, S, lds, tiles, B, ldb, Yf,
                        Z, nullptr, 0, ctrl, err);
-    hipLaunchKernelGGL((sweep_kernel<NT, true>), dim3(T * kSwR), dim3(256), 0, s, n, kc, S, lds, tiles + half, Yf,
+    hipLaunchKernelGGL((sweep_kernel<NT, true>), dim3(T * kSwRT), dim3(256), 0, s, n, kc, S, lds, tiles + half, Yf,
                        (i64)kSwKP, Yb, Z, B, ldb, ctrl + sweep_ctrl_ints(n), err);
 }
 
