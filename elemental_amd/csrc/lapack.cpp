@@ -666,7 +666,7 @@ MixedFactor carve_factor(void* mem, int n) {
     f.Xp = c.take<float>(sweep_xinv_count(n));
     f.tiles = c.take<float>(sweep_tiles_count(n));
     f.dinfo = c.take<int>(4);
-    f.scratch = c.take<double>((size_t)n + 8 + symm_part_count(n, 1));
+    f.scratch = c.take<double>((size_t)n + 8);  // row sums of |A|, their maximum
     return f;
 }
 
@@ -677,7 +677,7 @@ size_t mixed_factor_bytes(int n) {
     c.take<float>(sweep_xinv_count(n));
     c.take<float>(sweep_tiles_count(n));
     c.take<int>(4);
-    c.take<double>((size_t)n + 8 + symm_part_count(n, 1));
+    c.take<double>((size_t)n + 8);
     return c.off + 256;
 }
 
@@ -730,13 +730,16 @@ int mixed_factor(int n, const double* A, i64 lda, const MixedOpts& o, MixedFacto
     if (sp == nullptr) return kErrHip;
     // ||A||_inf (only the refinement's backward errors need it) on aux3
     // beside the factorization's first steps, into pinned memory behind
-    // ev_c; aux3 starts after the caller's work before this call
+    // ev_c; aux3 starts after the caller's work before this call.  Unsplit
+    // (one workgroup per 64-row block, about one per CU): it is needed only
+    // at the end of the factorization, and this way every CU keeps room for
+    // the chain's first diagonal factor and panel, which otherwise waited
+    // for the split form's three workgroups per CU to leave
     double* rowsum = f.scratch;
     double* dtmp = f.scratch + n;
     ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_c, s));
     ELEMAMD_HIP_TRY(hipStreamWaitEvent(sp->aux3, sp->ev_c, 0));
-    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, symm_part_count(n, 1) ? f.scratch + n + 8 : nullptr,
-                                                sp->aux3));
+    ELEMAMD_HIP_TRY(launch_sym_lower_abs_rowsum(n, A, lda, rowsum, nullptr, sp->aux3));
     ELEMAMD_HIP_TRY(launch_vec_max(n, rowsum, dtmp, sp->aux3));
     ELEMAMD_HIP_TRY(hipMemcpyAsync(sp->pinned, dtmp, sizeof(double), hipMemcpyDeviceToHost, sp->aux3));
     ELEMAMD_HIP_TRY(hipEventRecord(sp->ev_c, sp->aux3));
