@@ -24,8 +24,12 @@ Other workloads (each prints its own single JSON line):
                  dominant kernel, both precisions) and a 1 GiB device copy; the
                  command profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
                  for the HBM traffic figure (the copy calibrates the counters)
+  --mode counters : one warm-up and one measured mixed solve at n, k on a matrix
+                 built from elementwise operations only (symmetric random plus a
+                 diagonal shift, SPD), so that a counter run profiles no library
+                 GEMM: the command of the refinement kernels' traffic summary
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--mode single|batch|sweep|serve|roofline|inverse]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--mode single|batch|sweep|serve|roofline|inverse|counters]
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -63,7 +67,8 @@ def parse():
     p.add_argument("--seed", type=int, default=2024)
     p.add_argument("--cpu-baseline", type=int, default=1, help="0 skips the CPU baseline")
     p.add_argument("--cpu-runs", type=int, default=5)
-    p.add_argument("--mode", default="single", choices=["single", "batch", "sweep", "serve", "roofline", "inverse"])
+    p.add_argument("--mode", default="single",
+                   choices=["single", "batch", "sweep", "serve", "roofline", "inverse", "counters"])
     p.add_argument("--count", type=int, default=8192, help="batch: number of matrices")
     p.add_argument("--requests", type=int, default=160, help="serve: requests per arrival rate")
     return p.parse_args()
@@ -620,6 +625,47 @@ def bench_serve(args):
     return out
 
 
+def bench_counters(args):
+    """One warm-up and one measured mixed solve (split-BF16x3 factor, CG) at
+    n x k on A = (R + R^T)/2 + 2.2 sqrt(n) I, R iid N(0, 1): its spectrum lies
+    in about [2.2 - 1.42, 2.2 + 1.42] sqrt(n), so A is SPD with a condition
+    number near 4.6.  Only elementwise torch operations build it, so a
+    rocprofv3 counter run sees the library's kernels and no vendor GEMM."""
+    import torch
+
+    import elemental_amd as E
+
+    n, k = args.n, args.k
+    g = torch.Generator(device="cuda")
+    g.manual_seed(args.seed)
+    A, B = E.Matrix(n, n), E.Matrix(n, k)
+    R = torch.randn(n, n, generator=g, device="cuda", dtype=torch.float64)
+    At = A.tensor()
+    At.copy_(R)
+    At.add_(R.t())
+    At.mul_(0.5)
+    del R
+    At.diagonal().add_(2.2 * n ** 0.5)
+    B.tensor().copy_(torch.randn(n, k, generator=g, device="cuda", dtype=torch.float64))
+    opts = E.IROptions(nb=args.nb, trailing=args.trailing, refine=args.refine)
+    runs = []
+    for _ in range(2):
+        X = E.Matrix(n, k)
+        X.tensor().copy_(B.tensor())
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        rep = E.HPDSolveMixed(A, X, opts)
+        torch.cuda.synchronize()
+        runs.append({"ms": (time.perf_counter() - t) * 1e3, "iterations": rep.iterations,
+                     "backward_error": rep.backward_error, "converged": rep.converged})
+    return {"metric": "mixed solve for counter runs (elementwise-built SPD matrix)", "value": runs[-1]["ms"],
+            "unit": "ms", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": runs[-1]["ms"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"mixed solve n={n} k={k}, A = (R + R^T)/2 + 2.2 sqrt(n) I", "n": n, "k": k,
+                       "trailing": args.trailing, "refine": args.refine, "parallelism": "single GPU"},
+            "runs": runs}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -628,7 +674,7 @@ def main():
         sys.exit(2)
     if args.mode != "single":
         out = {"batch": bench_batch, "sweep": bench_sweep, "serve": bench_serve,
-               "roofline": bench_roofline, "inverse": bench_inverse}[args.mode](args)
+               "roofline": bench_roofline, "inverse": bench_inverse, "counters": bench_counters}[args.mode](args)
         print(json.dumps(out))
         return
     import torch
