@@ -299,7 +299,10 @@ __global__ __launch_bounds__(256, 3) void symm_lower_kernel(int n, int k, double
                                                          const double* __restrict__ B, i64 ldb, double* __restrict__ R,
                                                          i64 ldr, int nsplit, double* __restrict__ part) {
     __shared__ double sM[kSymT][kSymLd];     // sM[q][p] = Asym(p0+p, q0+q)
-    __shared__ double sX[kSymT][kSymCols + 1];  // sX[q][c] = X(q0+q, c_base+c)
+    // sX[q][c] = X(q0+q, c_base+c); the ABS form multiplies by a column of
+    // ones formed in registers and keeps only the tile in LDS (33 KB), which
+    // leaves room for the factorization's chain kernels beside it
+    __shared__ double sX[ABS ? 1 : kSymT][kSymCols + 1];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int I = blockIdx.x;
     const i64 p0 = (i64)I * kSymT;
@@ -338,12 +341,13 @@ __global__ __launch_bounds__(256, 3) void symm_lower_kernel(int n, int k, double
                 gm[s] = ABS ? fabs(v) : v;
             }
         }
+        if constexpr (!ABS) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int c = w + 4 * s;
-            const i64 q = q0 + lane;
-            gx[s] = ABS ? ((c == 0 && q < n) ? 1.0 : 0.0)
-                        : ((q < n && c < kcl) ? X[q + (i64)(c_base + c) * ldx] : 0.0);
+            for (int s = 0; s < 8; ++s) {
+                const int c = w + 4 * s;
+                const i64 q = q0 + lane;
+                gx[s] = (q < n && c < kcl) ? X[q + (i64)(c_base + c) * ldx] : 0.0;
+            }
         }
     };
     auto sstore = [&](int J) {
@@ -363,8 +367,10 @@ __global__ __launch_bounds__(256, 3) void symm_lower_kernel(int n, int k, double
                 }
             }
         }
+        if constexpr (!ABS) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s) sX[lane][w + 4 * s] = gx[s];
+            for (int s = 0; s < 8; ++s) sX[lane][w + 4 * s] = gx[s];
+        }
     };
 
     d4_t acc[2];
@@ -382,10 +388,16 @@ __global__ __launch_bounds__(256, 3) void symm_lower_kernel(int n, int k, double
         for (int ks = 0; ks < kSymT / 4; ++ks) {
             const int q = 4 * ks + (lane >> 4);
             const double bfrag = sM[q][16 * w + (lane & 15)];
-            const double a0 = sX[q][(lane & 15)];
-            const double a1 = sX[q][16 + (lane & 15)];
-            acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag, acc[0], 0, 0, 0);
-            acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag, acc[1], 0, 0, 0);
+            if constexpr (ABS) {
+                // X = ones in column 0 (rows past n meet zero tile entries)
+                const double a0 = (lane & 15) == 0 ? 1.0 : 0.0;
+                acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag, acc[0], 0, 0, 0);
+            } else {
+                const double a0 = sX[q][(lane & 15)];
+                const double a1 = sX[q][16 + (lane & 15)];
+                acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag, acc[1], 0, 0, 0);
+            }
         }
         __syncthreads();  // every wave is done with this tile
         sstore((J + 1) < Je ? J + 1 : J);
